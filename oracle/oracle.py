@@ -1,0 +1,196 @@
+"""ctypes wrapper of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by bench.py's
+cpu_baseline leg, always as the checker or the timed CPU baseline, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, '_build', 'liboracle.so')
+_lib = None
+
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags='C_CONTIGUOUS')
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags='C_CONTIGUOUS')
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags='C_CONTIGUOUS')
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags='C_CONTIGUOUS')
+
+
+def build():
+    subprocess.check_call(['make', '-s', '-C', _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_sincos.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.orc_fk.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, _f64p]
+        L.orc_xform_pt.argtypes = [_f64p, _f64p, _f64p]
+        L.orc_inv_xform_pt.argtypes = [_f64p, _f64p, _f64p]
+        L.orc_seg_tri.argtypes = [_f64p] * 5
+        L.orc_seg_tri.restype = C.c_int
+        L.orc_seg_tri_q.argtypes = [_f64p] * 6
+        L.orc_seg_tri_q.restype = C.c_int
+        L.orc_tube_decide.argtypes = [C.c_int, _f64p, C.c_double, _f64p, _f64p]
+        L.orc_tube_decide.restype = C.c_int
+        L.orc_mesh_create.argtypes = [_f32p, C.c_int, _i32p, C.c_int]
+        L.orc_mesh_create.restype = C.c_void_p
+        L.orc_mesh_destroy.argtypes = [C.c_void_p]
+        L.orc_mesh_num_edges.argtypes = [C.c_void_p]
+        L.orc_mesh_num_edges.restype = C.c_int
+        L.orc_mesh_edges.argtypes = [C.c_void_p, _i32p]
+        L.orc_robot_create.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p, _i32p, _u32p, _f32p,
+                                       _i32p, C.c_int]
+        L.orc_robot_create.restype = C.c_void_p
+        L.orc_robot_destroy.argtypes = [C.c_void_p]
+        L.orc_check_discrete.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _f32p, C.c_int, _u32p, C.c_int]
+        L.orc_check_discrete_brute.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _f32p, C.c_int, _u32p]
+        L.orc_sphere_centers.argtypes = [C.c_void_p, _f32p, _f64p]
+        L.orc_check_linear.argtypes = [C.c_void_p, C.c_void_p, _f32p, _u32p, C.c_int, _u32p, C.c_void_p, C.c_int]
+        L.orc_check_quadratic.argtypes = [C.c_void_p, C.c_void_p, _f32p, C.c_int, C.c_int, C.c_int, _f64p, _u32p,
+                                          C.c_void_p, C.c_int]
+        L.orc_fit_ctrl.argtypes = [_f64p, C.c_int, C.c_int, _f64p, _f64p]
+        _lib = L
+    return _lib
+
+
+def _d3(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(3))
+
+
+def sincos(x):
+    s, c = C.c_double(), C.c_double()
+    lib().orc_sincos(float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def fk(robot, q):
+    """Link transforms [n_links, 12] (float64) of one pose q (float32 [dof])."""
+    T = np.zeros((robot.n_links, 12), dtype=np.float64)
+    lib().orc_fk(robot.dof, robot.base, robot.cos_mask, robot.sin_mask, robot.one_mask,
+                 np.ascontiguousarray(q, dtype=np.float32), T)
+    return T
+
+
+def xform(T, p):
+    o = np.zeros(3)
+    lib().orc_xform_pt(np.ascontiguousarray(T, dtype=np.float64), _d3(p), o)
+    return o
+
+
+def inv_xform(T, p):
+    o = np.zeros(3)
+    lib().orc_inv_xform_pt(np.ascontiguousarray(T, dtype=np.float64), _d3(p), o)
+    return o
+
+
+def seg_tri(P, Q, A, B, Cc):
+    return lib().orc_seg_tri(_d3(P), _d3(Q), _d3(A), _d3(B), _d3(Cc))
+
+
+def seg_tri_q(P, Q, A, B, Cc):
+    q = np.zeros(4)
+    h = lib().orc_seg_tri_q(_d3(P), _d3(Q), _d3(A), _d3(B), _d3(Cc), q)
+    return h, q
+
+
+def tube_decide(ctrl, r, A, B):
+    ctrl = np.ascontiguousarray(np.asarray(ctrl, dtype=np.float64).reshape(-1))
+    deg = len(ctrl) // 3 - 1
+    return lib().orc_tube_decide(deg, ctrl, float(r), _d3(A), _d3(B))
+
+
+def fit_ctrl(W, X):
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    P = np.zeros((W.shape[0], 3))
+    lib().orc_fit_ctrl(W, W.shape[0], W.shape[1], X, P)
+    return P
+
+
+class Mesh:
+    def __init__(self, verts, tris):
+        self.verts = np.ascontiguousarray(verts, dtype=np.float32)
+        self.tris = np.ascontiguousarray(tris, dtype=np.int32)
+        self.h = lib().orc_mesh_create(self.verts, len(self.verts), self.tris, len(self.tris))
+
+    def edges(self):
+        ne = lib().orc_mesh_num_edges(self.h)
+        out = np.zeros((ne, 2), dtype=np.int32)
+        lib().orc_mesh_edges(self.h, out)
+        return out
+
+    def __del__(self):
+        if getattr(self, 'h', None) and _lib is not None:
+            _lib.orc_mesh_destroy(self.h)
+            self.h = None
+
+
+class Robot:
+    def __init__(self, desc):
+        self.desc = desc
+        v, vo, t, to = desc.concat_meshes()
+        self._keep = (v, vo, t, to)
+        self.h = lib().orc_robot_create(desc.dof, desc.base, desc.cos_mask, desc.sin_mask, desc.one_mask,
+                                        v, vo, t, to, desc.spheres, desc.sphere_link, len(desc.spheres))
+
+    def __del__(self):
+        if getattr(self, 'h', None) and _lib is not None:
+            _lib.orc_robot_destroy(self.h)
+            self.h = None
+
+
+def scene_mesh(scene):
+    return Mesh(scene.verts, scene.tris)
+
+
+def check_discrete(robot, mesh, poses, direction=3, skip_base=True, nthreads=None):
+    poses = np.ascontiguousarray(poses, dtype=np.float32)
+    out = np.zeros(len(poses), dtype=np.uint32)
+    nt = nthreads or os.cpu_count() or 1
+    lib().orc_check_discrete(robot.h, mesh.h, int(direction), int(bool(skip_base)), poses, len(poses), out, nt)
+    return out
+
+
+def check_discrete_brute(robot, mesh, poses, direction=3, skip_base=True):
+    poses = np.ascontiguousarray(poses, dtype=np.float32)
+    out = np.zeros(len(poses), dtype=np.uint32)
+    lib().orc_check_discrete_brute(robot.h, mesh.h, int(direction), int(bool(skip_base)), poses, len(poses), out)
+    return out
+
+
+def sphere_centers(robot, q):
+    out = np.zeros((len(robot.desc.spheres), 3))
+    lib().orc_sphere_centers(robot.h, np.ascontiguousarray(q, dtype=np.float32), out)
+    return out
+
+
+def check_linear(robot, mesh, traj, offsets, with_segments=False, nthreads=None):
+    traj = np.ascontiguousarray(traj, dtype=np.float32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+    n = len(offsets) - 1
+    out = np.zeros(n, dtype=np.uint32)
+    seg = np.zeros(max(int(offsets[-1]) - n, 1), dtype=np.uint32) if with_segments else None
+    nt = nthreads or os.cpu_count() or 1
+    lib().orc_check_linear(robot.h, mesh.h, traj, offsets, n, out,
+                           seg.ctypes.data_as(C.c_void_p) if seg is not None else None, nt)
+    return (out, seg[:int(offsets[-1]) - n]) if with_segments else out
+
+
+def check_quadratic(robot, mesh, wp, W, with_segments=False, nthreads=None):
+    wp = np.ascontiguousarray(wp, dtype=np.float32)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    n, nwp = wp.shape[0], wp.shape[1]
+    nc = W.shape[0]
+    out = np.zeros(n, dtype=np.uint32)
+    seg = np.zeros((n, nc - 2), dtype=np.uint32) if with_segments else None
+    nt = nthreads or os.cpu_count() or 1
+    lib().orc_check_quadratic(robot.h, mesh.h, wp, n, nwp, nc, W, out,
+                              seg.ctypes.data_as(C.c_void_p) if seg is not None else None, nt)
+    return (out, seg) if with_segments else out

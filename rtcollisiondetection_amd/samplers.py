@@ -1,0 +1,107 @@
+"""Pose and trajectory generators (numpy PCG64, ``numpy.random.default_rng(seed)``).
+
+The reference samples Halton poses and KNN trajectories for its benchmarks
+(RTCD/Planner/sampler/sampler.h:69-82, RTCD/Utils/Test/poseUtils.h:115-141); the workloads here use
+uniform joint samples, bounded random walks and clamped quadratic B-splines instead.  ``halton``
+restates the reference's radical-inverse sampler for callers that want its sequence.
+"""
+import numpy as np
+
+
+def uniform_poses(n, lower, upper, seed=0):
+    """n joint configurations uniform within [lower, upper], float32 [n, dof]."""
+    rng = np.random.default_rng(seed)
+    lower = np.asarray(lower, dtype=np.float64)
+    upper = np.asarray(upper, dtype=np.float64)
+    q = lower + (upper - lower) * rng.random((n, len(lower)))
+    return np.ascontiguousarray(q.astype(np.float32))
+
+
+def halton(n, dof, skip=0):
+    """Halton points in [0,1)^dof with the first dof primes as bases (radical inverse)."""
+    primes = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47][:dof]
+    out = np.zeros((n, dof))
+    for d, b in enumerate(primes):
+        for i in range(n):
+            k = i + 1 + skip
+            f, r = 1.0, 0.0
+            while k > 0:
+                f /= b
+                r += f * (k % b)
+                k //= b
+            out[i, d] = r
+    return out
+
+
+def random_walk_trajectories(n, lower, upper, lengths=32, max_step=0.1, seed=4):
+    """Random walks in joint space: per step each joint moves U[-max_step, max_step], clipped to the
+    limits.  ``lengths`` is an int (fixed) or a (lo, hi) pair for U{lo..hi} waypoint counts.
+    Returns (waypoints float32 [sum L, dof], offsets uint32 [n+1])."""
+    rng = np.random.default_rng(seed)
+    lower = np.asarray(lower, dtype=np.float64)
+    upper = np.asarray(upper, dtype=np.float64)
+    if isinstance(lengths, (tuple, list)):
+        L = rng.integers(lengths[0], lengths[1] + 1, size=n)
+    else:
+        L = np.full(n, int(lengths))
+    offs = np.zeros(n + 1, dtype=np.uint32)
+    offs[1:] = np.cumsum(L)
+    out = np.empty((int(offs[-1]), len(lower)))
+    for i in range(n):
+        q = lower + (upper - lower) * rng.random(len(lower))
+        steps = rng.uniform(-max_step, max_step, size=(L[i] - 1, len(lower)))
+        traj = [q]
+        for s in steps:
+            q = np.clip(q + s, lower, upper)
+            traj.append(q)
+        out[offs[i]:offs[i + 1]] = traj
+    return np.ascontiguousarray(out.astype(np.float32)), offs
+
+
+def clamped_quadratic_bspline(ctrl, n_samples):
+    """Evaluate a clamped uniform quadratic B-spline (knots 0,0,0,1,..,m,m,m with m = nC-2) at
+    n_samples parameters uniform on [0, m].  ctrl [nC, dim] -> [n_samples, dim]."""
+    ctrl = np.asarray(ctrl, dtype=np.float64)
+    nC = len(ctrl)
+    m = nC - 2
+    knots = np.concatenate([[0, 0], np.arange(0, m + 1), [m, m]]).astype(np.float64)
+    us = np.linspace(0.0, m, n_samples)
+    out = []
+    for u in us:
+        k = min(int(np.floor(u)), m - 1) + 2  # knot span: knots[k] <= u < knots[k+1]
+        d = [ctrl[j] for j in range(k - 2, k + 1)]
+        for r in range(1, 3):  # de Boor
+            for j in range(2, r - 1, -1):
+                i = j + k - 2
+                a = (u - knots[i]) / (knots[i + 3 - r] - knots[i])
+                d[j] = (1 - a) * d[j - 1] + a * d[j]
+        out.append(d[2])
+    return np.asarray(out)
+
+
+def bspline_trajectories(n, lower, upper, n_ctrl=8, n_wp=16, seed=3):
+    """n clamped quadratic B-splines with n_ctrl control points uniform within the limits, sampled at
+    n_wp waypoints.  Returns (waypoints float32 [n, n_wp, dof], control points float32 [n, n_ctrl, dof])."""
+    rng = np.random.default_rng(seed)
+    lower = np.asarray(lower, dtype=np.float64)
+    upper = np.asarray(upper, dtype=np.float64)
+    ctrl = lower + (upper - lower) * rng.random((n, n_ctrl, len(lower)))
+    wp = np.stack([clamped_quadratic_bspline(c, n_wp) for c in ctrl])
+    return np.ascontiguousarray(wp.astype(np.float32)), np.ascontiguousarray(ctrl.astype(np.float32))
+
+
+def quadratic_fit_weights(n_ctrl, n_wp):
+    """Least-squares fit W = pinv(B) (float64 [n_ctrl, n_wp]) of a uniform quadratic B-spline with
+    n_ctrl control points to n_wp samples at u = linspace(0, n_ctrl - 2, n_wp).  The reference ships
+    these matrices as RTCD/data/weights/quad/normal/{nC}_{nT}.txt (loaded at
+    RTCD/Robot/batchCurveRobot.h:560-605)."""
+    nseg = n_ctrl - 2
+    u = np.linspace(0.0, nseg, n_wp)
+    B = np.zeros((n_wp, n_ctrl))
+    for i, uu in enumerate(u):
+        s = min(int(np.floor(uu)), nseg - 1)
+        t = uu - s
+        B[i, s] = (1 - t) ** 2 / 2
+        B[i, s + 1] = (-2 * t * t + 2 * t + 1) / 2
+        B[i, s + 2] = t * t / 2
+    return np.ascontiguousarray(np.linalg.pinv(B))
