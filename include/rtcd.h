@@ -1,0 +1,135 @@
+/*
+ * librtcd_mi355 — C ABI of the MI355X collision-checking path.
+ *
+ * Drop-in for the hot path of RTCD's CollisionDetector (RTCD/CollisionDetector/CollisionDetector.h:18-156):
+ * discrete obstacle->robot and robot->obstacle edge-ray checks and the linear / quadratic B-spline
+ * swept-sphere checks.  Plain pointers and sizes only; every call returns an rtcd_status and never
+ * throws; rtcd_last_error() gives the message of the last failure on the calling thread.
+ *
+ * Result contract (RTCD returns a uint hit counter per query, collision <=> count > 0,
+ * CollisionDetector.h:135, 464-466): out[i] is 1 for a collision and 0 otherwise.
+ */
+#ifndef RTCD_H
+#define RTCD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t rtcd_status;
+enum {
+    RTCD_OK = 0,
+    RTCD_ERR_INVALID_ARG = 1,  /* null pointer, size out of range, NaN/inf input, bad mode */
+    RTCD_ERR_NO_DEVICE = 2,    /* no HIP device / device index out of range */
+    RTCD_ERR_HIP = 3,          /* a HIP runtime call failed */
+    RTCD_ERR_OOM = 4,          /* device allocation failed or the memory budget is exceeded */
+    RTCD_ERR_NOT_READY = 5,    /* robot not loaded / request still running */
+    RTCD_ERR_UNKNOWN_ID = 6,   /* unknown scene or request id */
+    RTCD_ERR_INTERNAL = 7      /* internal limit hit (e.g. BVH deeper than the traversal stack) */
+};
+
+/* Direction of discrete checks (RTCD LinkType MESH = obstacle->robot, RAY_ONLINE = robot->obstacle,
+ * RTCD/Meshes/meshTypes.h:38,45).  TWO_WAY is the union of both. */
+enum { RTCD_O2R = 1, RTCD_R2O = 2, RTCD_TWO_WAY = 3 };
+
+typedef struct rtcd_ctx rtcd_ctx;
+
+typedef struct {
+    int32_t device;          /* HIP device index */
+    uint32_t skip_base_link; /* 1: link 0 (the fixed base) is not checked in discrete modes
+                                (RTCD/Robot/robot.h:17,113-115 zeroes its OBB); default 1 */
+    uint32_t leaf_size;      /* BVH leaf size (1..16), 0 = default (4) */
+    uint64_t mem_budget;     /* bytes of device memory the context may hold for work buffers; 0 = no limit */
+} rtcd_config;
+
+/* Robot: RTCD robotConfig<DOF> (RTCD/Robot/robotConfig.h:14-38) + link meshes + spheres.
+ * T_0 = base; T_{i+1} = T_i * (cos q_i C_i + sin q_i S_i + O_i); all 3x4 row-major fp32. */
+typedef struct {
+    uint32_t dof;                 /* n_links = dof + 1 */
+    const float* base;            /* [12] */
+    const float* cos_mask;        /* [dof][12] */
+    const float* sin_mask;        /* [dof][12] */
+    const float* one_mask;        /* [dof][12] */
+    const float* verts;           /* [V][3] link-frame vertices, links concatenated */
+    const uint32_t* link_vert_offsets; /* [n_links+1] */
+    const int32_t* tris;          /* [F][3] vertex indices local to the link's vertex block */
+    const uint32_t* link_tri_offsets;  /* [n_links+1] */
+    const float* spheres;         /* [S][4] x y z r in the link frame (nullable if n_spheres == 0) */
+    const int32_t* sphere_link;   /* [S] */
+    uint32_t n_spheres;
+} rtcd_robot_desc;
+
+typedef struct {
+    uint64_t node_fetches;   /* BVH nodes visited (counting launches only) */
+    uint64_t prim_tests;     /* primitive tests (segment-triangle, segment-tube) */
+    uint64_t fp64_redecisions; /* tests whose FP32 result fell in the error band */
+    uint64_t candidates;     /* broad-phase (pose, link, triangle) / (segment, edge) candidates */
+    double last_ms;          /* device time of the last check call, ms */
+    uint32_t max_stack;      /* traversal stack size used by the last call */
+    uint32_t max_depth;      /* deepest BVH involved in the last call */
+    uint64_t edge_tests;     /* of prim_tests: tests that read a 32 B edge record (the rest read 48 B) */
+    double stage_ms[5];      /* discrete: FK, broad count, scan, broad emit, narrow (device time, ms) */
+} rtcd_stats;
+
+/* Pointers the caller passes to rtcd_create. */
+rtcd_status rtcd_create(const rtcd_config* cfg, rtcd_ctx** out);
+void rtcd_destroy(rtcd_ctx* ctx);
+const char* rtcd_last_error(void);
+
+/* Replaces robot<DOF>(robotConfig) + the per-link GAS builds (RTCD/Robot/robot.h:91-175,
+ * RTCD/Meshes/mesh.h:745-821): uploads the robot and builds one LBVH per link over triangles and
+ * one over unique edges. */
+rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* robot);
+
+/* Replaces obstacle + scene::addObstacle + batchScene (RTCD/CollisionScenes/obstacle.h:186-219,
+ * scene.h:65-147): world-frame triangles of all obstacles (indices into verts).  Builds the scene
+ * triangle LBVH and edge LBVH, kept resident until rtcd_scene_destroy. */
+rtcd_status rtcd_scene_create(rtcd_ctx* ctx, const float* verts, uint32_t n_verts, const int32_t* tris,
+                              uint32_t n_tris, uint32_t* scene_id);
+rtcd_status rtcd_scene_destroy(rtcd_ctx* ctx, uint32_t scene_id);
+/* Number of unique edges (RTCD/Meshes/mesh.h:200-216) of a scene. */
+rtcd_status rtcd_scene_info(rtcd_ctx* ctx, uint32_t scene_id, uint32_t* n_tris, uint32_t* n_edges);
+
+/* Discrete check of n poses [n][dof] (host memory), out [n] (host memory), blocking.
+ * Replaces CollisionDetector::setNCheck + check + downloadResult (CollisionDetector.h:354-471). */
+rtcd_status rtcd_check_discrete(rtcd_ctx* ctx, uint32_t scene_id, int32_t direction, const float* poses,
+                                uint32_t n, uint32_t* out);
+/* Same with device pointers, enqueued on `stream` (hipStream_t, may be NULL); returns before the
+ * work completes.  rtcd_sync(ctx, stream) waits and reports deferred errors. */
+rtcd_status rtcd_check_discrete_device(rtcd_ctx* ctx, uint32_t scene_id, int32_t direction, const float* d_poses,
+                                       uint32_t n, uint32_t* d_out, void* stream);
+rtcd_status rtcd_sync(rtcd_ctx* ctx, void* stream);
+
+/* Piecewise-linear sweep (RTCD LINEAR_CURVE, RTCD/Robot/batchCurveRobot.h:346-378): trajectory i
+ * has waypoints offsets[i]..offsets[i+1]-1 of traj [*][dof]; one capsule per sphere per waypoint
+ * segment.  out [n_traj]; out_seg (nullable) [offsets[n_traj] - n_traj], segment k of trajectory
+ * i at offsets[i] - i + k.  Host memory, blocking. */
+rtcd_status rtcd_check_linear(rtcd_ctx* ctx, uint32_t scene_id, const float* traj, const uint32_t* offsets,
+                              uint32_t n_traj, uint32_t* out, uint32_t* out_seg);
+rtcd_status rtcd_check_linear_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_traj,
+                                     const uint32_t* d_offsets, uint32_t n_traj, uint32_t n_waypoints_total,
+                                     uint32_t* d_out, uint32_t* d_out_seg, void* stream);
+
+/* Quadratic B-spline sweep (RTCD QUADRATIC_CURVE, batchCurveRobot.h:379-402, sweptVolume.cu:22-96):
+ * waypoints [n_traj][n_wp][dof]; control points P = W X per sphere path with W [n_ctrl][n_wp]
+ * row-major (NULL: least-squares fit of a uniform quadratic B-spline sampled at
+ * u = linspace(0, n_ctrl-2, n_wp), as RTCD/data/weights/quad/normal).  out [n_traj];
+ * out_seg (nullable) [n_traj][n_ctrl-2].  Host memory, blocking. */
+rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* waypoints, uint32_t n_traj,
+                                 uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg);
+rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_waypoints, uint32_t n_traj,
+                                        uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
+                                        uint32_t* d_out_seg, void* stream);
+/* W = pinv(B) for the uniform quadratic B-spline, [n_ctrl][n_wp] row-major. */
+rtcd_status rtcd_quadratic_weights(uint32_t n_ctrl, uint32_t n_wp, double* W);
+
+/* Counting mode: when enabled, check calls also count node fetches / primitive tests (slower). */
+rtcd_status rtcd_set_counting(rtcd_ctx* ctx, int32_t enable);
+rtcd_status rtcd_get_stats(rtcd_ctx* ctx, rtcd_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

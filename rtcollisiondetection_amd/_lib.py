@@ -1,0 +1,94 @@
+"""ctypes binding of librtcd_mi355.so (include/rtcd.h).
+
+The library is the product: if it is missing or fails to load, every entry point raises; there is
+no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'librtcd_mi355.so')
+
+RTCD_OK = 0
+STATUS_NAMES = {0: 'OK', 1: 'INVALID_ARG', 2: 'NO_DEVICE', 3: 'HIP', 4: 'OOM', 5: 'NOT_READY', 6: 'UNKNOWN_ID',
+                7: 'INTERNAL'}
+O2R, R2O, TWO_WAY = 1, 2, 3
+
+EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 'rtcd_scene_create',
+           'rtcd_scene_destroy', 'rtcd_scene_info', 'rtcd_check_discrete', 'rtcd_check_discrete_device', 'rtcd_sync',
+           'rtcd_check_linear', 'rtcd_check_linear_device', 'rtcd_check_quadratic', 'rtcd_check_quadratic_device',
+           'rtcd_quadratic_weights', 'rtcd_set_counting', 'rtcd_get_stats']
+
+
+class RtcdError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f'rtcd status {status} ({STATUS_NAMES.get(status, "?")}): {msg}')
+        self.status = status
+
+
+class Config(C.Structure):
+    _fields_ = [('device', C.c_int32), ('skip_base_link', C.c_uint32), ('leaf_size', C.c_uint32),
+                ('mem_budget', C.c_uint64)]
+
+
+class RobotDesc(C.Structure):
+    _fields_ = [('dof', C.c_uint32), ('base', C.c_void_p), ('cos_mask', C.c_void_p), ('sin_mask', C.c_void_p),
+                ('one_mask', C.c_void_p), ('verts', C.c_void_p), ('link_vert_offsets', C.c_void_p),
+                ('tris', C.c_void_p), ('link_tri_offsets', C.c_void_p), ('spheres', C.c_void_p),
+                ('sphere_link', C.c_void_p), ('n_spheres', C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [('node_fetches', C.c_uint64), ('prim_tests', C.c_uint64), ('fp64_redecisions', C.c_uint64),
+                ('candidates', C.c_uint64), ('last_ms', C.c_double), ('max_stack', C.c_uint32),
+                ('max_depth', C.c_uint32), ('edge_tests', C.c_uint64), ('stage_ms', C.c_double * 5)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f'{LIB_PATH} is missing: build it first (python -m rtcollisiondetection_amd.build)')
+        L = C.CDLL(LIB_PATH)
+        vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
+        L.rtcd_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.rtcd_destroy.argtypes = [vp]
+        L.rtcd_destroy.restype = None
+        L.rtcd_last_error.restype = C.c_char_p
+        L.rtcd_robot_load.argtypes = [vp, C.POINTER(RobotDesc)]
+        L.rtcd_scene_create.argtypes = [vp, vp, u32, vp, u32, C.POINTER(u32)]
+        L.rtcd_scene_destroy.argtypes = [vp, u32]
+        L.rtcd_scene_info.argtypes = [vp, u32, C.POINTER(u32), C.POINTER(u32)]
+        L.rtcd_check_discrete.argtypes = [vp, u32, i32, vp, u32, vp]
+        L.rtcd_check_discrete_device.argtypes = [vp, u32, i32, vp, u32, vp, vp]
+        L.rtcd_sync.argtypes = [vp, vp]
+        L.rtcd_check_linear.argtypes = [vp, u32, vp, vp, u32, vp, vp]
+        L.rtcd_check_linear_device.argtypes = [vp, u32, vp, vp, u32, u32, vp, vp, vp]
+        L.rtcd_check_quadratic.argtypes = [vp, u32, vp, u32, u32, u32, vp, vp, vp]
+        L.rtcd_check_quadratic_device.argtypes = [vp, u32, vp, u32, u32, u32, vp, vp, vp, vp]
+        L.rtcd_quadratic_weights.argtypes = [u32, u32, vp]
+        L.rtcd_set_counting.argtypes = [vp, i32]
+        L.rtcd_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        for name in EXPORTS:
+            if name not in ('rtcd_destroy', 'rtcd_last_error'):
+                getattr(L, name).restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != RTCD_OK:
+        raise RtcdError(status, lib().rtcd_last_error().decode(errors='replace'))
+
+
+def ptr(a):
+    """Address of a C-contiguous numpy array (or None)."""
+    if a is None:
+        return None
+    assert a.flags['C_CONTIGUOUS']
+    return a.ctypes.data_as(C.c_void_p)

@@ -1,0 +1,466 @@
+// C ABI of librtcd_mi355 (include/rtcd.h): context, robot and scene residency, orchestration of the
+// discrete and swept checks, stats.
+// Orchestration replaces CollisionDetector::check/detect/detectWithOBB (RTCD/CollisionDetector/
+// CollisionDetector.h:422-471, 874-983): one enqueue of FK -> broad -> scan -> broad -> narrow per
+// call, no per-batch acceleration-structure rebuild, no host synchronisation inside the pipeline.
+#include "ctx.hpp"
+
+using namespace rtcd;
+
+namespace rtcd {
+thread_local std::string g_err;
+}  // namespace rtcd
+
+namespace {
+
+int stack_for(int depth) { return depth + 2 <= 32 ? 32 : 64; }
+
+// Enqueue the discrete pipeline on stream s.
+void enqueue_discrete(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, uint32_t n, uint32_t* d_out,
+                      hipStream_t s) {
+    const int NL = c->robot.n_links;
+    const int l0 = c->cfg.skip_base_link ? 1 : 0;
+    const int n_active = NL - l0;
+    const size_t tasks = (size_t) n_active * n;
+    const size_t need_cap = std::max<size_t>((size_t) 1 << 22, tasks * 64);
+    if (c->cand_cap < need_cap && c->cand.n < need_cap) {
+        check_budget(c, need_cap * sizeof(uint2));
+        c->cand.reserve(need_cap);
+    }
+    c->cand_cap = (uint32_t) std::min<size_t>(c->cand.n, 0xffffffffu);
+    size_t npl = (size_t) n * NL;
+    check_budget(c, 0);
+    c->T64.reserve(npl * 12);
+    c->Ti.reserve(npl * 3);
+    c->wbox.reserve(npl * 2);
+    c->counts.reserve(tasks + 1);
+    c->offsets.reserve(tasks + 1);
+    c->scan_tmp.reserve(scan_tmp_words(tasks + 1));
+    c->scalars.reserve(4);
+    c->stats.reserve(1);
+
+    fill_u32(d_out, n, 0u, s);
+    fill_u32(c->scalars.p, 4, 0u, s);
+    if (c->counting) fill_u32(reinterpret_cast<uint32_t*>(c->stats.p), sizeof(Stats) / 4, 0u, s);
+    if (!c->ev0) {
+        RTCD_HIP(hipEventCreate(&c->ev0));
+        RTCD_HIP(hipEventCreate(&c->ev1));
+    }
+    if (!c->stage_ev[0])
+        for (auto& e : c->stage_ev) RTCD_HIP(hipEventCreate(&e));
+    RTCD_HIP(hipEventRecord(c->ev0, s));
+    launch_fk(c->d_robot.p, c->robot.dof, NL, d_poses, (int) n, c->T64.p, c->Ti.p, c->wbox.p, s);
+    DiscreteArgs a{};
+    a.robot = c->d_robot.p;
+    a.rb = RobotBvh{c->r_tri_nodes.p, c->r_tris.p, c->r_edge_nodes.p, c->r_edges.p};
+    a.sc = sc.dev();
+    a.n_poses = (int) n;
+    a.n_links = NL;
+    a.l0 = l0;
+    a.n_active = n_active;
+    a.dir = dir;
+    a.T64 = c->T64.p;
+    a.Ti = c->Ti.p;
+    a.wbox = c->wbox.p;
+    a.counts = c->counts.p;
+    a.offsets = c->offsets.p;
+    a.cand = c->cand.p;
+    a.cap = c->cand_cap;
+    a.total = c->scalars.p;
+    a.overflow = c->scalars.p + 1;
+    a.flags = d_out;
+    a.stats = c->stats.p;
+    a.broad_pad = 1e-5f * (1.f + sc.scale);
+    const int depth = std::max(c->robot_depth, std::max(sc.tri_bvh.depth, sc.edge_bvh.depth));
+    const int stack = stack_for(depth);
+    RTCD_HIP(hipEventRecord(c->stage_ev[0], s));
+    launch_broad(a, stack, false, s);
+    RTCD_HIP(hipEventRecord(c->stage_ev[1], s));
+    exclusive_scan(c->counts.p, c->offsets.p, tasks, c->scalars.p, c->scan_tmp.p, s);
+    RTCD_HIP(hipEventRecord(c->stage_ev[2], s));
+    launch_broad(a, stack, true, s);
+    RTCD_HIP(hipEventRecord(c->stage_ev[3], s));
+    launch_narrow(a, stack, c->counting, s);
+    RTCD_HIP(hipEventRecord(c->ev1, s));
+    c->timing_pending = true;
+    c->last.max_stack = (uint32_t) stack;
+    c->last.max_depth = (uint32_t) depth;
+}
+
+// After the stream finished: read overflow flags, re-run with a larger candidate buffer if needed.
+void finish_discrete(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, uint32_t n, uint32_t* d_out,
+                     hipStream_t s) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        std::vector<uint32_t> sv = c->scalars.download(2, s);
+        if (sv[1] & 2u) throw std::runtime_error("traversal stack overflow (BVH deeper than the stack)");
+        if (!(sv[1] & 1u)) {
+            if (c->timing_pending) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last.last_ms = ms;
+                hipEvent_t evs[6] = {c->ev0, c->stage_ev[0], c->stage_ev[1], c->stage_ev[2], c->stage_ev[3], c->ev1};
+                for (int k = 0; k < 5; ++k) {
+                    float sm = 0.f;
+                    c->last.stage_ms[k] = hipEventElapsedTime(&sm, evs[k], evs[k + 1]) == hipSuccess ? sm : -1.0;
+                }
+                c->timing_pending = false;
+            }
+            c->last.candidates = sv[0];
+            if (c->counting) {
+                Stats st = c->stats.download(1, s)[0];
+                c->last.node_fetches = st.node_fetches;
+                c->last.prim_tests = st.prim_tests;
+                c->last.fp64_redecisions = st.fp64;
+                c->last.edge_tests = st.edge_tests;
+            }
+            return;
+        }
+        size_t need = (size_t) sv[0] + sv[0] / 4 + 1024;
+        if (need > 0xfffffff0ull) throw std::runtime_error("candidate count exceeds 2^32");
+        check_budget(c, need * sizeof(uint2));
+        c->cand.alloc(need);
+        c->cand_cap = (uint32_t) need;
+        enqueue_discrete(c, sc, dir, d_poses, n, d_out, s);
+    }
+    throw std::runtime_error("candidate buffer overflow persisted");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rtcd_last_error(void) { return g_err.c_str(); }
+
+rtcd_status rtcd_create(const rtcd_config* cfg, rtcd_ctx** out) {
+    if (!out) return fail(RTCD_ERR_INVALID_ARG, "out is null");
+    *out = nullptr;
+    return guarded([&]() -> rtcd_status {
+        rtcd_config c{};
+        c.skip_base_link = 1;
+        c.leaf_size = 4;
+        if (cfg) {
+            c = *cfg;
+            if (c.leaf_size == 0) c.leaf_size = 4;
+        }
+        if (c.leaf_size > 16) return fail(RTCD_ERR_INVALID_ARG, "leaf_size must be 1..16");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(RTCD_ERR_NO_DEVICE, "no HIP device");
+        if (c.device < 0 || c.device >= ndev) return fail(RTCD_ERR_NO_DEVICE, "device index out of range");
+        RTCD_HIP(hipSetDevice(c.device));
+        auto ctx = std::make_unique<rtcd_ctx>();
+        ctx->cfg = c;
+        RTCD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        *out = ctx.release();
+        return RTCD_OK;
+    });
+}
+
+void rtcd_destroy(rtcd_ctx* ctx) {
+    if (!ctx) return;
+    (void) hipSetDevice(ctx->cfg.device);
+    (void) hipDeviceSynchronize();
+    serve_shutdown(ctx->sweep);
+    for (auto e : ctx->stage_ev)
+        if (e) (void) hipEventDestroy(e);
+    if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void) hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
+    if (!ctx || !d) return fail(RTCD_ERR_INVALID_ARG, "null argument");
+    if (d->dof < 1 || d->dof > (uint32_t) kMaxDof) return fail(RTCD_ERR_INVALID_ARG, "dof must be 1..15");
+    if (!d->base || !d->cos_mask || !d->sin_mask || !d->one_mask || !d->verts || !d->link_vert_offsets || !d->tris ||
+        !d->link_tri_offsets)
+        return fail(RTCD_ERR_INVALID_ARG, "null array in robot description");
+    if (d->n_spheres && (!d->spheres || !d->sphere_link)) return fail(RTCD_ERR_INVALID_ARG, "null sphere arrays");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return guarded([&]() -> rtcd_status {
+        RTCD_HIP(hipSetDevice(ctx->cfg.device));
+        hipStream_t s = ctx->stream;
+        const int NL = (int) d->dof + 1;
+        RobotDev R{};
+        R.dof = (int) d->dof;
+        R.n_links = NL;
+        R.n_spheres = (int) d->n_spheres;
+        memcpy(R.base, d->base, sizeof(float) * 12);
+        memcpy(R.cmask, d->cos_mask, sizeof(float) * 12 * d->dof);
+        memcpy(R.smask, d->sin_mask, sizeof(float) * 12 * d->dof);
+        memcpy(R.omask, d->one_mask, sizeof(float) * 12 * d->dof);
+        if (!finite_all(R.base, 12) || !finite_all(R.cmask, 12 * d->dof) || !finite_all(R.smask, 12 * d->dof) ||
+            !finite_all(R.omask, 12 * d->dof))
+            return fail(RTCD_ERR_INVALID_ARG, "non-finite FK mask");
+        std::vector<BNode> all_tn, all_en;
+        std::vector<Tri> all_t;
+        std::vector<Edge> all_e;
+        int depth = 1;
+        for (int l = 0; l < NL; ++l) {
+            const uint32_t v0 = d->link_vert_offsets[l], v1 = d->link_vert_offsets[l + 1];
+            const uint32_t t0 = d->link_tri_offsets[l], t1 = d->link_tri_offsets[l + 1];
+            if (v1 < v0 || t1 < t0) return fail(RTCD_ERR_INVALID_ARG, "offsets must be non-decreasing");
+            const float* V = d->verts + 3 * (size_t) v0;
+            const int32_t* F = d->tris + 3 * (size_t) t0;
+            const int nv = (int) (v1 - v0), nt = (int) (t1 - t0);
+            if (!finite_all(V, 3 * (size_t) nv)) return fail(RTCD_ERR_INVALID_ARG, "non-finite link vertex");
+            for (int i = 0; i < 3 * nt; ++i)
+                if (F[i] < 0 || F[i] >= nv) return fail(RTCD_ERR_INVALID_ARG, "link triangle index out of range");
+            LinkInfo& L = R.link[l];
+            double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            if (nv > 0)
+                for (int k = 0; k < 3; ++k) { lo[k] = V[k]; hi[k] = V[k]; }
+            for (int i = 0; i < nv; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    lo[k] = std::min(lo[k], (double) V[3 * i + k]);
+                    hi[k] = std::max(hi[k], (double) V[3 * i + k]);
+                }
+            double c[3], sc = 0, rv = 0, e1 = 0, e2 = 0, dm = 0;
+            for (int k = 0; k < 3; ++k) {
+                L.lo[k] = (float) lo[k];
+                L.hi[k] = (float) hi[k];
+                L.c[k] = (float) (0.5 * (lo[k] + hi[k]));
+                c[k] = L.c[k];
+                sc = std::max(sc, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
+            }
+            for (int i = 0; i < nv; ++i)
+                rv = std::max(rv, std::fabs(V[3 * i] - c[0]) + std::fabs(V[3 * i + 1] - c[1]) + std::fabs(V[3 * i + 2] - c[2]));
+            EdgeSet es = unique_edges(F, nt);
+            // triangles (exact vertices) and their boxes
+            std::vector<Tri> tri(nt);
+            std::vector<float4> tb(2 * (size_t) nt);
+            for (int f = 0; f < nt; ++f) {
+                const float* a = V + 3 * F[3 * f];
+                const float* b = V + 3 * F[3 * f + 1];
+                const float* cc = V + 3 * F[3 * f + 2];
+                tri[f].a = make_float4(a[0], a[1], a[2], bits_f((uint32_t) f));
+                tri[f].b = make_float4(b[0], b[1], b[2], bits_f(es.owned[f]));
+                tri[f].c = make_float4(cc[0], cc[1], cc[2], 0.f);
+                e1 = std::max(e1, std::fabs((double) b[0] - a[0]) + std::fabs((double) b[1] - a[1]) + std::fabs((double) b[2] - a[2]));
+                e2 = std::max(e2, std::fabs((double) cc[0] - a[0]) + std::fabs((double) cc[1] - a[1]) + std::fabs((double) cc[2] - a[2]));
+                tb[2 * f] = make_float4(std::min({a[0], b[0], cc[0]}), std::min({a[1], b[1], cc[1]}), std::min({a[2], b[2], cc[2]}), 0.f);
+                tb[2 * f + 1] = make_float4(std::max({a[0], b[0], cc[0]}), std::max({a[1], b[1], cc[1]}), std::max({a[2], b[2], cc[2]}), 0.f);
+            }
+            const int ne = (int) es.edges.size() / 2;
+            std::vector<Edge> edg(ne);
+            std::vector<float4> eb(2 * (size_t) ne);
+            for (int e = 0; e < ne; ++e) {
+                const float* p = V + 3 * es.edges[2 * e];
+                const float* q = V + 3 * es.edges[2 * e + 1];
+                edg[e].p = make_float4(p[0], p[1], p[2], bits_f((uint32_t) e));
+                edg[e].q = make_float4(q[0], q[1], q[2], 0.f);
+                dm = std::max(dm, std::fabs((double) q[0] - p[0]) + std::fabs((double) q[1] - p[1]) + std::fabs((double) q[2] - p[2]));
+                eb[2 * e] = make_float4(std::min(p[0], q[0]), std::min(p[1], q[1]), std::min(p[2], q[2]), 0.f);
+                eb[2 * e + 1] = make_float4(std::max(p[0], q[0]), std::max(p[1], q[1]), std::max(p[2], q[2]), 0.f);
+            }
+            L.scale = round_up(sc);
+            L.rv = round_up(rv * (1 + 1e-6));
+            L.e1max = round_up(e1 * (1 + 1e-6));
+            L.e2max = round_up(e2 * (1 + 1e-6));
+            L.dmax = round_up(dm * (1 + 1e-6));
+            BvhOut bt, be;
+            std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->cfg.leaf_size, s, bt);
+            std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->cfg.leaf_size, s, be);
+            L.tri_node0 = (int) all_tn.size();
+            L.tri_prim0 = (int) all_t.size();
+            L.n_tri = nt;
+            L.tri_has_root = bt.n_nodes > 0;
+            L.edge_node0 = (int) all_en.size();
+            L.edge_prim0 = (int) all_e.size();
+            L.n_edge = ne;
+            L.edge_has_root = be.n_nodes > 0;
+            std::vector<BNode> hn = bt.nodes.download(bt.n_nodes, s);
+            all_tn.insert(all_tn.end(), hn.begin(), hn.end());
+            for (int i = 0; i < nt; ++i) all_t.push_back(tri[pt[i]]);
+            hn = be.nodes.download(be.n_nodes, s);
+            all_en.insert(all_en.end(), hn.begin(), hn.end());
+            for (int i = 0; i < ne; ++i) all_e.push_back(edg[pe[i]]);
+            depth = std::max(depth, std::max(bt.depth, be.depth));
+        }
+        if (!d->n_spheres) R.n_spheres = 0;
+        std::vector<float4> sph(d->n_spheres);
+        ctx->sphere_r.assign(d->n_spheres, 0.f);
+        for (uint32_t i = 0; i < d->n_spheres; ++i) {
+            if (d->sphere_link[i] < 0 || d->sphere_link[i] >= NL) return fail(RTCD_ERR_INVALID_ARG, "sphere link out of range");
+            if (!finite_all(d->spheres + 4 * i, 4) || d->spheres[4 * i + 3] < 0.f)
+                return fail(RTCD_ERR_INVALID_ARG, "invalid sphere");
+            sph[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
+            ctx->sphere_r[i] = d->spheres[4 * i + 3];
+        }
+        ctx->sweep.sphere_link.assign(d->sphere_link, d->sphere_link + d->n_spheres);
+        ctx->r_tri_nodes.alloc(all_tn.size());
+        ctx->r_tri_nodes.upload(all_tn, s);
+        ctx->r_tris.alloc(all_t.size());
+        ctx->r_tris.upload(all_t, s);
+        ctx->r_edge_nodes.alloc(all_en.size());
+        ctx->r_edge_nodes.upload(all_en, s);
+        ctx->r_edges.alloc(all_e.size());
+        ctx->r_edges.upload(all_e, s);
+        ctx->r_spheres.alloc(sph.size());
+        ctx->r_spheres.upload(sph, s);
+        ctx->robot = R;
+        ctx->d_robot.alloc(1);
+        ctx->d_robot.upload(&ctx->robot, 1, s);
+        RTCD_HIP(hipStreamSynchronize(s));
+        ctx->robot_depth = depth;
+        ctx->robot_loaded = true;
+        sweep_robot_loaded(ctx->sweep, ctx->r_spheres.p, (int) d->n_spheres, s);
+        return RTCD_OK;
+    });
+}
+
+rtcd_status rtcd_scene_create(rtcd_ctx* ctx, const float* verts, uint32_t n_verts, const int32_t* tris,
+                              uint32_t n_tris, uint32_t* scene_id) {
+    if (!ctx || !scene_id || (n_verts && !verts) || (n_tris && !tris)) return fail(RTCD_ERR_INVALID_ARG, "null argument");
+    if (!finite_all(verts, 3 * (size_t) n_verts)) return fail(RTCD_ERR_INVALID_ARG, "non-finite scene vertex");
+    for (size_t i = 0; i < 3 * (size_t) n_tris; ++i)
+        if (tris[i] < 0 || (uint32_t) tris[i] >= n_verts) return fail(RTCD_ERR_INVALID_ARG, "scene triangle index out of range");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return guarded([&]() -> rtcd_status {
+        RTCD_HIP(hipSetDevice(ctx->cfg.device));
+        hipStream_t s = ctx->stream;
+        auto sc = std::make_unique<SceneRes>();
+        EdgeSet es = unique_edges(tris, n_tris);
+        const int nt = (int) n_tris, ne = (int) es.edges.size() / 2;
+        std::vector<Tri> tri(nt);
+        std::vector<float4> tb(2 * (size_t) nt);
+        double scale = 0;
+        for (uint32_t i = 0; i < 3 * n_verts; ++i) scale = std::max(scale, (double) std::fabs(verts[i]));
+        for (int f = 0; f < nt; ++f) {
+            const float* a = verts + 3 * tris[3 * f];
+            const float* b = verts + 3 * tris[3 * f + 1];
+            const float* c = verts + 3 * tris[3 * f + 2];
+            tri[f].a = make_float4(a[0], a[1], a[2], bits_f((uint32_t) f));
+            tri[f].b = make_float4(b[0], b[1], b[2], bits_f(es.owned[f]));
+            tri[f].c = make_float4(c[0], c[1], c[2], 0.f);
+            tb[2 * f] = make_float4(std::min({a[0], b[0], c[0]}), std::min({a[1], b[1], c[1]}), std::min({a[2], b[2], c[2]}), 0.f);
+            tb[2 * f + 1] = make_float4(std::max({a[0], b[0], c[0]}), std::max({a[1], b[1], c[1]}), std::max({a[2], b[2], c[2]}), 0.f);
+        }
+        std::vector<Edge> edg(ne);
+        std::vector<float4> eb(2 * (size_t) ne);
+        for (int e = 0; e < ne; ++e) {
+            const float* p = verts + 3 * es.edges[2 * e];
+            const float* q = verts + 3 * es.edges[2 * e + 1];
+            edg[e].p = make_float4(p[0], p[1], p[2], bits_f((uint32_t) e));
+            edg[e].q = make_float4(q[0], q[1], q[2], 0.f);
+            eb[2 * e] = make_float4(std::min(p[0], q[0]), std::min(p[1], q[1]), std::min(p[2], q[2]), 0.f);
+            eb[2 * e + 1] = make_float4(std::max(p[0], q[0]), std::max(p[1], q[1]), std::max(p[2], q[2]), 0.f);
+        }
+        std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->cfg.leaf_size, s, sc->tri_bvh);
+        std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->cfg.leaf_size, s, sc->edge_bvh);
+        std::vector<Tri> tri_s(nt);
+        for (int i = 0; i < nt; ++i) tri_s[i] = tri[pt[i]];
+        std::vector<Edge> edg_s(ne);
+        for (int i = 0; i < ne; ++i) edg_s[i] = edg[pe[i]];
+        sc->tris.alloc(nt);
+        sc->tris.upload(tri_s, s);
+        sc->edges.alloc(ne);
+        sc->edges.upload(edg_s, s);
+        sc->n_tri = nt;
+        sc->n_edge = ne;
+        sc->scale = (float) scale;
+        RTCD_HIP(hipStreamSynchronize(s));
+        uint32_t id = ctx->next_scene++;
+        ctx->scenes[id] = std::move(sc);
+        *scene_id = id;
+        return RTCD_OK;
+    });
+}
+
+rtcd_status rtcd_scene_destroy(rtcd_ctx* ctx, uint32_t scene_id) {
+    if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    auto it = ctx->scenes.find(scene_id);
+    if (it == ctx->scenes.end()) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
+    (void) hipStreamSynchronize(ctx->stream);
+    ctx->scenes.erase(it);
+    return RTCD_OK;
+}
+
+rtcd_status rtcd_scene_info(rtcd_ctx* ctx, uint32_t scene_id, uint32_t* n_tris, uint32_t* n_edges) {
+    if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    SceneRes* sc = find_scene(ctx, scene_id);
+    if (!sc) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
+    if (n_tris) *n_tris = (uint32_t) sc->n_tri;
+    if (n_edges) *n_edges = (uint32_t) sc->n_edge;
+    return RTCD_OK;
+}
+
+rtcd_status rtcd_check_discrete_device(rtcd_ctx* ctx, uint32_t scene_id, int32_t direction, const float* d_poses,
+                                       uint32_t n, uint32_t* d_out, void* stream) {
+    if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
+    if (direction < 1 || direction > 3) return fail(RTCD_ERR_INVALID_ARG, "direction must be 1, 2 or 3");
+    if (n && (!d_poses || !d_out)) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
+    if (n > (1u << 26)) return fail(RTCD_ERR_INVALID_ARG, "n too large for one call (max 2^26)");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (!ctx->robot_loaded) return fail(RTCD_ERR_NOT_READY, "robot not loaded");
+    SceneRes* sc = find_scene(ctx, scene_id);
+    if (!sc) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
+    return guarded([&]() -> rtcd_status {
+        RTCD_HIP(hipSetDevice(ctx->cfg.device));
+        hipStream_t s = stream ? (hipStream_t) stream : ctx->stream;
+        if (n == 0) return RTCD_OK;
+        enqueue_discrete(ctx, *sc, direction, d_poses, n, d_out, s);
+        ctx->pending = {true, scene_id, direction, d_poses, n, d_out, s};
+        return RTCD_OK;
+    });
+}
+
+rtcd_status rtcd_sync(rtcd_ctx* ctx, void* stream) {
+    if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return guarded([&]() -> rtcd_status {
+        hipStream_t s = stream ? (hipStream_t) stream : ctx->stream;
+        RTCD_HIP(hipStreamSynchronize(s));
+        if (ctx->pending.active && ctx->pending.stream == s) {
+            auto p = ctx->pending;
+            ctx->pending.active = false;
+            SceneRes* sc = find_scene(ctx, p.scene);
+            if (!sc) return fail(RTCD_ERR_UNKNOWN_ID, "scene destroyed while a check was pending");
+            finish_discrete(ctx, *sc, p.dir, p.poses, p.n, p.out, s);
+        }
+        return RTCD_OK;
+    });
+}
+
+rtcd_status rtcd_check_discrete(rtcd_ctx* ctx, uint32_t scene_id, int32_t direction, const float* poses, uint32_t n,
+                                uint32_t* out) {
+    if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
+    if (direction < 1 || direction > 3) return fail(RTCD_ERR_INVALID_ARG, "direction must be 1, 2 or 3");
+    if (n && (!poses || !out)) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
+    if (n > (1u << 26)) return fail(RTCD_ERR_INVALID_ARG, "n too large for one call (max 2^26)");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (!ctx->robot_loaded) return fail(RTCD_ERR_NOT_READY, "robot not loaded");
+    SceneRes* sc = find_scene(ctx, scene_id);
+    if (!sc) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
+    if (!finite_all(poses, (size_t) n * ctx->robot.dof)) return fail(RTCD_ERR_INVALID_ARG, "non-finite joint value");
+    if (n == 0) return RTCD_OK;
+    return guarded([&]() -> rtcd_status {
+        RTCD_HIP(hipSetDevice(ctx->cfg.device));
+        hipStream_t s = ctx->stream;
+        ctx->h2d_poses.reserve((size_t) n * ctx->robot.dof);
+        ctx->h2d_out.reserve(n);
+        RTCD_HIP(hipMemcpyAsync(ctx->h2d_poses.p, poses, sizeof(float) * n * ctx->robot.dof, hipMemcpyHostToDevice, s));
+        enqueue_discrete(ctx, *sc, direction, ctx->h2d_poses.p, n, ctx->h2d_out.p, s);
+        RTCD_HIP(hipStreamSynchronize(s));
+        finish_discrete(ctx, *sc, direction, ctx->h2d_poses.p, n, ctx->h2d_out.p, s);
+        RTCD_HIP(hipMemcpyAsync(out, ctx->h2d_out.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+        RTCD_HIP(hipStreamSynchronize(s));
+        return RTCD_OK;
+    });
+}
+
+rtcd_status rtcd_set_counting(rtcd_ctx* ctx, int32_t enable) {
+    if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->counting = enable != 0;
+    return RTCD_OK;
+}
+
+rtcd_status rtcd_get_stats(rtcd_ctx* ctx, rtcd_stats* st) {
+    if (!ctx || !st) return fail(RTCD_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    *st = ctx->last;
+    return RTCD_OK;
+}
+
+}  // extern "C"

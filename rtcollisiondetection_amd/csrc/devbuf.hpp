@@ -1,0 +1,77 @@
+// Minimal RAII device buffer and the host-side launch helpers shared by the .hip files.
+#pragma once
+#include "common.hpp"
+
+#include <utility>
+#include <vector>
+
+namespace rtcd {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n;
+            o.p = nullptr; o.n = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count) RTCD_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    // Grow (never shrink); contents are not preserved.
+    void reserve(size_t count) {
+        if (count > n) alloc(count + count / 4);
+    }
+    void release() {
+        if (p) (void) hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        reserve(count);
+        if (count) RTCD_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+    std::vector<T> download(size_t count, hipStream_t s) const {
+        std::vector<T> h(count);
+        if (count) RTCD_HIP(hipMemcpyAsync(h.data(), p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+        RTCD_HIP(hipStreamSynchronize(s));
+        return h;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned) ((n + bs - 1) / bs); }
+
+// ---- scan.hip ----
+// Exclusive prefix sum of n uint32 values.  tmp must hold scan_tmp_words(n) words.  If total is
+// non-null the grand total is written to *total (device).  Safe under stream capture (no memset).
+size_t scan_tmp_words(size_t n);
+void exclusive_scan(const uint32_t* in, uint32_t* out, size_t n, uint32_t* total, uint32_t* tmp, hipStream_t s);
+// out[i] = value (fill kernel; used instead of hipMemsetAsync so that captured graphs stay valid).
+void fill_u32(uint32_t* out, size_t n, uint32_t value, hipStream_t s);
+
+// ---- lbvh.hip ----
+struct BvhOut {
+    DevBuf<BNode> nodes;
+    DevBuf<int32_t> perm;  // leaf order -> original primitive index
+    int n_nodes = 0;
+    int depth = 0;         // max number of inner nodes on a root-to-leaf path
+};
+// boxes: 2 float4 per primitive (lo, hi).  Builds on the device: Morton codes, LDS radix sort,
+// Karras hierarchy, bottom-up boxes, leaf collapse to at most leaf_size primitives.
+void build_lbvh(const float4* boxes, int n, int leaf_size, hipStream_t s, BvhOut& out);
+
+}  // namespace rtcd

@@ -1,0 +1,391 @@
+// Discrete checks: batched FK, broad phase (link boxes vs scene LBVH) and the any-hit narrow phase.
+//
+// Reference path (RTCD/CollisionDetector/CollisionDetector.h:919-983 detectWithOBB):
+//   fkineKernelAdj (RTCD/Robot/robot.cu:202-241) -> per-batch IAS rebuild -> optixLaunch of
+//   __raygen__oneside/__anyhit__IAS (CCCuda.cu:31-77, obstacle edges vs link instances) or
+//   __raygen__robotRayCompact (CCRay.cu:74-116, link edges moved to the world vs the scene).
+// Here no acceleration structure is rebuilt per batch: for each (pose, link) the static scene LBVH
+// is queried with the link's boxes, and each candidate scene triangle is moved into the link frame,
+// where its owned edges trace the static link-triangle LBVH (obstacle->robot) and the triangle
+// itself queries the static link-edge LBVH (robot->obstacle).
+#include "band32.hpp"
+#include "exact64.hpp"
+#include "kernels.hpp"
+
+namespace rtcd {
+namespace {
+
+constexpr int kBS = 256;
+
+__device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+__device__ __forceinline__ float3 sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 xf(const float4& r0, const float4& r1, const float4& r2, float3 p) {
+    return make_float3(r0.x * p.x + r0.y * p.y + r0.z * p.z + r0.w, r1.x * p.x + r1.y * p.y + r1.z * p.z + r1.w,
+                       r2.x * p.x + r2.y * p.y + r2.z * p.z + r2.w);
+}
+__device__ __forceinline__ uint32_t load_flag(const uint32_t* f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------------------------------------ FK
+__global__ __launch_bounds__(kBS) void k_fk(const RobotDev* __restrict__ R, const float* __restrict__ poses, int n,
+                                           double* __restrict__ T64, float4* __restrict__ Ti,
+                                           float4* __restrict__ wbox) {
+    int p = blockIdx.x * kBS + threadIdx.x;
+    if (p >= n) return;
+    const int dof = R->dof, NL = dof + 1;
+    double T[12];
+    for (int k = 0; k < 12; ++k) T[k] = (double) R->base[k];
+    for (int l = 0; l < NL; ++l) {
+        if (l > 0) {
+            const int i = l - 1;
+            double s, c, L[12], Tn[12];
+            x64::sincos((double) poses[(size_t) p * dof + i], &s, &c);
+            {
+#pragma clang fp contract(off)
+                for (int k = 0; k < 12; ++k)
+                    L[k] = (c * (double) R->cmask[i * 12 + k] + s * (double) R->smask[i * 12 + k]) +
+                           (double) R->omask[i * 12 + k];
+            }
+            x64::affine_mul(T, L, Tn);
+            for (int k = 0; k < 12; ++k) T[k] = Tn[k];
+        }
+        const size_t pl = (size_t) p * NL + l;
+        for (int k = 0; k < 12; ++k) T64[pl * 12 + k] = T[k];
+        // inverse rows [R^T | -R^T t]
+        for (int j = 0; j < 3; ++j) {
+            double tj = -((T[0 * 4 + j] * T[3] + T[1 * 4 + j] * T[7]) + T[2 * 4 + j] * T[11]);
+            Ti[pl * 3 + j] = make_float4((float) T[0 * 4 + j], (float) T[1 * 4 + j], (float) T[2 * 4 + j], (float) tj);
+        }
+        // world AABB of the link box, rounded outwards
+        const LinkInfo& L = R->link[l];
+        double c[3] = {L.c[0], L.c[1], L.c[2]};
+        double h[3] = {0.5 * ((double) L.hi[0] - L.lo[0]), 0.5 * ((double) L.hi[1] - L.lo[1]),
+                       0.5 * ((double) L.hi[2] - L.lo[2])};
+        float lo[3], hi[3];
+        for (int r = 0; r < 3; ++r) {
+            double wc = T[r * 4 + 0] * c[0] + T[r * 4 + 1] * c[1] + T[r * 4 + 2] * c[2] + T[r * 4 + 3];
+            double wh = fabs(T[r * 4 + 0]) * h[0] + fabs(T[r * 4 + 1]) * h[1] + fabs(T[r * 4 + 2]) * h[2];
+            double pad = 1e-6 * (1.0 + fabs(wc) + wh);
+            lo[r] = (float) (wc - wh - pad);
+            hi[r] = (float) (wc + wh + pad);
+        }
+        wbox[pl * 2] = make_float4(lo[0], lo[1], lo[2], 0.f);
+        wbox[pl * 2 + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+    }
+}
+
+// ------------------------------------------------------------------------------------------ broad
+// Child box (lo, hi) of a scene node against link (p, l): world AABB test and link-frame test.
+__device__ __forceinline__ bool broad_test(float lx, float ly, float lz, float hx, float hy, float hz, float4 wlo,
+                                           float4 whi, const float4& r0, const float4& r1, const float4& r2,
+                                           const float3& lc, const float3& lh, float pad) {
+    if (lx > whi.x || hx < wlo.x || ly > whi.y || hy < wlo.y || lz > whi.z || hz < wlo.z) return false;
+    float cx = 0.5f * (lx + hx), cy = 0.5f * (ly + hy), cz = 0.5f * (lz + hz);
+    float ex = 0.5f * (hx - lx), ey = 0.5f * (hy - ly), ez = 0.5f * (hz - lz);
+    float3 c = xf(r0, r1, r2, make_float3(cx, cy, cz));
+    float hx2 = fabsf(r0.x) * ex + fabsf(r0.y) * ey + fabsf(r0.z) * ez;
+    float hy2 = fabsf(r1.x) * ex + fabsf(r1.y) * ey + fabsf(r1.z) * ez;
+    float hz2 = fabsf(r2.x) * ex + fabsf(r2.y) * ey + fabsf(r2.z) * ez;
+    return fabsf(c.x - lc.x) <= hx2 + lh.x + pad && fabsf(c.y - lc.y) <= hy2 + lh.y + pad &&
+           fabsf(c.z - lc.z) <= hz2 + lh.z + pad;
+}
+
+template <int STACK, bool EMIT>
+__global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a) {
+    __shared__ int stk[STACK * kBS];
+    int* st = stk + threadIdx.x;
+    const int task = blockIdx.x * kBS + threadIdx.x;
+    if (task >= a.n_active * a.n_poses) return;
+    const int li = task / a.n_poses, p = task - li * a.n_poses, l = a.l0 + li;
+    const size_t pl = (size_t) p * a.n_links + l;
+    const float4 wlo = a.wbox[pl * 2], whi = a.wbox[pl * 2 + 1];
+    const float4 r0 = a.Ti[pl * 3], r1 = a.Ti[pl * 3 + 1], r2 = a.Ti[pl * 3 + 2];
+    const LinkInfo& L = a.robot->link[l];
+    const float3 lc = make_float3(L.c[0], L.c[1], L.c[2]);
+    const float3 lh = make_float3(0.5f * (L.hi[0] - L.lo[0]), 0.5f * (L.hi[1] - L.lo[1]), 0.5f * (L.hi[2] - L.lo[2]));
+    const float pad = a.broad_pad;
+    uint32_t count = 0;
+    uint32_t base = 0;
+    if (EMIT) base = a.offsets[task];
+    const uint32_t key = (uint32_t) pl;
+    if (a.sc.tri_has_root && (L.n_tri > 0 || L.n_edge > 0)) {
+        int sp = 0, ni = 0;
+        for (;;) {
+            const BNode nd = a.sc.tri_nodes[ni];
+            bool h0 = nd.c0 >= 0 && broad_test(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, wlo, whi, r0, r1,
+                                               r2, lc, lh, pad);
+            bool h1 = nd.c1 >= 0 && broad_test(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, wlo, whi, r0, r1,
+                                               r2, lc, lh, pad);
+            if (h0 && nd.n0 > 0) {
+                if (EMIT)
+                    for (int k = 0; k < nd.n0; ++k) {
+                        uint32_t o = base + count + k;
+                        if (o < a.cap) a.cand[o] = make_uint2(key, (uint32_t) (nd.c0 + k));
+                        else atomicOr(a.overflow, 1u);
+                    }
+                count += nd.n0;
+                h0 = false;
+            }
+            if (h1 && nd.n1 > 0) {
+                if (EMIT)
+                    for (int k = 0; k < nd.n1; ++k) {
+                        uint32_t o = base + count + k;
+                        if (o < a.cap) a.cand[o] = make_uint2(key, (uint32_t) (nd.c1 + k));
+                        else atomicOr(a.overflow, 1u);
+                    }
+                count += nd.n1;
+                h1 = false;
+            }
+            if (h0 && h1) {
+                if (sp >= STACK) { atomicOr(a.overflow, 2u); break; }
+                st[sp * kBS] = nd.c1;
+                ++sp;
+                ni = nd.c0;
+            } else if (h0) {
+                ni = nd.c0;
+            } else if (h1) {
+                ni = nd.c1;
+            } else {
+                if (sp == 0) break;
+                --sp;
+                ni = st[sp * kBS];
+            }
+        }
+    }
+    if (!EMIT) a.counts[task] = count;
+}
+
+// ------------------------------------------------------------------------------------------ narrow
+struct Ctr {
+    uint32_t nodes = 0, prims = 0, fp64 = 0, edges = 0;
+};
+
+// Obstacle edge (link frame P -> Q, world endpoints wa -> wb) against the link-triangle LBVH.
+template <int STACK, bool COUNT>
+__device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, const double* T64, float3 P, float3 Q,
+                        float delta, float4 wa, float4 wb, int* st, const uint32_t* flag, Ctr& ctr) {
+    if (!L.tri_has_root) return 0;
+    const float3 d = sub(Q, P);
+    const float3 inv = safe_inv(d);
+    const float pad = 2.f * delta + 1e-6f * L.scale;
+    if (!slab(P, inv, L.lo[0], L.lo[1], L.lo[2], L.hi[0], L.hi[1], L.hi[2], pad)) return 0;
+    const float D = l1(d.x, d.y, d.z) * (1.f + 2.f * kEps32) + 6.f * delta;
+    const float ad = 2.f * delta + 1.01f * kEps32 * D;
+    const float S = l1(P.x - L.c[0], P.y - L.c[1], P.z - L.c[2]) * (1.f + 2.f * kEps32) + 3.f * delta + L.rv;
+    const float as = delta + kEps32 * S;
+    const MTBands bd = make_bands(D, ad, S, as, L.e1max, kEps32 * L.e1max, L.e2max, kEps32 * L.e2max);
+    const BNode* nodes = a.rb.tri_nodes + L.tri_node0;
+    const Tri* prims = a.rb.tri_prims + L.tri_prim0;
+    bool have64 = false;
+    double P64[3], Q64[3];
+    int sp = 0, ni = 0, iter = 0;
+    for (;;) {
+        if ((++iter & 15) == 0 && load_flag(flag)) return 0;
+        const BNode nd = nodes[ni];
+        if (COUNT) ctr.nodes++;
+        bool h0 = nd.c0 >= 0 && slab(P, inv, nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, pad);
+        bool h1 = nd.c1 >= 0 && slab(P, inv, nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, pad);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const bool hit = side ? h1 : h0;
+            const int cnt = side ? nd.n1 : nd.n0;
+            const int start = side ? nd.c1 : nd.c0;
+            if (hit && cnt > 0) {
+                for (int k = 0; k < cnt; ++k) {
+                    const Tri t = prims[start + k];
+                    if (COUNT) ctr.prims++;
+                    int r = mt32(P, d, f3(t.a), f3(t.b), f3(t.c), bd);
+                    if (r == kHit) return 1;
+                    if (r == kUnsure) {
+                        if (COUNT) ctr.fp64++;
+                        if (!have64) {
+                            double A[3] = {wa.x, wa.y, wa.z}, B[3] = {wb.x, wb.y, wb.z};
+                            x64::inv_xform(T64, A, P64);
+                            x64::inv_xform(T64, B, Q64);
+                            have64 = true;
+                        }
+                        double ta[3] = {t.a.x, t.a.y, t.a.z}, tb[3] = {t.b.x, t.b.y, t.b.z},
+                               tc[3] = {t.c.x, t.c.y, t.c.z};
+                        if (x64::seg_tri(P64, Q64, ta, tb, tc)) return 1;
+                    }
+                }
+                if (side) h1 = false; else h0 = false;
+            }
+        }
+        if (h0 && h1) {
+            if (sp >= STACK) { atomicOr(a.overflow, 2u); return 0; }
+            st[sp * kBS] = nd.c1;
+            ++sp;
+            ni = nd.c0;
+        } else if (h0) {
+            ni = nd.c0;
+        } else if (h1) {
+            ni = nd.c1;
+        } else {
+            if (sp == 0) return 0;
+            --sp;
+            ni = st[sp * kBS];
+        }
+    }
+}
+
+// Scene triangle (link frame A', B', C'; world wa, wb, wc) against the link-edge LBVH.
+template <int STACK, bool COUNT>
+__device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, const double* T64, float3 A, float3 B, float3 C,
+                       float delta, float4 wa, float4 wb, float4 wc, int* st, const uint32_t* flag, Ctr& ctr) {
+    if (!L.edge_has_root) return 0;
+    const float pad = 2.f * delta + 1e-6f * L.scale;
+    const float lx = fminf(fminf(A.x, B.x), C.x) - pad, ly = fminf(fminf(A.y, B.y), C.y) - pad,
+                lz = fminf(fminf(A.z, B.z), C.z) - pad;
+    const float hx = fmaxf(fmaxf(A.x, B.x), C.x) + pad, hy = fmaxf(fmaxf(A.y, B.y), C.y) + pad,
+                hz = fmaxf(fmaxf(A.z, B.z), C.z) + pad;
+    if (lx > L.hi[0] || hx < L.lo[0] || ly > L.hi[1] || hy < L.lo[1] || lz > L.hi[2] || hz < L.lo[2]) return 0;
+    const float E1 = l1(B.x - A.x, B.y - A.y, B.z - A.z) * (1.f + 2.f * kEps32) + 6.f * delta;
+    const float E2 = l1(C.x - A.x, C.y - A.y, C.z - A.z) * (1.f + 2.f * kEps32) + 6.f * delta;
+    const float S = l1(A.x - L.c[0], A.y - L.c[1], A.z - L.c[2]) * (1.f + 2.f * kEps32) + 3.f * delta + L.rv;
+    const MTBands bd = make_bands(L.dmax, kEps32 * L.dmax, S, delta + kEps32 * S, E1, 2.f * delta + kEps32 * E1, E2,
+                                  2.f * delta + kEps32 * E2);
+    const BNode* nodes = a.rb.edge_nodes + L.edge_node0;
+    const Edge* prims = a.rb.edge_prims + L.edge_prim0;
+    int sp = 0, ni = 0, iter = 0;
+    for (;;) {
+        if ((++iter & 15) == 0 && load_flag(flag)) return 0;
+        const BNode nd = nodes[ni];
+        if (COUNT) ctr.nodes++;
+        bool h0 = nd.c0 >= 0 && !(lx > nd.hi0x || hx < nd.lo0x || ly > nd.hi0y || hy < nd.lo0y || lz > nd.hi0z ||
+                                  hz < nd.lo0z);
+        bool h1 = nd.c1 >= 0 && !(lx > nd.hi1x || hx < nd.lo1x || ly > nd.hi1y || hy < nd.lo1y || lz > nd.hi1z ||
+                                  hz < nd.lo1z);
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+            const bool hit = side ? h1 : h0;
+            const int cnt = side ? nd.n1 : nd.n0;
+            const int start = side ? nd.c1 : nd.c0;
+            if (hit && cnt > 0) {
+                for (int k = 0; k < cnt; ++k) {
+                    const Edge e = prims[start + k];
+                    if (COUNT) { ctr.prims++; ctr.edges++; }
+                    const float3 ep = f3(e.p);
+                    int r = mt32(ep, sub(f3(e.q), ep), A, B, C, bd);
+                    if (r == kHit) return 1;
+                    if (r == kUnsure) {
+                        if (COUNT) ctr.fp64++;
+                        double p0[3] = {e.p.x, e.p.y, e.p.z}, q0[3] = {e.q.x, e.q.y, e.q.z}, P64[3], Q64[3];
+                        x64::xform(T64, p0, P64);
+                        x64::xform(T64, q0, Q64);
+                        double ta[3] = {wa.x, wa.y, wa.z}, tb[3] = {wb.x, wb.y, wb.z}, tc[3] = {wc.x, wc.y, wc.z};
+                        if (x64::seg_tri(P64, Q64, ta, tb, tc)) return 1;
+                    }
+                }
+                if (side) h1 = false; else h0 = false;
+            }
+        }
+        if (h0 && h1) {
+            if (sp >= STACK) { atomicOr(a.overflow, 2u); return 0; }
+            st[sp * kBS] = nd.c1;
+            ++sp;
+            ni = nd.c0;
+        } else if (h0) {
+            ni = nd.c0;
+        } else if (h1) {
+            ni = nd.c1;
+        } else {
+            if (sp == 0) return 0;
+            --sp;
+            ni = st[sp * kBS];
+        }
+    }
+}
+
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
+    __shared__ int stk[STACK * kBS];
+    int* st = stk + threadIdx.x;
+    const uint32_t total = min(*a.total, a.cap);
+    const uint32_t stride = gridDim.x * kBS;
+    Ctr ctr;
+    for (uint32_t i = blockIdx.x * kBS + threadIdx.x; i < total; i += stride) {
+        const uint2 c = a.cand[i];
+        const int p = (int) (c.x / (uint32_t) a.n_links), l = (int) (c.x - (uint32_t) p * a.n_links);
+        uint32_t* flag = a.flags + p;
+        if (load_flag(flag)) continue;
+        const LinkInfo& L = a.robot->link[l];
+        const float4 r0 = a.Ti[(size_t) c.x * 3], r1 = a.Ti[(size_t) c.x * 3 + 1], r2 = a.Ti[(size_t) c.x * 3 + 2];
+        const double* T64 = a.T64 + (size_t) c.x * 12;
+        const Tri t = a.sc.tri_prims[c.y];
+        const float3 A = xf(r0, r1, r2, f3(t.a)), B = xf(r0, r1, r2, f3(t.b)), C = xf(r0, r1, r2, f3(t.c));
+        // |P' - P_exact| <= 4 eps (|P|_1 + |t'|) per component; 6 eps with the FP64 margin
+        const float tnorm = fabsf(r0.w) + fabsf(r1.w) + fabsf(r2.w);
+        const float vmax = fmaxf(fmaxf(l1(t.a.x, t.a.y, t.a.z), l1(t.b.x, t.b.y, t.b.z)), l1(t.c.x, t.c.y, t.c.z));
+        const float delta = 6.f * kEps32 * (vmax + tnorm) * 1.01f;
+        int hit = 0;
+        if (a.dir & 1) {
+            const uint32_t owned = __float_as_uint(t.b.w);
+            const float3 V[3] = {A, B, C};
+            const float4 W[3] = {t.a, t.b, t.c};
+            for (int j = 0; j < 3 && !hit; ++j)
+                if (owned & (1u << j))
+                    hit = o2r_edge<STACK, COUNT>(a, L, T64, V[j], V[(j + 1) % 3], delta, W[j], W[(j + 1) % 3], st,
+                                                 flag, ctr);
+        }
+        if (!hit && (a.dir & 2)) hit = r2o_tri<STACK, COUNT>(a, L, T64, A, B, C, delta, t.a, t.b, t.c, st, flag, ctr);
+        if (hit) atomicOr(flag, 1u);
+    }
+    if (COUNT) {
+        atomicAdd(&a.stats->node_fetches, (unsigned long long) ctr.nodes);
+        atomicAdd(&a.stats->prim_tests, (unsigned long long) ctr.prims);
+        atomicAdd(&a.stats->fp64, (unsigned long long) ctr.fp64);
+        atomicAdd(&a.stats->edge_tests, (unsigned long long) ctr.edges);
+    }
+}
+
+int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void) hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+}  // namespace
+
+void launch_fk(const RobotDev* robot, int, int, const float* poses, int n, double* T64, float4* Ti, float4* wbox,
+               hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fk, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, robot, poses, n, T64, Ti, wbox);
+    RTCD_LAUNCH_CHECK();
+}
+
+template <int STACK>
+static void broad_t(const DiscreteArgs& a, bool emit, hipStream_t s) {
+    unsigned nb = blocks_for((size_t) a.n_active * a.n_poses, kBS);
+    if (!nb) return;
+    if (emit) hipLaunchKernelGGL((k_broad<STACK, true>), dim3(nb), dim3(kBS), 0, s, a);
+    else hipLaunchKernelGGL((k_broad<STACK, false>), dim3(nb), dim3(kBS), 0, s, a);
+    RTCD_LAUNCH_CHECK();
+}
+
+void launch_broad(const DiscreteArgs& a, int stack, bool emit, hipStream_t s) {
+    if (stack <= 32) broad_t<32>(a, emit, s);
+    else if (stack <= 64) broad_t<64>(a, emit, s);
+    else throw std::runtime_error("BVH deeper than 64 levels");
+}
+
+template <int STACK>
+static void narrow_t(const DiscreteArgs& a, bool count, hipStream_t s) {
+    unsigned nb = (unsigned) device_cus() * (STACK <= 32 ? 8u : 4u);
+    if (count) hipLaunchKernelGGL((k_narrow<STACK, true>), dim3(nb), dim3(kBS), 0, s, a);
+    else hipLaunchKernelGGL((k_narrow<STACK, false>), dim3(nb), dim3(kBS), 0, s, a);
+    RTCD_LAUNCH_CHECK();
+}
+
+void launch_narrow(const DiscreteArgs& a, int stack, bool count, hipStream_t s) {
+    if (stack <= 32) narrow_t<32>(a, count, s);
+    else if (stack <= 64) narrow_t<64>(a, count, s);
+    else throw std::runtime_error("BVH deeper than 64 levels");
+}
+
+}  // namespace rtcd

@@ -1,0 +1,79 @@
+// Launch interfaces of the hot-path kernels (discrete.hip, swept.hip).
+#pragma once
+#include "devbuf.hpp"
+
+namespace rtcd {
+
+struct RobotBvh {
+    const BNode* tri_nodes;
+    const Tri* tri_prims;
+    const BNode* edge_nodes;
+    const Edge* edge_prims;
+};
+
+struct SceneDev {
+    const BNode* tri_nodes;
+    const Tri* tri_prims;
+    const BNode* edge_nodes;
+    const Edge* edge_prims;
+    int n_tri, n_edge;
+    int tri_has_root, edge_has_root;
+    float scale;   // max |coordinate| of the scene
+};
+
+struct Stats {
+    unsigned long long node_fetches, prim_tests, fp64, candidates, edge_tests;
+};
+
+struct DiscreteArgs {
+    const RobotDev* robot;
+    RobotBvh rb;
+    SceneDev sc;
+    int n_poses, n_links, l0, n_active, dir;
+    const double* T64;     // [n][NL][12]
+    const float4* Ti;      // [n][NL][3]  rows of [R^T | -R^T t], fp32
+    const float4* wbox;    // [n][NL][2]  world AABB of each link
+    uint32_t* counts;      // [n_active * n]
+    const uint32_t* offsets;
+    uint2* cand;           // (pose * NL + link, scene triangle)
+    uint32_t cap;
+    const uint32_t* total;
+    uint32_t* overflow;
+    uint32_t* flags;       // [n] output
+    Stats* stats;
+    float broad_pad;
+};
+
+// FK for n poses: T64, Ti, wbox for every link (exact FP64 sequence shared with the oracle).
+void launch_fk(const RobotDev* robot, int dof, int n_links, const float* poses, int n, double* T64, float4* Ti,
+               float4* wbox, hipStream_t s);
+// Broad phase: emit == false writes counts[task], emit == true writes candidates at offsets[task].
+void launch_broad(const DiscreteArgs& a, int stack, bool emit, hipStream_t s);
+// Narrow phase over *total candidates (grid-stride, sized for the device).
+void launch_narrow(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
+
+// ---- swept volumes ----
+struct SweepArgs {
+    const RobotDev* robot;
+    int robot_n_spheres_host;
+    SceneDev sc;
+    const float* poses;        // waypoints [n_wp_total][dof]
+    int n_traj, n_wp_total;
+    const uint32_t* offsets;   // linear: [n_traj+1]; quadratic: nullptr (fixed n_wp)
+    int n_wp, n_ctrl;          // quadratic
+    const double* W;           // quadratic [n_ctrl][n_wp]
+    double* centers64;         // [n_wp_total][S][3]
+    double* ctrl64;            // quadratic: [n_traj][S][n_ctrl][3]
+    const uint32_t* seg_traj;  // segment -> trajectory (linear) [n_seg]
+    const uint32_t* seg_wp;    // segment -> global index of its first waypoint (linear) [n_seg]
+    int n_seg;                 // linear: sum(L_i - 1); quadratic: n_traj * (n_ctrl - 2)
+    uint32_t* out;             // [n_traj]
+    uint32_t* out_seg;         // [n_seg] or nullptr
+    Stats* stats;
+};
+// Tubes of degree 1 (capsules between waypoints) or 2 (quadratic B-spline pieces) vs scene edges.
+void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s, const float4* c32,
+                  const float4* P32);
+void quadratic_weights(int nc, int nw, double* W);
+
+}  // namespace rtcd

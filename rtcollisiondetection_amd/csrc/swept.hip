@@ -1,0 +1,617 @@
+// Swept-sphere checks: piecewise-linear capsules and quadratic B-spline tubes.
+//
+// Reference path (RTCD/Robot/batchCurveRobot.h:346-403, RTCD/Robot/sweptVolume.cu:22-96,
+// RTCD/CollisionDetector/CCCuda.cu:37-77): FK per waypoint (robot.cu:155-200), sphere centres
+// (robot.cu:284-320), least-squares control points P = W X (quadratic), a per-batch OptiX curve GAS
+// over all capsules / B-spline pieces, then every obstacle loop edge traced as a ray with an any-hit
+// counter per trajectory.
+// Here nothing is rebuilt per batch: every (segment or piece, sphere) tube queries the static scene
+// edge LBVH with its box, and each candidate edge is tested against the tube in FP32 with a
+// certified band (closest point + convexity lower bound); undecided pairs are re-decided by the
+// FP64 Bernstein-subdivision test that the oracle uses, bit for bit.
+#include "band32.hpp"
+#include "ctx.hpp"
+#include "exact64.hpp"
+
+namespace rtcd {
+namespace {
+
+constexpr int kBS = 256;
+
+__device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
+__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator*(float s, float3 a) { return make_float3(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ float linf(float3 a) { return fmaxf(fmaxf(fabsf(a.x), fabsf(a.y)), fabsf(a.z)); }
+__device__ __forceinline__ float nl1(float3 a) { return fabsf(a.x) + fabsf(a.y) + fabsf(a.z); }
+__device__ __forceinline__ uint32_t load_flag(const uint32_t* f) {
+    return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------------- centres / fit
+__global__ __launch_bounds__(kBS) void k_centers(const RobotDev* __restrict__ R, const float4* __restrict__ sph,
+                                                const int32_t* __restrict__ sl, int S, const float* __restrict__ poses,
+                                                int nw, double* __restrict__ c64, float4* __restrict__ c32) {
+    int w = blockIdx.x * kBS + threadIdx.x;
+    if (w >= nw) return;
+    const int dof = R->dof, NL = dof + 1;
+    double T[12];
+    for (int k = 0; k < 12; ++k) T[k] = (double) R->base[k];
+    for (int l = 0; l < NL; ++l) {
+        if (l > 0) {
+            const int i = l - 1;
+            double s, c, L[12], Tn[12];
+            x64::sincos((double) poses[(size_t) w * dof + i], &s, &c);
+            {
+#pragma clang fp contract(off)
+                for (int k = 0; k < 12; ++k)
+                    L[k] = (c * (double) R->cmask[i * 12 + k] + s * (double) R->smask[i * 12 + k]) +
+                           (double) R->omask[i * 12 + k];
+            }
+            x64::affine_mul(T, L, Tn);
+            for (int k = 0; k < 12; ++k) T[k] = Tn[k];
+        }
+        for (int s = 0; s < S; ++s) {
+            if (sl[s] != l) continue;
+            const float4 sp = sph[s];
+            double p[3] = {sp.x, sp.y, sp.z}, o[3];
+            x64::xform(T, p, o);
+            const size_t idx = (size_t) w * S + s;
+            c64[idx * 3] = o[0];
+            c64[idx * 3 + 1] = o[1];
+            c64[idx * 3 + 2] = o[2];
+            c32[idx] = make_float4((float) o[0], (float) o[1], (float) o[2], sp.w);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBS) void k_seg_maps(const uint32_t* __restrict__ offs, int n_traj, int n_seg,
+                                                 uint32_t* __restrict__ seg_traj, uint32_t* __restrict__ seg_wp) {
+    int i = blockIdx.x * kBS + threadIdx.x;
+    if (i >= n_traj) return;
+    const uint32_t a = offs[i], b = offs[i + 1];
+    for (uint32_t w = a; w + 1 < b; ++w) {
+        const uint32_t g = w - (uint32_t) i;
+        if (g < (uint32_t) n_seg) {
+            seg_traj[g] = (uint32_t) i;
+            seg_wp[g] = w;
+        }
+    }
+}
+
+// P = W X for each (trajectory, sphere): same summation order as the oracle.
+__global__ __launch_bounds__(kBS) void k_fit(const double* __restrict__ c64, const float4* __restrict__ c32,
+                                            const double* __restrict__ W, int n_traj, int nw, int nc, int S,
+                                            double* __restrict__ P64, float4* __restrict__ P32) {
+    int id = blockIdx.x * kBS + threadIdx.x;
+    if (id >= n_traj * S) return;
+    const int t = id / S, s = id - t * S;
+    const float r = c32[(size_t) t * nw * S + s].w;
+    for (int j = 0; j < nc; ++j) {
+        double o[3];
+        for (int c = 0; c < 3; ++c) {
+#pragma clang fp contract(off)
+            double acc = W[(size_t) j * nw] * c64[(((size_t) t * nw + 0) * S + s) * 3 + c];
+            for (int k = 1; k < nw; ++k) acc = acc + W[(size_t) j * nw + k] * c64[(((size_t) t * nw + k) * S + s) * 3 + c];
+            o[c] = acc;
+        }
+        const size_t pi = ((size_t) t * S + s) * nc + j;
+        P64[pi * 3] = o[0];
+        P64[pi * 3 + 1] = o[1];
+        P64[pi * 3 + 2] = o[2];
+        P32[pi] = make_float4((float) o[0], (float) o[1], (float) o[2], r);
+    }
+}
+
+// ---------------------------------------------------------------------------------- FP32 certificate
+// Segment AB vs the capsule axis C0C1 (C0, C1 within `ein` per component of the exact axis, A, B
+// exact).  kHit: the exact minimum distance is certainly < r_hit; kMiss: certainly > r_miss.
+// Upper bound: F at the FP32 closest-point parameters (s, t).  Lower bound: F(s,t) + min over the
+// unit square of grad F . (x - (s,t)), valid because F = |C(t) - A - s D|^2 is convex.
+__device__ int cap_cert(float3 C0, float3 C1, float3 A, float3 B, float r_hit, float r_miss, float ein) {
+    const float3 E = C1 - C0, D = B - A, w0 = C0 - A;
+    const float a = dot(E, E), b = dot(E, D), c = dot(D, D), d = dot(E, w0), e = dot(D, w0);
+    float t = 0.f, s = 0.f;
+    const float den = a * c - b * b;
+    if (den > 1e-12f * a * c && c > 0.f) t = fminf(fmaxf((b * e - c * d) / den, 0.f), 1.f);
+    if (c > 0.f) {
+        s = (b * t + e) / c;
+        if (s < 0.f) { s = 0.f; t = a > 0.f ? fminf(fmaxf(-d / a, 0.f), 1.f) : 0.f; }
+        else if (s > 1.f) { s = 1.f; t = a > 0.f ? fminf(fmaxf((b - d) / a, 0.f), 1.f) : 0.f; }
+    } else {
+        t = a > 0.f ? fminf(fmaxf(-d / a, 0.f), 1.f) : 0.f;
+    }
+    const float3 v = w0 + t * E - s * D;
+    const float f = dot(v, v);
+    const float gt = 2.f * dot(E, v), gs = -2.f * dot(D, v);
+    const float corr = (gt > 0.f ? -gt * t : gt * (1.f - t)) + (gs > 0.f ? -gs * s : gs * (1.f - s));
+    // error bounds
+    const float K = linf(w0) + linf(E) + linf(D);
+    const float ev = 1.01f * (3.f * ein + 6.f * kEps32 * K);
+    const float Vn = nl1(v) + 3.f * ev;
+    const float ef = 2.f * Vn * ev + 3.f * ev * ev + 4.f * kEps32 * Vn * Vn;
+    const float En = nl1(E) + 6.f * ein, eE = 2.f * ein + kEps32 * En;
+    const float Dn = nl1(D), eD = kEps32 * Dn;
+    const float eg = 2.f * (En * ev + Vn * eE + 4.f * kEps32 * En * Vn) + 2.f * (Dn * ev + Vn * eD + 4.f * kEps32 * Dn * Vn) +
+                     2.f * kEps32 * (fabsf(gt) + fabsf(gs));
+    const float ub = f + 1.25f * ef;
+    const float lb = f + corr - 1.25f * (ef + eg) - 4.f * kEps32 * (f + fabsf(corr));
+    if (r_hit > 0.f && ub < r_hit * r_hit * (1.f - 4.f * kEps32)) return kHit;
+    if (lb > r_miss * r_miss * (1.f + 4.f * kEps32)) return kMiss;
+    return kUnsure;
+}
+
+// Quadratic Bezier (q0, q1, q2) vs segment: split into 4 sub-pieces (de Casteljau); each sub-piece
+// lies within h = |q1' - (q0' + q2')/2| / 2 of its chord.
+__device__ int quad_cert(float3 q0, float3 q1, float3 q2, float3 A, float3 B, float r, float ein) {
+    float3 pts[9];
+    {
+        const float3 m01 = 0.5f * (q0 + q1), m12 = 0.5f * (q1 + q2), mid = 0.5f * (m01 + m12);
+        const float3 l1 = 0.5f * (q0 + m01), l2 = 0.5f * (m01 + mid), lm = 0.5f * (l1 + l2);
+        const float3 r1 = 0.5f * (mid + m12), r2 = 0.5f * (m12 + q2), rm = 0.5f * (r1 + r2);
+        pts[0] = q0; pts[1] = l1; pts[2] = lm; pts[3] = l2; pts[4] = mid; pts[5] = r1; pts[6] = rm; pts[7] = r2; pts[8] = q2;
+    }
+    const float M = fmaxf(fmaxf(linf(q0), linf(q1)), linf(q2));
+    const float e2 = ein + 8.f * kEps32 * M;  // error of the sub-piece control points
+    bool all_miss = true;
+    for (int k = 0; k < 4; ++k) {
+        const float3 a = pts[2 * k], m = pts[2 * k + 1], b = pts[2 * k + 2];
+        const float3 dev = m - 0.5f * (a + b);
+        const float h = 0.5f * sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 2.f * 1.7321f * e2;
+        const int cls = cap_cert(a, b, A, B, r - h, r + h, e2);
+        if (cls == kHit) return kHit;
+        if (cls != kMiss) all_miss = false;
+    }
+    return all_miss ? kMiss : kUnsure;
+}
+
+// ---------------------------------------------------------------------------------- sweep kernel
+template <int DEG, int STACK, bool COUNT>
+__global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, const float4* __restrict__ c32, const float4* __restrict__ P32) {
+    __shared__ int stk[STACK * kBS];
+    int* st = stk + threadIdx.x;
+    const int S = a.robot->n_spheres;
+    const int npc = a.n_ctrl - 2;
+    const size_t n_items = (size_t) a.n_seg * S;
+    const size_t stride = (size_t) gridDim.x * kBS;
+    uint32_t c_nodes = 0, c_prims = 0, c_fp64 = 0, c_items = 0;
+    for (size_t it = (size_t) blockIdx.x * kBS + threadIdx.x; it < n_items; it += stride) {
+        const uint32_t g = (uint32_t) (it / S);
+        const int s = (int) (it - (size_t) g * S);
+        uint32_t traj;
+        float3 q[3];
+        float r;
+        if (DEG == 1) {
+            traj = a.seg_traj[g];
+            const uint32_t w = a.seg_wp[g];
+            const float4 x0 = c32[(size_t) w * S + s], x1 = c32[(size_t) (w + 1) * S + s];
+            q[0] = f3(x0);
+            q[1] = f3(x1);
+            r = x0.w;
+        } else {
+            traj = g / (uint32_t) npc;
+            const int piece = (int) (g - traj * npc);
+            const size_t pb = ((size_t) traj * S + s) * a.n_ctrl + piece;
+            const float4 p0 = P32[pb], p1 = P32[pb + 1], p2 = P32[pb + 2];
+            q[0] = 0.5f * (f3(p0) + f3(p1));
+            q[1] = f3(p1);
+            q[2] = 0.5f * (f3(p1) + f3(p2));
+            r = p0.w;
+        }
+        uint32_t* flag = a.out_seg ? a.out_seg + g : a.out + traj;
+        if (load_flag(flag)) continue;
+        if (COUNT) c_items++;
+        const float M = fmaxf(fmaxf(linf(q[0]), linf(q[1])), DEG == 2 ? linf(q[2]) : 0.f);
+        const float ein = (DEG == 1 ? 1.01f : 3.f) * kEps32 * M;
+        const float pad = r + 4.f * ein + 1e-6f * (1.f + a.sc.scale);
+        float lx = fminf(q[0].x, q[1].x), ly = fminf(q[0].y, q[1].y), lz = fminf(q[0].z, q[1].z);
+        float hx = fmaxf(q[0].x, q[1].x), hy = fmaxf(q[0].y, q[1].y), hz = fmaxf(q[0].z, q[1].z);
+        if (DEG == 2) {
+            lx = fminf(lx, q[2].x); ly = fminf(ly, q[2].y); lz = fminf(lz, q[2].z);
+            hx = fmaxf(hx, q[2].x); hy = fmaxf(hy, q[2].y); hz = fmaxf(hz, q[2].z);
+        }
+        lx -= pad; ly -= pad; lz -= pad; hx += pad; hy += pad; hz += pad;
+        int hit = 0;
+        if (a.sc.edge_has_root) {
+            int sp = 0, ni = 0, iter = 0;
+            for (;;) {
+                if ((++iter & 15) == 0 && load_flag(flag)) break;
+                const BNode nd = a.sc.edge_nodes[ni];
+                if (COUNT) c_nodes++;
+                bool h0 = nd.c0 >= 0 && !(lx > nd.hi0x || hx < nd.lo0x || ly > nd.hi0y || hy < nd.lo0y ||
+                                          lz > nd.hi0z || hz < nd.lo0z);
+                bool h1 = nd.c1 >= 0 && !(lx > nd.hi1x || hx < nd.lo1x || ly > nd.hi1y || hy < nd.lo1y ||
+                                          lz > nd.hi1z || hz < nd.lo1z);
+#pragma unroll
+                for (int side = 0; side < 2 && !hit; ++side) {
+                    const bool hb = side ? h1 : h0;
+                    const int cnt = side ? nd.n1 : nd.n0;
+                    const int start = side ? nd.c1 : nd.c0;
+                    if (hb && cnt > 0) {
+                        for (int k = 0; k < cnt && !hit; ++k) {
+                            const Edge e = a.sc.edge_prims[start + k];
+                            if (COUNT) c_prims++;
+                            const float3 A = f3(e.p), B = f3(e.q);
+                            int cls = DEG == 1 ? cap_cert(q[0], q[1], A, B, r, r, ein) : quad_cert(q[0], q[1], q[2], A, B, r, ein);
+                            if (cls == kHit) {
+                                hit = 1;
+                            } else if (cls == kUnsure) {
+                                if (COUNT) c_fp64++;
+                                double ctrl[9];
+                                if (DEG == 1) {
+                                    const uint32_t w = a.seg_wp[g];
+                                    for (int c = 0; c < 3; ++c) {
+                                        ctrl[c] = a.centers64[((size_t) w * S + s) * 3 + c];
+                                        ctrl[3 + c] = a.centers64[((size_t) (w + 1) * S + s) * 3 + c];
+                                    }
+                                } else {
+                                    const int piece = (int) (g - traj * npc);
+                                    const double* P = a.ctrl64 + (((size_t) traj * S + s) * a.n_ctrl + piece) * 3;
+                                    for (int c = 0; c < 3; ++c) {
+#pragma clang fp contract(off)
+                                        ctrl[c] = (P[c] + P[3 + c]) * 0.5;
+                                        ctrl[3 + c] = P[3 + c];
+                                        ctrl[6 + c] = (P[3 + c] + P[6 + c]) * 0.5;
+                                    }
+                                }
+                                double pa[3] = {e.p.x, e.p.y, e.p.z}, pb2[3] = {e.q.x, e.q.y, e.q.z};
+                                hit = x64::tube(DEG, ctrl, (double) r, pa, pb2);
+                            }
+                        }
+                        if (side) h1 = false; else h0 = false;
+                    }
+                }
+                if (hit) break;
+                if (h0 && h1) {
+                    if (sp >= STACK) break;
+                    st[sp * kBS] = nd.c1;
+                    ++sp;
+                    ni = nd.c0;
+                } else if (h0) {
+                    ni = nd.c0;
+                } else if (h1) {
+                    ni = nd.c1;
+                } else {
+                    if (sp == 0) break;
+                    --sp;
+                    ni = st[sp * kBS];
+                }
+            }
+        }
+        if (hit) {
+            atomicOr(a.out + traj, 1u);
+            if (a.out_seg) atomicOr(a.out_seg + g, 1u);
+        }
+    }
+    if (COUNT) {
+        atomicAdd(&a.stats->node_fetches, (unsigned long long) c_nodes);
+        atomicAdd(&a.stats->prim_tests, (unsigned long long) c_prims);
+        atomicAdd(&a.stats->fp64, (unsigned long long) c_fp64);
+        atomicAdd(&a.stats->candidates, (unsigned long long) c_items);
+        atomicAdd(&a.stats->edge_tests, (unsigned long long) c_prims);
+    }
+}
+
+int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void) hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+template <int DEG, int STACK>
+void sweep_t(const SweepArgs& a, const float4* c32, const float4* P32, bool count, hipStream_t s) {
+    const size_t items = (size_t) a.n_seg * a.robot_n_spheres_host;
+    unsigned nb = (unsigned) std::min<size_t>(blocks_for(items, kBS), (size_t) device_cus() * 8);
+    if (!nb) return;
+    if (count) hipLaunchKernelGGL((k_sweep<DEG, STACK, true>), dim3(nb), dim3(kBS), 0, s, a, c32, P32);
+    else hipLaunchKernelGGL((k_sweep<DEG, STACK, false>), dim3(nb), dim3(kBS), 0, s, a, c32, P32);
+    RTCD_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+// W = pinv(B) of the uniform quadratic B-spline sampled at u = linspace(0, nc - 2, nw).
+void quadratic_weights(int nc, int nw, double* W) {
+    const int nseg = nc - 2;
+    std::vector<double> B((size_t) nw * nc, 0.0);
+    for (int i = 0; i < nw; ++i) {
+        double u = nw > 1 ? (double) nseg * i / (nw - 1) : 0.0;
+        int s = std::min((int) std::floor(u), nseg - 1);
+        double t = u - s;
+        B[(size_t) i * nc + s] = (1 - t) * (1 - t) / 2;
+        B[(size_t) i * nc + s + 1] = (-2 * t * t + 2 * t + 1) / 2;
+        B[(size_t) i * nc + s + 2] = t * t / 2;
+    }
+    // W = (B^T B)^{-1} B^T by Gauss-Jordan with partial pivoting on [B^T B | B^T]
+    const int cols = nc + nw;
+    std::vector<double> M((size_t) nc * cols, 0.0);
+    for (int r = 0; r < nc; ++r) {
+        for (int c = 0; c < nc; ++c) {
+            double acc = 0;
+            for (int k = 0; k < nw; ++k) acc += B[(size_t) k * nc + r] * B[(size_t) k * nc + c];
+            M[(size_t) r * cols + c] = acc;
+        }
+        for (int k = 0; k < nw; ++k) M[(size_t) r * cols + nc + k] = B[(size_t) k * nc + r];
+    }
+    for (int col = 0; col < nc; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < nc; ++r)
+            if (std::fabs(M[(size_t) r * cols + col]) > std::fabs(M[(size_t) piv * cols + col])) piv = r;
+        if (std::fabs(M[(size_t) piv * cols + col]) < 1e-300) throw std::runtime_error("singular B-spline fit");
+        if (piv != col)
+            for (int c = 0; c < cols; ++c) std::swap(M[(size_t) piv * cols + c], M[(size_t) col * cols + c]);
+        const double inv = 1.0 / M[(size_t) col * cols + col];
+        for (int c = 0; c < cols; ++c) M[(size_t) col * cols + c] *= inv;
+        for (int r = 0; r < nc; ++r) {
+            if (r == col) continue;
+            const double f = M[(size_t) r * cols + col];
+            if (f != 0.0)
+                for (int c = 0; c < cols; ++c) M[(size_t) r * cols + c] -= f * M[(size_t) col * cols + c];
+        }
+    }
+    for (int r = 0; r < nc; ++r)
+        for (int k = 0; k < nw; ++k) W[(size_t) r * nw + k] = M[(size_t) r * cols + nc + k];
+}
+
+void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s, const float4* c32,
+                  const float4* P32) {
+    if (degree == 1) {
+        if (stack <= 32) sweep_t<1, 32>(a, c32, P32, count, s);
+        else sweep_t<1, 64>(a, c32, P32, count, s);
+    } else {
+        if (stack <= 32) sweep_t<2, 32>(a, c32, P32, count, s);
+        else sweep_t<2, 64>(a, c32, P32, count, s);
+    }
+}
+
+}  // namespace rtcd
+
+// ============================================================================================ C ABI
+namespace {
+
+void enqueue_centers(rtcd_ctx* c, const float* d_poses, int nw, hipStream_t s) {
+    SweepBuffers& b = c->sweep;
+    const int S = b.n_spheres;
+    check_budget(c, 0);
+    b.c64.reserve((size_t) nw * S * 3);
+    b.c32.reserve((size_t) nw * S);
+    if (nw > 0 && S > 0) {
+        hipLaunchKernelGGL(k_centers, dim3(blocks_for(nw, kBS)), dim3(kBS), 0, s, c->d_robot.p, b.spheres,
+                           b.d_sphere_link.p, S, d_poses, nw, b.c64.p, b.c32.p);
+        RTCD_LAUNCH_CHECK();
+    }
+}
+
+SweepArgs base_args(rtcd_ctx* c, SceneRes& sc, uint32_t* d_out, uint32_t* d_seg) {
+    SweepArgs a{};
+    a.robot = c->d_robot.p;
+    a.robot_n_spheres_host = c->sweep.n_spheres;
+    a.sc = sc.dev();
+    a.centers64 = c->sweep.c64.p;
+    a.out = d_out;
+    a.out_seg = d_seg;
+    c->sweep.stats.reserve(1);
+    a.stats = c->sweep.stats.p;
+    return a;
+}
+
+void sweep_begin(rtcd_ctx* c, hipStream_t s) {
+    if (!c->ev0) {
+        RTCD_HIP(hipEventCreate(&c->ev0));
+        RTCD_HIP(hipEventCreate(&c->ev1));
+    }
+    c->sweep.stats.reserve(1);
+    if (c->counting) fill_u32(reinterpret_cast<uint32_t*>(c->sweep.stats.p), sizeof(Stats) / 4, 0u, s);
+    RTCD_HIP(hipEventRecord(c->ev0, s));
+}
+
+void sweep_end(rtcd_ctx* c, hipStream_t s, int depth) {
+    RTCD_HIP(hipEventRecord(c->ev1, s));
+    c->timing_pending = true;
+    c->last.max_depth = (uint32_t) depth;
+    c->last.max_stack = (uint32_t) (depth + 2 <= 32 ? 32 : 64);
+}
+
+void sweep_finish(rtcd_ctx* c, hipStream_t s) {
+    RTCD_HIP(hipStreamSynchronize(s));
+    if (c->timing_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last.last_ms = ms;
+        c->timing_pending = false;
+    }
+    if (c->counting) {
+        Stats st = c->sweep.stats.download(1, s)[0];
+        c->last.node_fetches = st.node_fetches;
+        c->last.prim_tests = st.prim_tests;
+        c->last.fp64_redecisions = st.fp64;
+        c->last.candidates = st.candidates;
+        c->last.edge_tests = st.edge_tests;
+    }
+}
+
+void enqueue_linear(rtcd_ctx* c, SceneRes& sc, const float* d_traj, const uint32_t* d_offs, uint32_t n_traj,
+                    uint32_t nw, uint32_t* d_out, uint32_t* d_seg, hipStream_t s) {
+    SweepBuffers& b = c->sweep;
+    const int n_seg = (int) (nw - n_traj);
+    sweep_begin(c, s);
+    fill_u32(d_out, n_traj, 0u, s);
+    if (d_seg && n_seg > 0) fill_u32(d_seg, (size_t) n_seg, 0u, s);
+    enqueue_centers(c, d_traj, (int) nw, s);
+    b.seg_traj.reserve(std::max(n_seg, 1));
+    b.seg_wp.reserve(std::max(n_seg, 1));
+    hipLaunchKernelGGL(k_seg_maps, dim3(blocks_for(n_traj, kBS)), dim3(kBS), 0, s, d_offs, (int) n_traj, n_seg,
+                       b.seg_traj.p, b.seg_wp.p);
+    RTCD_LAUNCH_CHECK();
+    SweepArgs a = base_args(c, sc, d_out, d_seg);
+    a.n_traj = (int) n_traj;
+    a.n_wp_total = (int) nw;
+    a.seg_traj = b.seg_traj.p;
+    a.seg_wp = b.seg_wp.p;
+    a.n_seg = n_seg;
+    a.n_ctrl = 2;
+    const int depth = sc.edge_bvh.depth;
+    if (n_seg > 0) launch_sweep(a, 1, depth + 2 <= 32 ? 32 : 64, c->counting, s, b.c32.p, nullptr);
+    sweep_end(c, s, depth);
+}
+
+void enqueue_quadratic(rtcd_ctx* c, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw, uint32_t nc,
+                       const double* W, uint32_t* d_out, uint32_t* d_seg, hipStream_t s) {
+    SweepBuffers& b = c->sweep;
+    const int S = b.n_spheres;
+    const int npc = (int) nc - 2;
+    std::vector<double> Wh((size_t) nc * nw);
+    if (W) std::copy(W, W + Wh.size(), Wh.begin());
+    else quadratic_weights((int) nc, (int) nw, Wh.data());
+    sweep_begin(c, s);
+    b.W64.reserve(Wh.size());
+    RTCD_HIP(hipMemcpyAsync(b.W64.p, Wh.data(), sizeof(double) * Wh.size(), hipMemcpyHostToDevice, s));
+    fill_u32(d_out, n_traj, 0u, s);
+    if (d_seg) fill_u32(d_seg, (size_t) n_traj * npc, 0u, s);
+    enqueue_centers(c, d_wp, (int) (n_traj * nw), s);
+    b.P64.reserve((size_t) n_traj * S * nc * 3);
+    b.P32.reserve((size_t) n_traj * S * nc);
+    if (n_traj * S > 0) {
+        hipLaunchKernelGGL(k_fit, dim3(blocks_for((size_t) n_traj * S, kBS)), dim3(kBS), 0, s, b.c64.p, b.c32.p,
+                           b.W64.p, (int) n_traj, (int) nw, (int) nc, S, b.P64.p, b.P32.p);
+        RTCD_LAUNCH_CHECK();
+    }
+    SweepArgs a = base_args(c, sc, d_out, d_seg);
+    a.n_traj = (int) n_traj;
+    a.n_wp = (int) nw;
+    a.n_ctrl = (int) nc;
+    a.W = b.W64.p;
+    a.ctrl64 = b.P64.p;
+    a.n_seg = (int) n_traj * npc;
+    const int depth = sc.edge_bvh.depth;
+    if (a.n_seg > 0) launch_sweep(a, 2, depth + 2 <= 32 ? 32 : 64, c->counting, s, b.c32.p, b.P32.p);
+    sweep_end(c, s, depth);
+}
+
+rtcd_status sweep_precheck(rtcd_ctx* ctx, uint32_t scene_id, SceneRes** sc) {
+    if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
+    if (!ctx->robot_loaded) return fail(RTCD_ERR_NOT_READY, "robot not loaded");
+    *sc = find_scene(ctx, scene_id);
+    if (!*sc) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
+    return RTCD_OK;
+}
+
+}  // namespace
+
+namespace rtcd {
+void sweep_robot_loaded(SweepBuffers& b, const float4* sph, int n, hipStream_t s) {
+    b.spheres = sph;
+    b.n_spheres = n;
+    b.d_sphere_link.alloc(std::max(n, 1));
+    if (n) b.d_sphere_link.upload(b.sphere_link, s);
+    RTCD_HIP(hipStreamSynchronize(s));
+}
+}  // namespace rtcd
+
+extern "C" {
+
+rtcd_status rtcd_quadratic_weights(uint32_t n_ctrl, uint32_t n_wp, double* W) {
+    if (!W || n_ctrl < 3 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)
+        return fail(RTCD_ERR_INVALID_ARG, "need 3 <= n_ctrl <= 64, n_ctrl <= n_wp <= 4096, W non-null");
+    return guarded([&]() -> rtcd_status {
+        quadratic_weights((int) n_ctrl, (int) n_wp, W);
+        return RTCD_OK;
+    });
+}
+
+rtcd_status rtcd_check_linear_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_traj, const uint32_t* d_offsets,
+                                     uint32_t n_traj, uint32_t n_wp_total, uint32_t* d_out, uint32_t* d_out_seg,
+                                     void* stream) {
+    SceneRes* sc = nullptr;
+    if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
+    if (n_traj && (!d_traj || !d_offsets || !d_out)) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
+    if (n_wp_total < n_traj) return fail(RTCD_ERR_INVALID_ARG, "every trajectory needs at least one waypoint");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return guarded([&]() -> rtcd_status {
+        RTCD_HIP(hipSetDevice(ctx->cfg.device));
+        hipStream_t s = stream ? (hipStream_t) stream : ctx->stream;
+        if (n_traj == 0) return RTCD_OK;
+        enqueue_linear(ctx, *sc, d_traj, d_offsets, n_traj, n_wp_total, d_out, d_out_seg, s);
+        return RTCD_OK;
+    });
+}
+
+rtcd_status rtcd_check_linear(rtcd_ctx* ctx, uint32_t scene_id, const float* traj, const uint32_t* offsets,
+                              uint32_t n_traj, uint32_t* out, uint32_t* out_seg) {
+    SceneRes* sc = nullptr;
+    if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
+    if (n_traj == 0) return RTCD_OK;
+    if (!traj || !offsets || !out) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
+    for (uint32_t i = 0; i < n_traj; ++i)
+        if (offsets[i + 1] <= offsets[i]) return fail(RTCD_ERR_INVALID_ARG, "every trajectory needs at least one waypoint");
+    if (offsets[0] != 0) return fail(RTCD_ERR_INVALID_ARG, "offsets[0] must be 0");
+    const uint32_t nw = offsets[n_traj];
+    if (!finite_all(traj, (size_t) nw * ctx->robot.dof)) return fail(RTCD_ERR_INVALID_ARG, "non-finite joint value");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return guarded([&]() -> rtcd_status {
+        RTCD_HIP(hipSetDevice(ctx->cfg.device));
+        hipStream_t s = ctx->stream;
+        SweepBuffers& b = ctx->sweep;
+        const uint32_t n_seg = nw - n_traj;
+        b.h_poses.upload(traj, (size_t) nw * ctx->robot.dof, s);
+        b.h_offsets.upload(offsets, n_traj + 1, s);
+        b.h_out.reserve(n_traj);
+        b.h_seg.reserve(std::max<uint32_t>(n_seg, 1));
+        enqueue_linear(ctx, *sc, b.h_poses.p, b.h_offsets.p, n_traj, nw, b.h_out.p, out_seg ? b.h_seg.p : nullptr, s);
+        RTCD_HIP(hipMemcpyAsync(out, b.h_out.p, sizeof(uint32_t) * n_traj, hipMemcpyDeviceToHost, s));
+        if (out_seg && n_seg)
+            RTCD_HIP(hipMemcpyAsync(out_seg, b.h_seg.p, sizeof(uint32_t) * n_seg, hipMemcpyDeviceToHost, s));
+        sweep_finish(ctx, s);
+        return RTCD_OK;
+    });
+}
+
+rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_wp, uint32_t n_traj,
+                                        uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
+                                        uint32_t* d_out_seg, void* stream) {
+    SceneRes* sc = nullptr;
+    if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
+    if (n_ctrl < 3 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)
+        return fail(RTCD_ERR_INVALID_ARG, "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
+    if (n_traj && (!d_wp || !d_out)) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return guarded([&]() -> rtcd_status {
+        RTCD_HIP(hipSetDevice(ctx->cfg.device));
+        hipStream_t s = stream ? (hipStream_t) stream : ctx->stream;
+        if (n_traj == 0) return RTCD_OK;
+        enqueue_quadratic(ctx, *sc, d_wp, n_traj, n_wp, n_ctrl, W, d_out, d_out_seg, s);
+        return RTCD_OK;
+    });
+}
+
+rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* wp, uint32_t n_traj, uint32_t n_wp,
+                                 uint32_t n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
+    SceneRes* sc = nullptr;
+    if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
+    if (n_ctrl < 3 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)
+        return fail(RTCD_ERR_INVALID_ARG, "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
+    if (n_traj == 0) return RTCD_OK;
+    if (!wp || !out) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
+    if (!finite_all(wp, (size_t) n_traj * n_wp * ctx->robot.dof)) return fail(RTCD_ERR_INVALID_ARG, "non-finite joint value");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return guarded([&]() -> rtcd_status {
+        RTCD_HIP(hipSetDevice(ctx->cfg.device));
+        hipStream_t s = ctx->stream;
+        SweepBuffers& b = ctx->sweep;
+        const size_t npc = n_ctrl - 2;
+        b.h_poses.upload(wp, (size_t) n_traj * n_wp * ctx->robot.dof, s);
+        b.h_out.reserve(n_traj);
+        b.h_seg.reserve(n_traj * npc);
+        enqueue_quadratic(ctx, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, out_seg ? b.h_seg.p : nullptr, s);
+        RTCD_HIP(hipMemcpyAsync(out, b.h_out.p, sizeof(uint32_t) * n_traj, hipMemcpyDeviceToHost, s));
+        if (out_seg) RTCD_HIP(hipMemcpyAsync(out_seg, b.h_seg.p, sizeof(uint32_t) * n_traj * npc, hipMemcpyDeviceToHost, s));
+        sweep_finish(ctx, s);
+        return RTCD_OK;
+    });
+}
+
+}  // extern "C"

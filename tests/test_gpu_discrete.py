@@ -1,0 +1,161 @@
+"""Parity of the discrete checks (HIP, through the C ABI) with the FP64 oracle: 0 flag mismatches."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rtcollisiondetection_amd import samplers, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def ctx(panda):
+    from rtcollisiondetection_amd.detector import Context
+    c = Context()
+    c.load_robot(panda)
+    yield c
+    c.close()
+
+
+def _mismatch(a, b):
+    return int(np.sum(a.astype(bool) != b.astype(bool)))
+
+
+@pytest.mark.parametrize('direction', [1, 2, 3])
+def test_boxes8_1024_poses(ctx, panda, orc_panda, direction):
+    sc = scenes.scene_boxes8()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    q = samplers.uniform_poses(1024, panda.lower, panda.upper, seed=0)
+    got = ctx.check_discrete(sid, q, direction)
+    ref = O.check_discrete(orc_panda, O.scene_mesh(sc), q, direction)
+    print(f'dir {direction}: hits {int(ref.sum())}/1024')
+    assert _mismatch(got, ref) == 0
+    ctx.destroy_scene(sid)
+
+
+@pytest.mark.parametrize('direction', [1, 2, 3])
+def test_near_boxes_high_hit_rate(ctx, panda, orc_panda, direction):
+    sc = scenes.random_boxes(8, seed=1, center_lo=(0.2, -0.4, 0.1), center_hi=(0.6, 0.4, 0.8))
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    q = samplers.uniform_poses(2048, panda.lower, panda.upper, seed=7)
+    got = ctx.check_discrete(sid, q, direction)
+    ref = O.check_discrete(orc_panda, O.scene_mesh(sc), q, direction)
+    assert 0 < ref.sum() < len(ref)
+    assert _mismatch(got, ref) == 0
+    ctx.destroy_scene(sid)
+
+
+def test_dense_scene_two_way(ctx, panda, orc_panda):
+    sc = scenes.scene_dense()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    assert ctx.scene_info(sid) == (50768, 76152)
+    q = samplers.uniform_poses(1024, panda.lower, panda.upper, seed=1)
+    m = O.scene_mesh(sc)
+    for d in (1, 2, 3):
+        got = ctx.check_discrete(sid, q, d)
+        ref = O.check_discrete(orc_panda, m, q, d)
+        assert _mismatch(got, ref) == 0, d
+    ctx.destroy_scene(sid)
+
+
+def test_reference_pose_fixture_on_shelf(ctx, panda, orc_panda):
+    import os
+    poses = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'traj_poses_panda_256.npy'))
+    sc = scenes.scene_shelf()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    got = ctx.check_discrete(sid, poses, 3)
+    ref = O.check_discrete(orc_panda, O.scene_mesh(sc), poses, 3)
+    assert _mismatch(got, ref) == 0
+    ctx.destroy_scene(sid)
+
+
+def _self_scene(panda, q0, links=(3, 4, 5, 6, 7)):
+    """World copy of link meshes at pose q0 (fp32-rounded): edges coincide with robot geometry."""
+    T = O.fk(panda, q0)
+    meshes = []
+    for l in links:
+        R = T[l].reshape(3, 4)
+        v = (panda.link_verts[l].astype(np.float64) @ R[:, :3].T + R[:, 3]).astype(np.float32)
+        meshes.append((v, panda.link_tris[l]))
+    return scenes.merge(meshes)
+
+
+def test_degenerate_self_overlap_exercises_fp64(ctx, panda, orc_panda):
+    """Obstacles that coincide with the robot's own surface: many tests fall inside the FP32 band
+    and are re-decided in FP64; the decisions must still match the oracle exactly."""
+    q0 = samplers.uniform_poses(1, panda.lower, panda.upper, seed=99)[0]
+    sc = _self_scene(panda, q0)
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    rng = np.random.default_rng(3)
+    q = np.repeat(q0[None], 64, axis=0)
+    q[1:] += rng.normal(0, 1e-6, size=(63, 7)).astype(np.float32)
+    q = q.astype(np.float32)
+    m = O.scene_mesh(sc)
+    ctx.set_counting(True)
+    try:
+        for d in (1, 2, 3):
+            got = ctx.check_discrete(sid, q, d)
+            ref = O.check_discrete(orc_panda, m, q, d)
+            st = ctx.stats()
+            print(f'dir {d}: hits {int(ref.sum())}/64, fp64 re-decisions {st["fp64_redecisions"]}')
+            assert _mismatch(got, ref) == 0, d
+        assert st['fp64_redecisions'] > 0
+    finally:
+        ctx.set_counting(False)
+    ctx.destroy_scene(sid)
+
+
+def test_skip_base_false(panda, orc_panda):
+    from rtcollisiondetection_amd.detector import Context
+    c = Context(skip_base_link=False)
+    c.load_robot(panda)
+    v, t = scenes.box_mesh([-0.05, -0.05, 0.05], [0.05, 0.05, 0.5])
+    sid = c.create_scene(v, t)
+    q = samplers.uniform_poses(64, panda.lower, panda.upper, seed=4)
+    got = c.check_discrete(sid, q, 3)
+    ref = O.check_discrete(orc_panda, O.Mesh(v, t), q, 3, skip_base=False)
+    assert ref.all() and _mismatch(got, ref) == 0
+    c.close()
+
+
+def test_empty_and_single(ctx, panda, orc_panda):
+    sc = scenes.scene_boxes8()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    assert len(ctx.check_discrete(sid, np.zeros((0, 7), np.float32))) == 0
+    q = samplers.uniform_poses(1, panda.lower, panda.upper, seed=2)
+    assert _mismatch(ctx.check_discrete(sid, q, 3), O.check_discrete(orc_panda, O.scene_mesh(sc), q, 3)) == 0
+    empty = ctx.create_scene(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32))
+    assert ctx.check_discrete(empty, q, 3).sum() == 0
+    ctx.destroy_scene(sid)
+    ctx.destroy_scene(empty)
+
+
+def test_invalid_inputs(ctx, panda):
+    from rtcollisiondetection_amd._lib import RtcdError
+    sc = scenes.scene_boxes8()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    q = samplers.uniform_poses(4, panda.lower, panda.upper, seed=2)
+    q[1, 3] = np.nan
+    with pytest.raises(RtcdError) as e:
+        ctx.check_discrete(sid, q, 3)
+    assert e.value.status == 1
+    with pytest.raises(RtcdError) as e:
+        ctx.check_discrete(sid + 1000, q[:1], 3)
+    assert e.value.status == 6
+    with pytest.raises(RtcdError):
+        ctx.check_discrete(sid, q[:1], 5)
+    ctx.destroy_scene(sid)
+
+
+def test_device_api_matches_host_api(ctx, panda):
+    import torch
+    sc = scenes.scene_dense()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    q = samplers.uniform_poses(4096, panda.lower, panda.upper, seed=5)
+    host = ctx.check_discrete(sid, q, 3)
+    dq = torch.from_numpy(q).cuda()
+    dout = torch.zeros(len(q), dtype=torch.int32, device='cuda')
+    ctx.check_discrete_device(sid, dq, dout, 3)
+    ctx.sync()
+    assert np.array_equal(dout.cpu().numpy().astype(np.uint32), host)
+    ctx.destroy_scene(sid)

@@ -1,0 +1,100 @@
+"""Parity of the swept-sphere checks (HIP, through the C ABI) with the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from rtcollisiondetection_amd import samplers, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def ctx(panda):
+    from rtcollisiondetection_amd.detector import Context
+    c = Context()
+    c.load_robot(panda)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope='module')
+def shelf(ctx):
+    sc = scenes.scene_shelf()
+    return sc, ctx.create_scene(sc.verts, sc.tris), O.scene_mesh(sc)
+
+
+def test_linear_uneven_lengths(ctx, panda, orc_panda, shelf):
+    sc, sid, m = shelf
+    traj, offs = samplers.random_walk_trajectories(256, panda.lower, panda.upper, lengths=(8, 64), seed=4)
+    got, gseg = ctx.check_linear(sid, traj, offs, with_segments=True)
+    ref, rseg = O.check_linear(orc_panda, m, traj, offs, with_segments=True)
+    print(f'linear: {int(ref.sum())}/256 trajectories, {int(rseg.sum())}/{len(rseg)} segments collide')
+    assert 0 < ref.sum() < len(ref)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(gseg, rseg)
+    # flags without per-segment output
+    assert np.array_equal(ctx.check_linear(sid, traj, offs), ref)
+
+
+def test_linear_single_waypoint_and_fixed_length(ctx, panda, orc_panda, shelf):
+    sc, sid, m = shelf
+    traj, offs = samplers.random_walk_trajectories(64, panda.lower, panda.upper, lengths=32, seed=8)
+    assert np.array_equal(ctx.check_linear(sid, traj, offs), O.check_linear(orc_panda, m, traj, offs))
+    one = traj[:1]
+    assert ctx.check_linear(sid, one, np.array([0, 1], np.uint32))[0] == 0  # no segment, no capsule
+
+
+def test_quadratic_matches_oracle(ctx, panda, orc_panda, shelf):
+    sc, sid, m = shelf
+    wp, _ = samplers.bspline_trajectories(128, panda.lower, panda.upper, n_ctrl=8, n_wp=16, seed=3)
+    W = samplers.quadratic_fit_weights(8, 16)
+    got, gseg = ctx.check_quadratic(sid, wp, 8, W, with_segments=True)
+    ref, rseg = O.check_quadratic(orc_panda, m, wp, W, with_segments=True)
+    print(f'quadratic: {int(ref.sum())}/128 trajectories, {int(rseg.sum())}/{rseg.size} pieces collide')
+    assert 0 < ref.sum() < len(ref)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(gseg, rseg)
+
+
+def test_quadratic_default_weights(ctx, panda, shelf):
+    sc, sid, m = shelf
+    wp, _ = samplers.bspline_trajectories(32, panda.lower, panda.upper, seed=6)
+    a = ctx.check_quadratic(sid, wp, 8, None)
+    b = ctx.check_quadratic(sid, wp, 8, samplers.quadratic_fit_weights(8, 16))
+    assert np.array_equal(a, b)
+
+
+def _tangent_scene(panda, traj):
+    """Edges placed at exactly one sphere radius from the swept sphere centres (fp32-rounded):
+    the FP32 certificate cannot decide these and the FP64 test must agree with the oracle."""
+    cen = O.sphere_centers(O.Robot(panda), traj[0])
+    verts, tris = [], []
+    for s in range(0, len(cen), 5):
+        c = cen[s]
+        r = float(panda.spheres[s, 3])
+        # a thin triangle whose first edge is tangent to the sphere around the first waypoint
+        a = c + np.array([r, -0.2, 0.0])
+        b = c + np.array([r, 0.2, 0.0])
+        d = c + np.array([r + 0.3, 0.0, 0.0])
+        base = len(verts)
+        verts += [a, b, d]
+        tris.append([base, base + 1, base + 2])
+    return np.array(verts, np.float32), np.array(tris, np.int32)
+
+
+def test_linear_tangent_edges_exercise_fp64(ctx, panda, orc_panda):
+    traj, offs = samplers.random_walk_trajectories(16, panda.lower, panda.upper, lengths=2, max_step=0.0, seed=12)
+    v, t = _tangent_scene(panda, traj)
+    sid = ctx.create_scene(v, t)
+    m = O.Mesh(v, t)
+    ctx.set_counting(True)
+    try:
+        got, gseg = ctx.check_linear(sid, traj, offs, with_segments=True)
+        st = ctx.stats()
+    finally:
+        ctx.set_counting(False)
+    ref, rseg = O.check_linear(orc_panda, m, traj, offs, with_segments=True)
+    print('tangent: fp64 re-decisions', st['fp64_redecisions'], 'hits', int(ref.sum()))
+    assert np.array_equal(got, ref) and np.array_equal(gseg, rseg)
+    assert st['fp64_redecisions'] > 0
+    ctx.destroy_scene(sid)
