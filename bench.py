@@ -160,6 +160,7 @@ def main():
         t = time.perf_counter()
         ref = O.check_discrete(orb, m, qs, 3, nthreads=cores)
         cdt = time.perf_counter() - t
+        run(3)
         got = dout.cpu().numpy()[:sample].astype(np.uint32)
         mism = int(np.sum(ref.astype(bool) != got.astype(bool)))
         cpu = {'value': round(sample / cdt, 1), 'unit': 'poses/s', 'cores': cores, 'kind': 'port',

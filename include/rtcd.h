@@ -125,6 +125,36 @@ rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const 
 /* W = pinv(B) for the uniform quadratic B-spline, [n_ctrl][n_wp] row-major. */
 rtcd_status rtcd_quadratic_weights(uint32_t n_ctrl, uint32_t n_wp, double* W);
 
+/* ---- Serving: continuous batching over resident scenes (no RTCD counterpart; RTCD's closest
+ * analogue is the NSTREAM ThreadPool of CollisionDetector::check, CollisionDetector.h:396-433). ---- */
+enum { RTCD_REQ_DISCRETE = 1, RTCD_REQ_LINEAR = 2, RTCD_REQ_QUADRATIC = 3 };
+typedef struct {
+    int32_t kind;            /* RTCD_REQ_* */
+    int32_t direction;       /* discrete: RTCD_O2R / RTCD_R2O / RTCD_TWO_WAY */
+    uint32_t scene_id;
+    uint32_t n;              /* poses or trajectories */
+    const float* input;      /* host: poses [n][dof]; linear waypoints [offsets[n]][dof]; quadratic [n][n_wp][dof] */
+    const uint32_t* offsets; /* linear: [n+1] */
+    uint32_t n_wp, n_ctrl;   /* quadratic (default fit weights) */
+    uint32_t* out;           /* host [n]; written before the request completes */
+} rtcd_request;
+typedef struct {
+    uint64_t completed, failed;
+    double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> flags in the caller's buffer */
+    double span_ms;                         /* first submit -> last completion */
+    uint32_t queued_max;
+} rtcd_serve_stats;
+/* Starts the worker: `slots` concurrent requests (each with its own stream and buffers), admitted
+ * while their estimated device bytes fit `mem_budget` (0 = 8 GiB).  input/offsets are copied at
+ * launch; the caller keeps them and `out` alive until rtcd_wait returns. */
+rtcd_status rtcd_serve_start(rtcd_ctx* ctx, uint32_t slots, uint64_t mem_budget);
+rtcd_status rtcd_serve_stop(rtcd_ctx* ctx);
+rtcd_status rtcd_submit(rtcd_ctx* ctx, const rtcd_request* req, uint64_t* id);
+rtcd_status rtcd_poll(rtcd_ctx* ctx, uint64_t id, int32_t* done);
+/* Blocks until request `id` is done; returns its status and frees its record. */
+rtcd_status rtcd_wait(rtcd_ctx* ctx, uint64_t id, double* latency_ms);
+rtcd_status rtcd_serve_get_stats(rtcd_ctx* ctx, rtcd_serve_stats* stats, int32_t reset);
+
 /* Counting mode: when enabled, check calls also count node fetches / primitive tests (slower). */
 rtcd_status rtcd_set_counting(rtcd_ctx* ctx, int32_t enable);
 rtcd_status rtcd_get_stats(rtcd_ctx* ctx, rtcd_stats* stats);

@@ -19,7 +19,9 @@ O2R, R2O, TWO_WAY = 1, 2, 3
 EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 'rtcd_scene_create',
            'rtcd_scene_destroy', 'rtcd_scene_info', 'rtcd_check_discrete', 'rtcd_check_discrete_device', 'rtcd_sync',
            'rtcd_check_linear', 'rtcd_check_linear_device', 'rtcd_check_quadratic', 'rtcd_check_quadratic_device',
-           'rtcd_quadratic_weights', 'rtcd_set_counting', 'rtcd_get_stats']
+           'rtcd_quadratic_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
+           'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_serve_get_stats']
+REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC = 1, 2, 3
 
 
 class RtcdError(RuntimeError):
@@ -46,6 +48,18 @@ class Stats(C.Structure):
                 ('max_depth', C.c_uint32), ('edge_tests', C.c_uint64), ('stage_ms', C.c_double * 5)]
 
 
+class Request(C.Structure):
+    _fields_ = [('kind', C.c_int32), ('direction', C.c_int32), ('scene_id', C.c_uint32), ('n', C.c_uint32),
+                ('input', C.c_void_p), ('offsets', C.c_void_p), ('n_wp', C.c_uint32), ('n_ctrl', C.c_uint32),
+                ('out', C.c_void_p)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [('completed', C.c_uint64), ('failed', C.c_uint64), ('p50_ms', C.c_double), ('p99_ms', C.c_double),
+                ('mean_ms', C.c_double), ('max_ms', C.c_double), ('span_ms', C.c_double),
+                ('queued_max', C.c_uint32)]
+
+
 _lib = None
 
 
@@ -54,6 +68,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f'{LIB_PATH} is missing: build it first (python -m rtcollisiondetection_amd.build)')
+        # PyTorch ships its own libamdhip64 (same soname).  Load it first so that this library binds to
+        # the same HIP runtime instance and device pointers from torch tensors are valid here.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
         L.rtcd_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
@@ -74,6 +94,12 @@ def lib():
         L.rtcd_quadratic_weights.argtypes = [u32, u32, vp]
         L.rtcd_set_counting.argtypes = [vp, i32]
         L.rtcd_get_stats.argtypes = [vp, C.POINTER(Stats)]
+        L.rtcd_serve_start.argtypes = [vp, u32, C.c_uint64]
+        L.rtcd_serve_stop.argtypes = [vp]
+        L.rtcd_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
+        L.rtcd_poll.argtypes = [vp, C.c_uint64, C.POINTER(i32)]
+        L.rtcd_wait.argtypes = [vp, C.c_uint64, C.POINTER(C.c_double)]
+        L.rtcd_serve_get_stats.argtypes = [vp, C.POINTER(ServeStats), i32]
         for name in EXPORTS:
             if name not in ('rtcd_destroy', 'rtcd_last_error'):
                 getattr(L, name).restype = C.c_int32

@@ -12,44 +12,55 @@ thread_local std::string g_err;
 }  // namespace rtcd
 
 namespace {
-
 int stack_for(int depth) { return depth + 2 <= 32 ? 32 : 64; }
 
-// Enqueue the discrete pipeline on stream s.
-void enqueue_discrete(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, uint32_t n, uint32_t* d_out,
-                      hipStream_t s) {
+void budget_check(const rtcd_ctx* c, const Workspace& ws, size_t budget, size_t extra) {
+    if (budget && ws.bytes() + extra > budget)
+        throw HipError(hipErrorOutOfMemory, "request needs more than its memory budget of " + std::to_string(budget) +
+                                                " bytes");
+    if (&ws == &c->main) check_budget(c, extra);
+}
+}  // namespace
+
+namespace rtcd {
+
+size_t discrete_bytes(const rtcd_ctx* c, uint32_t n) {
+    const size_t NL = (size_t) c->robot.n_links, tasks = (NL - (c->cfg.skip_base_link ? 1 : 0)) * n;
+    const size_t cap = std::max<size_t>((size_t) 1 << 22, tasks * 64);
+    return n * NL * (12 * 8 + 3 * 16 + 2 * 16) + 2 * (tasks + 1) * 4 + scan_tmp_words(tasks + 1) * 4 + cap * 8 +
+           (size_t) n * (4 * c->robot.dof + 4) + 64;
+}
+
+// Enqueue FK -> broad(count) -> scan -> broad(emit) -> narrow on ws.stream.  No host synchronisation.
+void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
+                      uint32_t* d_out, size_t budget) {
+    const hipStream_t s = ws.stream;
     const int NL = c->robot.n_links;
     const int l0 = c->cfg.skip_base_link ? 1 : 0;
     const int n_active = NL - l0;
     const size_t tasks = (size_t) n_active * n;
     const size_t need_cap = std::max<size_t>((size_t) 1 << 22, tasks * 64);
-    if (c->cand_cap < need_cap && c->cand.n < need_cap) {
-        check_budget(c, need_cap * sizeof(uint2));
-        c->cand.reserve(need_cap);
+    if (ws.cand.n < need_cap) {
+        budget_check(c, ws, budget, (need_cap - ws.cand.n) * sizeof(uint2));
+        ws.cand.reserve(need_cap);
     }
-    c->cand_cap = (uint32_t) std::min<size_t>(c->cand.n, 0xffffffffu);
-    size_t npl = (size_t) n * NL;
-    check_budget(c, 0);
-    c->T64.reserve(npl * 12);
-    c->Ti.reserve(npl * 3);
-    c->wbox.reserve(npl * 2);
-    c->counts.reserve(tasks + 1);
-    c->offsets.reserve(tasks + 1);
-    c->scan_tmp.reserve(scan_tmp_words(tasks + 1));
-    c->scalars.reserve(4);
-    c->stats.reserve(1);
+    ws.cand_cap = (uint32_t) std::min<size_t>(ws.cand.n, 0xffffffffu);
+    const size_t npl = (size_t) n * NL;
+    budget_check(c, ws, budget, 0);
+    ws.T64.reserve(npl * 12);
+    ws.Ti.reserve(npl * 3);
+    ws.wbox.reserve(npl * 2);
+    ws.counts.reserve(tasks + 1);
+    ws.offsets.reserve(tasks + 1);
+    ws.scan_tmp.reserve(scan_tmp_words(tasks + 1));
+    ws.scalars.reserve(4);
+    ws.stats.reserve(1);
 
     fill_u32(d_out, n, 0u, s);
-    fill_u32(c->scalars.p, 4, 0u, s);
-    if (c->counting) fill_u32(reinterpret_cast<uint32_t*>(c->stats.p), sizeof(Stats) / 4, 0u, s);
-    if (!c->ev0) {
-        RTCD_HIP(hipEventCreate(&c->ev0));
-        RTCD_HIP(hipEventCreate(&c->ev1));
-    }
-    if (!c->stage_ev[0])
-        for (auto& e : c->stage_ev) RTCD_HIP(hipEventCreate(&e));
-    RTCD_HIP(hipEventRecord(c->ev0, s));
-    launch_fk(c->d_robot.p, c->robot.dof, NL, d_poses, (int) n, c->T64.p, c->Ti.p, c->wbox.p, s);
+    fill_u32(ws.scalars.p, 4, 0u, s);
+    if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, s);
+    RTCD_HIP(hipEventRecord(ws.ev[0], s));
+    launch_fk(c->d_robot.p, c->robot.dof, NL, d_poses, (int) n, ws.T64.p, ws.Ti.p, ws.wbox.p, s);
     DiscreteArgs a{};
     a.robot = c->d_robot.p;
     a.rb = RobotBvh{c->r_tri_nodes.p, c->r_tris.p, c->r_edge_nodes.p, c->r_edges.p};
@@ -59,72 +70,76 @@ void enqueue_discrete(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, 
     a.l0 = l0;
     a.n_active = n_active;
     a.dir = dir;
-    a.T64 = c->T64.p;
-    a.Ti = c->Ti.p;
-    a.wbox = c->wbox.p;
-    a.counts = c->counts.p;
-    a.offsets = c->offsets.p;
-    a.cand = c->cand.p;
-    a.cap = c->cand_cap;
-    a.total = c->scalars.p;
-    a.overflow = c->scalars.p + 1;
+    a.T64 = ws.T64.p;
+    a.Ti = ws.Ti.p;
+    a.wbox = ws.wbox.p;
+    a.counts = ws.counts.p;
+    a.offsets = ws.offsets.p;
+    a.cand = ws.cand.p;
+    a.cap = ws.cand_cap;
+    a.total = ws.scalars.p;
+    a.overflow = ws.scalars.p + 1;
     a.flags = d_out;
-    a.stats = c->stats.p;
+    a.stats = ws.stats.p;
     a.broad_pad = 1e-5f * (1.f + sc.scale);
     const int depth = std::max(c->robot_depth, std::max(sc.tri_bvh.depth, sc.edge_bvh.depth));
     const int stack = stack_for(depth);
-    RTCD_HIP(hipEventRecord(c->stage_ev[0], s));
+    RTCD_HIP(hipEventRecord(ws.ev[1], s));
     launch_broad(a, stack, false, s);
-    RTCD_HIP(hipEventRecord(c->stage_ev[1], s));
-    exclusive_scan(c->counts.p, c->offsets.p, tasks, c->scalars.p, c->scan_tmp.p, s);
-    RTCD_HIP(hipEventRecord(c->stage_ev[2], s));
+    RTCD_HIP(hipEventRecord(ws.ev[2], s));
+    exclusive_scan(ws.counts.p, ws.offsets.p, tasks, ws.scalars.p, ws.scan_tmp.p, s);
+    RTCD_HIP(hipEventRecord(ws.ev[3], s));
     launch_broad(a, stack, true, s);
-    RTCD_HIP(hipEventRecord(c->stage_ev[3], s));
-    launch_narrow(a, stack, c->counting, s);
-    RTCD_HIP(hipEventRecord(c->ev1, s));
-    c->timing_pending = true;
-    c->last.max_stack = (uint32_t) stack;
-    c->last.max_depth = (uint32_t) depth;
+    RTCD_HIP(hipEventRecord(ws.ev[4], s));
+    launch_narrow(a, stack, ws.counting, s);
+    RTCD_HIP(hipEventRecord(ws.ev[5], s));
+    ws.timing_pending = true;
 }
 
-// After the stream finished: read overflow flags, re-run with a larger candidate buffer if needed.
-void finish_discrete(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, uint32_t n, uint32_t* d_out,
-                     hipStream_t s) {
+void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
+                     uint32_t* d_out, size_t budget, rtcd_stats* st) {
+    const hipStream_t s = ws.stream;
     for (int attempt = 0; attempt < 4; ++attempt) {
-        std::vector<uint32_t> sv = c->scalars.download(2, s);
+        std::vector<uint32_t> sv = ws.scalars.download(2, s);
         if (sv[1] & 2u) throw std::runtime_error("traversal stack overflow (BVH deeper than the stack)");
         if (!(sv[1] & 1u)) {
-            if (c->timing_pending) {
-                float ms = 0.f;
-                if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last.last_ms = ms;
-                hipEvent_t evs[6] = {c->ev0, c->stage_ev[0], c->stage_ev[1], c->stage_ev[2], c->stage_ev[3], c->ev1};
-                for (int k = 0; k < 5; ++k) {
-                    float sm = 0.f;
-                    c->last.stage_ms[k] = hipEventElapsedTime(&sm, evs[k], evs[k + 1]) == hipSuccess ? sm : -1.0;
+            if (st) {
+                if (ws.timing_pending) {
+                    float ms = 0.f;
+                    if (hipEventElapsedTime(&ms, ws.ev[0], ws.ev[5]) == hipSuccess) st->last_ms = ms;
+                    // stage k spans ev[k] (k = 0: start) .. ev[k+1]; ev[1] directly follows FK
+                    hipEvent_t b[6] = {ws.ev[0], ws.ev[1], ws.ev[2], ws.ev[3], ws.ev[4], ws.ev[5]};
+                    for (int k = 0; k < 5; ++k) {
+                        float sm = 0.f;
+                        st->stage_ms[k] = hipEventElapsedTime(&sm, b[k], b[k + 1]) == hipSuccess ? sm : -1.0;
+                    }
                 }
-                c->timing_pending = false;
+                st->candidates = sv[0];
+                const int depth = std::max(c->robot_depth, std::max(sc.tri_bvh.depth, sc.edge_bvh.depth));
+                st->max_depth = (uint32_t) depth;
+                st->max_stack = (uint32_t) stack_for(depth);
+                if (ws.counting) {
+                    Stats k = ws.stats.download(1, s)[0];
+                    st->node_fetches = k.node_fetches;
+                    st->prim_tests = k.prim_tests;
+                    st->fp64_redecisions = k.fp64;
+                    st->edge_tests = k.edge_tests;
+                }
             }
-            c->last.candidates = sv[0];
-            if (c->counting) {
-                Stats st = c->stats.download(1, s)[0];
-                c->last.node_fetches = st.node_fetches;
-                c->last.prim_tests = st.prim_tests;
-                c->last.fp64_redecisions = st.fp64;
-                c->last.edge_tests = st.edge_tests;
-            }
+            ws.timing_pending = false;
             return;
         }
-        size_t need = (size_t) sv[0] + sv[0] / 4 + 1024;
+        const size_t need = (size_t) sv[0] + sv[0] / 4 + 1024;
         if (need > 0xfffffff0ull) throw std::runtime_error("candidate count exceeds 2^32");
-        check_budget(c, need * sizeof(uint2));
-        c->cand.alloc(need);
-        c->cand_cap = (uint32_t) need;
-        enqueue_discrete(c, sc, dir, d_poses, n, d_out, s);
+        budget_check(c, ws, budget, need > ws.cand.n ? (need - ws.cand.n) * sizeof(uint2) : 0);
+        ws.cand.alloc(need);
+        enqueue_discrete(c, ws, sc, dir, d_poses, n, d_out, budget);
+        RTCD_HIP(hipStreamSynchronize(s));
     }
     throw std::runtime_error("candidate buffer overflow persisted");
 }
 
-}  // namespace
+}  // namespace rtcd
 
 extern "C" {
 
@@ -148,7 +163,9 @@ rtcd_status rtcd_create(const rtcd_config* cfg, rtcd_ctx** out) {
         RTCD_HIP(hipSetDevice(c.device));
         auto ctx = std::make_unique<rtcd_ctx>();
         ctx->cfg = c;
-        RTCD_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+        hipStream_t st = nullptr;
+        RTCD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        ctx->main.init(st, true);
         *out = ctx.release();
         return RTCD_OK;
     });
@@ -158,12 +175,8 @@ void rtcd_destroy(rtcd_ctx* ctx) {
     if (!ctx) return;
     (void) hipSetDevice(ctx->cfg.device);
     (void) hipDeviceSynchronize();
-    serve_shutdown(ctx->sweep);
-    for (auto e : ctx->stage_ev)
-        if (e) (void) hipEventDestroy(e);
-    if (ctx->ev0) (void) hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void) hipEventDestroy(ctx->ev1);
-    if (ctx->stream) (void) hipStreamDestroy(ctx->stream);
+    serve_shutdown(ctx);
+    ctx->main.destroy();
     delete ctx;
 }
 
@@ -177,7 +190,7 @@ rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = ctx->stream;
+        hipStream_t s = ctx->main.stream;
         const int NL = (int) d->dof + 1;
         RobotDev R{};
         R.dof = (int) d->dof;
@@ -277,15 +290,12 @@ rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
         }
         if (!d->n_spheres) R.n_spheres = 0;
         std::vector<float4> sph(d->n_spheres);
-        ctx->sphere_r.assign(d->n_spheres, 0.f);
         for (uint32_t i = 0; i < d->n_spheres; ++i) {
             if (d->sphere_link[i] < 0 || d->sphere_link[i] >= NL) return fail(RTCD_ERR_INVALID_ARG, "sphere link out of range");
             if (!finite_all(d->spheres + 4 * i, 4) || d->spheres[4 * i + 3] < 0.f)
                 return fail(RTCD_ERR_INVALID_ARG, "invalid sphere");
             sph[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
-            ctx->sphere_r[i] = d->spheres[4 * i + 3];
         }
-        ctx->sweep.sphere_link.assign(d->sphere_link, d->sphere_link + d->n_spheres);
         ctx->r_tri_nodes.alloc(all_tn.size());
         ctx->r_tri_nodes.upload(all_tn, s);
         ctx->r_tris.alloc(all_t.size());
@@ -294,15 +304,18 @@ rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
         ctx->r_edge_nodes.upload(all_en, s);
         ctx->r_edges.alloc(all_e.size());
         ctx->r_edges.upload(all_e, s);
-        ctx->r_spheres.alloc(sph.size());
+        ctx->r_spheres.alloc(std::max<size_t>(sph.size(), 1));
         ctx->r_spheres.upload(sph, s);
+        std::vector<int32_t> sl(d->sphere_link, d->sphere_link + d->n_spheres);
+        ctx->r_sphere_link.alloc(std::max<size_t>(sl.size(), 1));
+        ctx->r_sphere_link.upload(sl, s);
+        ctx->n_spheres = (int) d->n_spheres;
         ctx->robot = R;
         ctx->d_robot.alloc(1);
         ctx->d_robot.upload(&ctx->robot, 1, s);
         RTCD_HIP(hipStreamSynchronize(s));
         ctx->robot_depth = depth;
         ctx->robot_loaded = true;
-        sweep_robot_loaded(ctx->sweep, ctx->r_spheres.p, (int) d->n_spheres, s);
         return RTCD_OK;
     });
 }
@@ -316,8 +329,8 @@ rtcd_status rtcd_scene_create(rtcd_ctx* ctx, const float* verts, uint32_t n_vert
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = ctx->stream;
-        auto sc = std::make_unique<SceneRes>();
+        hipStream_t s = ctx->main.stream;
+        auto sc = std::make_shared<SceneRes>();
         EdgeSet es = unique_edges(tris, n_tris);
         const int nt = (int) n_tris, ne = (int) es.edges.size() / 2;
         std::vector<Tri> tri(nt);
@@ -370,15 +383,15 @@ rtcd_status rtcd_scene_destroy(rtcd_ctx* ctx, uint32_t scene_id) {
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     auto it = ctx->scenes.find(scene_id);
     if (it == ctx->scenes.end()) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
-    (void) hipStreamSynchronize(ctx->stream);
-    ctx->scenes.erase(it);
+    (void) hipStreamSynchronize(ctx->main.stream);
+    ctx->scenes.erase(it);  // served requests still in flight hold their own reference
     return RTCD_OK;
 }
 
 rtcd_status rtcd_scene_info(rtcd_ctx* ctx, uint32_t scene_id, uint32_t* n_tris, uint32_t* n_edges) {
     if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    SceneRes* sc = find_scene(ctx, scene_id);
+    auto sc = find_scene(ctx, scene_id);
     if (!sc) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
     if (n_tris) *n_tris = (uint32_t) sc->n_tri;
     if (n_edges) *n_edges = (uint32_t) sc->n_edge;
@@ -393,14 +406,23 @@ rtcd_status rtcd_check_discrete_device(rtcd_ctx* ctx, uint32_t scene_id, int32_t
     if (n > (1u << 26)) return fail(RTCD_ERR_INVALID_ARG, "n too large for one call (max 2^26)");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     if (!ctx->robot_loaded) return fail(RTCD_ERR_NOT_READY, "robot not loaded");
-    SceneRes* sc = find_scene(ctx, scene_id);
+    auto sc = find_scene(ctx, scene_id);
     if (!sc) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = stream ? (hipStream_t) stream : ctx->stream;
         if (n == 0) return RTCD_OK;
-        enqueue_discrete(ctx, *sc, direction, d_poses, n, d_out, s);
-        ctx->pending = {true, scene_id, direction, d_poses, n, d_out, s};
+        Workspace& ws = ctx->main;
+        hipStream_t own = ws.stream;
+        if (stream) ws.stream = (hipStream_t) stream;
+        ws.counting = ctx->counting;
+        try {
+            enqueue_discrete(ctx, ws, *sc, direction, d_poses, n, d_out, 0);
+        } catch (...) {
+            ws.stream = own;
+            throw;
+        }
+        ws.stream = own;
+        ctx->pending = {true, sc, direction, d_poses, n, d_out, stream ? (hipStream_t) stream : own};
         return RTCD_OK;
     });
 }
@@ -409,14 +431,22 @@ rtcd_status rtcd_sync(rtcd_ctx* ctx, void* stream) {
     if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
-        hipStream_t s = stream ? (hipStream_t) stream : ctx->stream;
+        hipStream_t s = stream ? (hipStream_t) stream : ctx->main.stream;
         RTCD_HIP(hipStreamSynchronize(s));
         if (ctx->pending.active && ctx->pending.stream == s) {
             auto p = ctx->pending;
             ctx->pending.active = false;
-            SceneRes* sc = find_scene(ctx, p.scene);
-            if (!sc) return fail(RTCD_ERR_UNKNOWN_ID, "scene destroyed while a check was pending");
-            finish_discrete(ctx, *sc, p.dir, p.poses, p.n, p.out, s);
+            ctx->pending.scene.reset();
+            Workspace& ws = ctx->main;
+            hipStream_t own = ws.stream;
+            ws.stream = s;
+            try {
+                finish_discrete(ctx, ws, *p.scene, p.dir, p.poses, p.n, p.out, 0, &ctx->last);
+            } catch (...) {
+                ws.stream = own;
+                throw;
+            }
+            ws.stream = own;
         }
         return RTCD_OK;
     });
@@ -430,20 +460,22 @@ rtcd_status rtcd_check_discrete(rtcd_ctx* ctx, uint32_t scene_id, int32_t direct
     if (n > (1u << 26)) return fail(RTCD_ERR_INVALID_ARG, "n too large for one call (max 2^26)");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     if (!ctx->robot_loaded) return fail(RTCD_ERR_NOT_READY, "robot not loaded");
-    SceneRes* sc = find_scene(ctx, scene_id);
+    auto sc = find_scene(ctx, scene_id);
     if (!sc) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
     if (!finite_all(poses, (size_t) n * ctx->robot.dof)) return fail(RTCD_ERR_INVALID_ARG, "non-finite joint value");
     if (n == 0) return RTCD_OK;
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = ctx->stream;
-        ctx->h2d_poses.reserve((size_t) n * ctx->robot.dof);
-        ctx->h2d_out.reserve(n);
-        RTCD_HIP(hipMemcpyAsync(ctx->h2d_poses.p, poses, sizeof(float) * n * ctx->robot.dof, hipMemcpyHostToDevice, s));
-        enqueue_discrete(ctx, *sc, direction, ctx->h2d_poses.p, n, ctx->h2d_out.p, s);
+        Workspace& ws = ctx->main;
+        ws.counting = ctx->counting;
+        const hipStream_t s = ws.stream;
+        ws.h_poses.reserve((size_t) n * ctx->robot.dof);
+        ws.h_out.reserve(n);
+        RTCD_HIP(hipMemcpyAsync(ws.h_poses.p, poses, sizeof(float) * n * ctx->robot.dof, hipMemcpyHostToDevice, s));
+        enqueue_discrete(ctx, ws, *sc, direction, ws.h_poses.p, n, ws.h_out.p, 0);
         RTCD_HIP(hipStreamSynchronize(s));
-        finish_discrete(ctx, *sc, direction, ctx->h2d_poses.p, n, ctx->h2d_out.p, s);
-        RTCD_HIP(hipMemcpyAsync(out, ctx->h2d_out.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+        finish_discrete(ctx, ws, *sc, direction, ws.h_poses.p, n, ws.h_out.p, 0, &ctx->last);
+        RTCD_HIP(hipMemcpyAsync(out, ws.h_out.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
         RTCD_HIP(hipStreamSynchronize(s));
         return RTCD_OK;
     });

@@ -48,10 +48,10 @@ __device__ __forceinline__ MTBands make_bands(float D, float ad, float S, float 
 
 enum { kMiss = 0, kHit = 1, kUnsure = 2 };
 
-// Segment P + s d (s in [0,1]) vs triangle (A, B, C), all in one frame, FP32.
-__device__ __forceinline__ int mt32(float3 P, float3 d, float3 A, float3 B, float3 C, const MTBands& bd) {
-    float e1x = B.x - A.x, e1y = B.y - A.y, e1z = B.z - A.z;
-    float e2x = C.x - A.x, e2y = C.y - A.y, e2z = C.z - A.z;
+// Segment P + s d (s in [0,1]) vs triangle (A, A + e1, A + e2), all in one frame, FP32.
+__device__ __forceinline__ int mt32(float3 P, float3 d, float3 A, float3 e1, float3 e2, const MTBands& bd) {
+    const float e1x = e1.x, e1y = e1.y, e1z = e1.z;
+    const float e2x = e2.x, e2y = e2.y, e2z = e2.z;
     float sx = P.x - A.x, sy = P.y - A.y, sz = P.z - A.z;
     float pvx = d.y * e2z - d.z * e2y, pvy = d.z * e2x - d.x * e2z, pvz = d.x * e2y - d.y * e2x;
     float det = e1x * pvx + e1y * pvy + e1z * pvz;

@@ -12,9 +12,9 @@
 #include "../../include/rtcd.h"
 #include "geometry.hpp"
 #include "kernels.hpp"
-#include "serve.hpp"
 
 namespace rtcd {
+struct ServeState;
 extern thread_local std::string g_err;
 
 inline rtcd_status fail(rtcd_status code, const std::string& msg) {
@@ -73,11 +73,59 @@ struct SceneRes {
 
 }  // namespace rtcd
 
+namespace rtcd {
+
+// Every buffer one in-flight check needs, bound to one stream.  The blocking and *_device entry
+// points use the context's main workspace; the serving loop owns one workspace per slot.
+struct Workspace {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // discrete
+    DevBuf<double> T64;
+    DevBuf<float4> Ti, wbox;
+    DevBuf<uint32_t> counts, offsets, scan_tmp, scalars;  // scalars: [0] total, [1] overflow
+    DevBuf<uint2> cand;
+    uint32_t cand_cap = 0;
+    // host staging
+    DevBuf<float> h_poses;
+    DevBuf<uint32_t> h_offsets, h_out, h_seg;
+    // sweeps
+    DevBuf<double> c64;       // sphere centres [W][S][3]
+    DevBuf<float4> c32;       // [W][S] (x, y, z, r)
+    DevBuf<uint32_t> seg_traj, seg_wp;
+    DevBuf<double> P64, W64;  // quadratic control points [n_traj][S][n_ctrl][3], weights
+    DevBuf<float4> P32;       // [n_traj][S][n_ctrl]
+    DevBuf<Stats> stats;
+    std::vector<double> W_host;  // last quadratic weights uploaded to W64
+    // timing: ev[0] start, ev[1..4] after FK, broad count, scan, broad emit, ev[5] end
+    hipEvent_t ev[6] = {};
+    bool timing_pending = false;
+    bool counting = false;
+
+    size_t bytes() const {
+        return T64.bytes() + Ti.bytes() + wbox.bytes() + counts.bytes() + offsets.bytes() + scan_tmp.bytes() +
+               scalars.bytes() + cand.bytes() + h_poses.bytes() + h_offsets.bytes() + h_out.bytes() + h_seg.bytes() +
+               c64.bytes() + c32.bytes() + seg_traj.bytes() + seg_wp.bytes() + P64.bytes() + W64.bytes() + P32.bytes();
+    }
+    void init(hipStream_t s, bool own) {
+        stream = s;
+        own_stream = own;
+        for (auto& e : ev) RTCD_HIP(hipEventCreate(&e));
+    }
+    void destroy() {
+        for (auto& e : ev)
+            if (e) (void) hipEventDestroy(e), e = nullptr;
+        if (own_stream && stream) (void) hipStreamDestroy(stream);
+        stream = nullptr;
+    }
+};
+
+}  // namespace rtcd
+
 using namespace rtcd;
 
 struct rtcd_ctx {
     rtcd_config cfg{};
-    hipStream_t stream = nullptr;
     std::recursive_mutex mu;
     // robot
     bool robot_loaded = false;
@@ -86,46 +134,35 @@ struct rtcd_ctx {
     DevBuf<BNode> r_tri_nodes, r_edge_nodes;
     DevBuf<Tri> r_tris;
     DevBuf<Edge> r_edges;
-    DevBuf<float4> r_spheres;  // x y z r (link frame), .w = radius
-    std::vector<float> sphere_r;
+    DevBuf<float4> r_spheres;  // x y z r (link frame)
+    DevBuf<int32_t> r_sphere_link;
+    int n_spheres = 0;
     int robot_depth = 1;
-    // scenes
-    std::map<uint32_t, std::unique_ptr<SceneRes>> scenes;
+    // scenes (shared so that in-flight served requests keep a scene alive)
+    std::map<uint32_t, std::shared_ptr<SceneRes>> scenes;
     uint32_t next_scene = 1;
-    // discrete work buffers
-    DevBuf<double> T64;
-    DevBuf<float4> Ti, wbox;
-    DevBuf<uint32_t> counts, offsets, scan_tmp, scalars;  // scalars: [0] total, [1] overflow
-    DevBuf<uint2> cand;
-    DevBuf<float> h2d_poses;
-    DevBuf<uint32_t> h2d_out;
-    DevBuf<Stats> stats;
-    uint32_t cand_cap = 0;
+    // main workspace (blocking and *_device calls)
+    Workspace main;
     bool counting = false;
     rtcd_stats last{};
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t stage_ev[4] = {nullptr, nullptr, nullptr, nullptr};  // after FK, broad count, scan, broad emit
-    bool timing_pending = false;
-    // sweep work buffers
-    SweepBuffers sweep;
     // async discrete call waiting for rtcd_sync (for the capacity re-run)
     struct Pending {
         bool active = false;
-        uint32_t scene;
+        std::shared_ptr<SceneRes> scene;
         int dir;
         const float* poses;
         uint32_t n;
         uint32_t* out;
         hipStream_t stream;
     } pending;
+    // serving loop
+    ServeState* serve = nullptr;
+    size_t serve_bytes = 0;  // device bytes held by serving slots
 };
 
 namespace rtcd {
 
-inline size_t mem_in_use(const rtcd_ctx* c) {
-    return c->T64.bytes() + c->Ti.bytes() + c->wbox.bytes() + c->counts.bytes() + c->offsets.bytes() +
-           c->scan_tmp.bytes() + c->cand.bytes() + c->h2d_poses.bytes() + c->h2d_out.bytes() + c->sweep.bytes();
-}
+inline size_t mem_in_use(const rtcd_ctx* c) { return c->main.bytes() + c->serve_bytes; }
 
 inline void check_budget(const rtcd_ctx* c, size_t extra) {
     if (c->cfg.mem_budget && mem_in_use(c) + extra > c->cfg.mem_budget)
@@ -133,9 +170,27 @@ inline void check_budget(const rtcd_ctx* c, size_t extra) {
                                                 " bytes exceeded (need " + std::to_string(mem_in_use(c) + extra) + ")");
 }
 
-inline SceneRes* find_scene(rtcd_ctx* c, uint32_t id) {
+inline std::shared_ptr<SceneRes> find_scene(rtcd_ctx* c, uint32_t id) {
     auto it = c->scenes.find(id);
-    return it == c->scenes.end() ? nullptr : it->second.get();
+    return it == c->scenes.end() ? nullptr : it->second;
 }
+
+// ---- orchestration (api.hip: discrete, swept.hip: sweeps); all enqueue on ws.stream ----
+// Budget checks use `budget` bytes (0 = unlimited) for this workspace alone.
+void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
+                      uint32_t* d_out, size_t budget);
+// After ws.stream finished: re-runs on candidate overflow; fills *st (timing, counters) if non-null.
+void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
+                     uint32_t* d_out, size_t budget, rtcd_stats* st);
+void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_traj, const uint32_t* d_offs,
+                    uint32_t n_traj, uint32_t nw, uint32_t* d_out, uint32_t* d_seg, size_t budget);
+void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
+                       uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget);
+void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st);
+// Device bytes a request of this shape needs in a fresh workspace (admission control).
+size_t discrete_bytes(const rtcd_ctx* c, uint32_t n);
+size_t sweep_bytes(const rtcd_ctx* c, uint32_t n_traj, uint32_t n_wp_total, uint32_t n_ctrl);
+// Stops the serving loop (drains in-flight requests) and frees its slots.
+void serve_shutdown(rtcd_ctx* c);
 
 }  // namespace rtcd

@@ -19,10 +19,16 @@ constexpr int kBS = 256;
 
 __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
 __device__ __forceinline__ float3 sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ float3 xf(const float4& r0, const float4& r1, const float4& r2, float3 p) {
-    return make_float3(r0.x * p.x + r0.y * p.y + r0.z * p.z + r0.w, r1.x * p.x + r1.y * p.y + r1.z * p.z + r1.w,
-                       r2.x * p.x + r2.y * p.y + r2.z * p.z + r2.w);
+// Rows r_j = (R_0j, R_1j, R_2j, t_j): rot() applies R^T, to_link() applies R^T (p - t).
+__device__ __forceinline__ float3 rot(const float4& r0, const float4& r1, const float4& r2, float3 p) {
+    return make_float3(r0.x * p.x + r0.y * p.y + r0.z * p.z, r1.x * p.x + r1.y * p.y + r1.z * p.z,
+                       r2.x * p.x + r2.y * p.y + r2.z * p.z);
 }
+__device__ __forceinline__ float3 to_link(const float4& r0, const float4& r1, const float4& r2, float3 p) {
+    return rot(r0, r1, r2, make_float3(p.x - r0.w, p.y - r1.w, p.z - r2.w));
+}
+__device__ __forceinline__ float3 add(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 neg(float3 a) { return make_float3(-a.x, -a.y, -a.z); }
 __device__ __forceinline__ uint32_t load_flag(const uint32_t* f) {
     return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -52,11 +58,10 @@ __global__ __launch_bounds__(kBS) void k_fk(const RobotDev* __restrict__ R, cons
         }
         const size_t pl = (size_t) p * NL + l;
         for (int k = 0; k < 12; ++k) T64[pl * 12 + k] = T[k];
-        // inverse rows [R^T | -R^T t]
-        for (int j = 0; j < 3; ++j) {
-            double tj = -((T[0 * 4 + j] * T[3] + T[1 * 4 + j] * T[7]) + T[2 * 4 + j] * T[11]);
-            Ti[pl * 3 + j] = make_float4((float) T[0 * 4 + j], (float) T[1 * 4 + j], (float) T[2 * 4 + j], (float) tj);
-        }
+        // rows of R^T with the world translation in w: p_link = R^T (p - t)
+        for (int j = 0; j < 3; ++j)
+            Ti[pl * 3 + j] = make_float4((float) T[0 * 4 + j], (float) T[1 * 4 + j], (float) T[2 * 4 + j],
+                                         (float) T[j * 4 + 3]);
         // world AABB of the link box, rounded outwards
         const LinkInfo& L = R->link[l];
         double c[3] = {L.c[0], L.c[1], L.c[2]};
@@ -83,7 +88,7 @@ __device__ __forceinline__ bool broad_test(float lx, float ly, float lz, float h
     if (lx > whi.x || hx < wlo.x || ly > whi.y || hy < wlo.y || lz > whi.z || hz < wlo.z) return false;
     float cx = 0.5f * (lx + hx), cy = 0.5f * (ly + hy), cz = 0.5f * (lz + hz);
     float ex = 0.5f * (hx - lx), ey = 0.5f * (hy - ly), ez = 0.5f * (hz - lz);
-    float3 c = xf(r0, r1, r2, make_float3(cx, cy, cz));
+    float3 c = to_link(r0, r1, r2, make_float3(cx, cy, cz));
     float hx2 = fabsf(r0.x) * ex + fabsf(r0.y) * ey + fabsf(r0.z) * ez;
     float hy2 = fabsf(r1.x) * ex + fabsf(r1.y) * ey + fabsf(r1.z) * ez;
     float hz2 = fabsf(r2.x) * ex + fabsf(r2.y) * ey + fabsf(r2.z) * ez;
@@ -162,16 +167,16 @@ struct Ctr {
 };
 
 // Obstacle edge (link frame P -> Q, world endpoints wa -> wb) against the link-triangle LBVH.
+// P: link-frame origin (abs. error <= delta per component), d: R^T (Q - P) (rel. error <= 6 eps).
 template <int STACK, bool COUNT>
-__device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, const double* T64, float3 P, float3 Q,
+__device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, const double* T64, float3 P, float3 d,
                         float delta, float4 wa, float4 wb, int* st, const uint32_t* flag, Ctr& ctr) {
     if (!L.tri_has_root) return 0;
-    const float3 d = sub(Q, P);
     const float3 inv = safe_inv(d);
-    const float pad = 2.f * delta + 1e-6f * L.scale;
+    const float D = l1(d.x, d.y, d.z) * (1.f + 8.f * kEps32);
+    const float ad = 6.1f * kEps32 * D;
+    const float pad = 2.f * delta + 2.f * ad + 1e-6f * L.scale;
     if (!slab(P, inv, L.lo[0], L.lo[1], L.lo[2], L.hi[0], L.hi[1], L.hi[2], pad)) return 0;
-    const float D = l1(d.x, d.y, d.z) * (1.f + 2.f * kEps32) + 6.f * delta;
-    const float ad = 2.f * delta + 1.01f * kEps32 * D;
     const float S = l1(P.x - L.c[0], P.y - L.c[1], P.z - L.c[2]) * (1.f + 2.f * kEps32) + 3.f * delta + L.rv;
     const float as = delta + kEps32 * S;
     const MTBands bd = make_bands(D, ad, S, as, L.e1max, kEps32 * L.e1max, L.e2max, kEps32 * L.e2max);
@@ -195,7 +200,8 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, const double* 
                 for (int k = 0; k < cnt; ++k) {
                     const Tri t = prims[start + k];
                     if (COUNT) ctr.prims++;
-                    int r = mt32(P, d, f3(t.a), f3(t.b), f3(t.c), bd);
+                    const float3 ta = f3(t.a);
+                    int r = mt32(P, d, ta, sub(f3(t.b), ta), sub(f3(t.c), ta), bd);
                     if (r == kHit) return 1;
                     if (r == kUnsure) {
                         if (COUNT) ctr.fp64++;
@@ -231,21 +237,23 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, const double* 
 }
 
 // Scene triangle (link frame A', B', C'; world wa, wb, wc) against the link-edge LBVH.
+// A: link-frame vertex (abs. error <= delta), e1 = R^T (B - A), e2 = R^T (C - A) (rel. error <= 6 eps).
 template <int STACK, bool COUNT>
-__device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, const double* T64, float3 A, float3 B, float3 C,
+__device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, const double* T64, float3 A, float3 e1, float3 e2,
                        float delta, float4 wa, float4 wb, float4 wc, int* st, const uint32_t* flag, Ctr& ctr) {
     if (!L.edge_has_root) return 0;
-    const float pad = 2.f * delta + 1e-6f * L.scale;
+    const float3 B = add(A, e1), C = add(A, e2);
+    const float pad = 2.f * delta + 12.f * kEps32 * (l1(e1.x, e1.y, e1.z) + l1(e2.x, e2.y, e2.z)) + 1e-6f * L.scale;
     const float lx = fminf(fminf(A.x, B.x), C.x) - pad, ly = fminf(fminf(A.y, B.y), C.y) - pad,
                 lz = fminf(fminf(A.z, B.z), C.z) - pad;
     const float hx = fmaxf(fmaxf(A.x, B.x), C.x) + pad, hy = fmaxf(fmaxf(A.y, B.y), C.y) + pad,
                 hz = fmaxf(fmaxf(A.z, B.z), C.z) + pad;
     if (lx > L.hi[0] || hx < L.lo[0] || ly > L.hi[1] || hy < L.lo[1] || lz > L.hi[2] || hz < L.lo[2]) return 0;
-    const float E1 = l1(B.x - A.x, B.y - A.y, B.z - A.z) * (1.f + 2.f * kEps32) + 6.f * delta;
-    const float E2 = l1(C.x - A.x, C.y - A.y, C.z - A.z) * (1.f + 2.f * kEps32) + 6.f * delta;
+    const float E1 = l1(e1.x, e1.y, e1.z) * (1.f + 8.f * kEps32);
+    const float E2 = l1(e2.x, e2.y, e2.z) * (1.f + 8.f * kEps32);
     const float S = l1(A.x - L.c[0], A.y - L.c[1], A.z - L.c[2]) * (1.f + 2.f * kEps32) + 3.f * delta + L.rv;
-    const MTBands bd = make_bands(L.dmax, kEps32 * L.dmax, S, delta + kEps32 * S, E1, 2.f * delta + kEps32 * E1, E2,
-                                  2.f * delta + kEps32 * E2);
+    const MTBands bd = make_bands(L.dmax, kEps32 * L.dmax, S, delta + kEps32 * S, E1, 6.1f * kEps32 * E1, E2,
+                                  6.1f * kEps32 * E2);
     const BNode* nodes = a.rb.edge_nodes + L.edge_node0;
     const Edge* prims = a.rb.edge_prims + L.edge_prim0;
     int sp = 0, ni = 0, iter = 0;
@@ -267,7 +275,7 @@ __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, const double* T
                     const Edge e = prims[start + k];
                     if (COUNT) { ctr.prims++; ctr.edges++; }
                     const float3 ep = f3(e.p);
-                    int r = mt32(ep, sub(f3(e.q), ep), A, B, C, bd);
+                    int r = mt32(ep, sub(f3(e.q), ep), A, e1, e2, bd);
                     if (r == kHit) return 1;
                     if (r == kUnsure) {
                         if (COUNT) ctr.fp64++;
@@ -314,22 +322,28 @@ __global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
         const float4 r0 = a.Ti[(size_t) c.x * 3], r1 = a.Ti[(size_t) c.x * 3 + 1], r2 = a.Ti[(size_t) c.x * 3 + 2];
         const double* T64 = a.T64 + (size_t) c.x * 12;
         const Tri t = a.sc.tri_prims[c.y];
-        const float3 A = xf(r0, r1, r2, f3(t.a)), B = xf(r0, r1, r2, f3(t.b)), C = xf(r0, r1, r2, f3(t.c));
-        // |P' - P_exact| <= 4 eps (|P|_1 + |t'|) per component; 6 eps with the FP64 margin
-        const float tnorm = fabsf(r0.w) + fabsf(r1.w) + fabsf(r2.w);
-        const float vmax = fmaxf(fmaxf(l1(t.a.x, t.a.y, t.a.z), l1(t.b.x, t.b.y, t.b.z)), l1(t.c.x, t.c.y, t.c.z));
-        const float delta = 6.f * kEps32 * (vmax + tnorm) * 1.01f;
+        // vertex A moved as R^T (A - t); edges rotated from world differences, so that their error is
+        // relative to their own length:  |A' - A'_exact| <= eps |t| + 5 eps |A - t|_1 per component
+        const float3 wA = f3(t.a), wB = f3(t.b), wC = f3(t.c);
+        const float3 rel = make_float3(wA.x - r0.w, wA.y - r1.w, wA.z - r2.w);
+        const float3 A = rot(r0, r1, r2, rel);
+        const float3 e1 = rot(r0, r1, r2, sub(wB, wA)), e2 = rot(r0, r1, r2, sub(wC, wA));
+        const float tinf = fmaxf(fmaxf(fabsf(r0.w), fabsf(r1.w)), fabsf(r2.w));
+        const float en = fmaxf(l1(e1.x, e1.y, e1.z), l1(e2.x, e2.y, e2.z));
+        // B' = A' + e1', C' = A' + e2' add their own rounding: 8 eps |e|
+        const float delta = 1.05f * kEps32 * (tinf + 5.f * l1(rel.x, rel.y, rel.z) + 8.f * en) + 1e-30f;
         int hit = 0;
         if (a.dir & 1) {
             const uint32_t owned = __float_as_uint(t.b.w);
-            const float3 V[3] = {A, B, C};
-            const float4 W[3] = {t.a, t.b, t.c};
-            for (int j = 0; j < 3 && !hit; ++j)
-                if (owned & (1u << j))
-                    hit = o2r_edge<STACK, COUNT>(a, L, T64, V[j], V[(j + 1) % 3], delta, W[j], W[(j + 1) % 3], st,
-                                                 flag, ctr);
+            if (owned & 1u)
+                hit = o2r_edge<STACK, COUNT>(a, L, T64, A, e1, delta, t.a, t.b, st, flag, ctr);
+            if (!hit && (owned & 2u))
+                hit = o2r_edge<STACK, COUNT>(a, L, T64, add(A, e1), rot(r0, r1, r2, sub(wC, wB)), delta, t.b, t.c,
+                                             st, flag, ctr);
+            if (!hit && (owned & 4u))
+                hit = o2r_edge<STACK, COUNT>(a, L, T64, add(A, e2), neg(e2), delta, t.c, t.a, st, flag, ctr);
         }
-        if (!hit && (a.dir & 2)) hit = r2o_tri<STACK, COUNT>(a, L, T64, A, B, C, delta, t.a, t.b, t.c, st, flag, ctr);
+        if (!hit && (a.dir & 2)) hit = r2o_tri<STACK, COUNT>(a, L, T64, A, e1, e2, delta, t.a, t.b, t.c, st, flag, ctr);
         if (hit) atomicOr(flag, 1u);
     }
     if (COUNT) {

@@ -371,128 +371,144 @@ void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStre
 
 }  // namespace rtcd
 
-// ============================================================================================ C ABI
-namespace {
+// ============================================================================================ host side
+namespace rtcd {
 
-void enqueue_centers(rtcd_ctx* c, const float* d_poses, int nw, hipStream_t s) {
-    SweepBuffers& b = c->sweep;
-    const int S = b.n_spheres;
-    check_budget(c, 0);
-    b.c64.reserve((size_t) nw * S * 3);
-    b.c32.reserve((size_t) nw * S);
+size_t sweep_bytes(const rtcd_ctx* c, uint32_t n_traj, uint32_t nw, uint32_t nc) {
+    const size_t S = (size_t) c->n_spheres;
+    size_t b = (size_t) nw * S * (24 + 16) + (size_t) nw * (4 * c->robot.dof) + (size_t) n_traj * 8 + 64;
+    if (nc) b += (size_t) n_traj * S * nc * (24 + 16) + (size_t) nc * (nw / std::max<uint32_t>(n_traj, 1)) * 8;
+    else b += (size_t) nw * 8;
+    return b;
+}
+
+static void budget_check_ws(const rtcd_ctx* c, const Workspace& ws, size_t budget) {
+    if (budget && ws.bytes() > budget)
+        throw HipError(hipErrorOutOfMemory, "request needs more than its memory budget of " + std::to_string(budget) +
+                                                " bytes");
+    if (&ws == &c->main) check_budget(c, 0);
+}
+
+static void enqueue_centers(rtcd_ctx* c, Workspace& ws, const float* d_poses, int nw) {
+    const int S = c->n_spheres;
+    ws.c64.reserve((size_t) nw * S * 3);
+    ws.c32.reserve((size_t) nw * S);
     if (nw > 0 && S > 0) {
-        hipLaunchKernelGGL(k_centers, dim3(blocks_for(nw, kBS)), dim3(kBS), 0, s, c->d_robot.p, b.spheres,
-                           b.d_sphere_link.p, S, d_poses, nw, b.c64.p, b.c32.p);
+        hipLaunchKernelGGL(k_centers, dim3(blocks_for(nw, kBS)), dim3(kBS), 0, ws.stream, c->d_robot.p,
+                           c->r_spheres.p, c->r_sphere_link.p, S, d_poses, nw, ws.c64.p, ws.c32.p);
         RTCD_LAUNCH_CHECK();
     }
 }
 
-SweepArgs base_args(rtcd_ctx* c, SceneRes& sc, uint32_t* d_out, uint32_t* d_seg) {
+static SweepArgs base_args(rtcd_ctx* c, Workspace& ws, SceneRes& sc, uint32_t* d_out, uint32_t* d_seg) {
     SweepArgs a{};
     a.robot = c->d_robot.p;
-    a.robot_n_spheres_host = c->sweep.n_spheres;
+    a.robot_n_spheres_host = c->n_spheres;
     a.sc = sc.dev();
-    a.centers64 = c->sweep.c64.p;
+    a.centers64 = ws.c64.p;
     a.out = d_out;
     a.out_seg = d_seg;
-    c->sweep.stats.reserve(1);
-    a.stats = c->sweep.stats.p;
+    a.stats = ws.stats.p;
     return a;
 }
 
-void sweep_begin(rtcd_ctx* c, hipStream_t s) {
-    if (!c->ev0) {
-        RTCD_HIP(hipEventCreate(&c->ev0));
-        RTCD_HIP(hipEventCreate(&c->ev1));
-    }
-    c->sweep.stats.reserve(1);
-    if (c->counting) fill_u32(reinterpret_cast<uint32_t*>(c->sweep.stats.p), sizeof(Stats) / 4, 0u, s);
-    RTCD_HIP(hipEventRecord(c->ev0, s));
+static void sweep_begin(Workspace& ws) {
+    ws.stats.reserve(1);
+    if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, ws.stream);
+    RTCD_HIP(hipEventRecord(ws.ev[0], ws.stream));
 }
 
-void sweep_end(rtcd_ctx* c, hipStream_t s, int depth) {
-    RTCD_HIP(hipEventRecord(c->ev1, s));
-    c->timing_pending = true;
-    c->last.max_depth = (uint32_t) depth;
-    c->last.max_stack = (uint32_t) (depth + 2 <= 32 ? 32 : 64);
+static void sweep_end(Workspace& ws) {
+    RTCD_HIP(hipEventRecord(ws.ev[5], ws.stream));
+    ws.timing_pending = true;
 }
 
-void sweep_finish(rtcd_ctx* c, hipStream_t s) {
-    RTCD_HIP(hipStreamSynchronize(s));
-    if (c->timing_pending) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) c->last.last_ms = ms;
-        c->timing_pending = false;
+void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st) {
+    RTCD_HIP(hipStreamSynchronize(ws.stream));
+    if (st) {
+        if (ws.timing_pending) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, ws.ev[0], ws.ev[5]) == hipSuccess) st->last_ms = ms;
+            for (double& x : st->stage_ms) x = 0.0;
+        }
+        if (ws.counting) {
+            Stats k = ws.stats.download(1, ws.stream)[0];
+            st->node_fetches = k.node_fetches;
+            st->prim_tests = k.prim_tests;
+            st->fp64_redecisions = k.fp64;
+            st->candidates = k.candidates;
+            st->edge_tests = k.edge_tests;
+        }
     }
-    if (c->counting) {
-        Stats st = c->sweep.stats.download(1, s)[0];
-        c->last.node_fetches = st.node_fetches;
-        c->last.prim_tests = st.prim_tests;
-        c->last.fp64_redecisions = st.fp64;
-        c->last.candidates = st.candidates;
-        c->last.edge_tests = st.edge_tests;
-    }
+    ws.timing_pending = false;
 }
 
-void enqueue_linear(rtcd_ctx* c, SceneRes& sc, const float* d_traj, const uint32_t* d_offs, uint32_t n_traj,
-                    uint32_t nw, uint32_t* d_out, uint32_t* d_seg, hipStream_t s) {
-    SweepBuffers& b = c->sweep;
+void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_traj, const uint32_t* d_offs,
+                    uint32_t n_traj, uint32_t nw, uint32_t* d_out, uint32_t* d_seg, size_t budget) {
+    const hipStream_t s = ws.stream;
     const int n_seg = (int) (nw - n_traj);
-    sweep_begin(c, s);
+    sweep_begin(ws);
     fill_u32(d_out, n_traj, 0u, s);
     if (d_seg && n_seg > 0) fill_u32(d_seg, (size_t) n_seg, 0u, s);
-    enqueue_centers(c, d_traj, (int) nw, s);
-    b.seg_traj.reserve(std::max(n_seg, 1));
-    b.seg_wp.reserve(std::max(n_seg, 1));
+    enqueue_centers(c, ws, d_traj, (int) nw);
+    ws.seg_traj.reserve(std::max(n_seg, 1));
+    ws.seg_wp.reserve(std::max(n_seg, 1));
+    budget_check_ws(c, ws, budget);
     hipLaunchKernelGGL(k_seg_maps, dim3(blocks_for(n_traj, kBS)), dim3(kBS), 0, s, d_offs, (int) n_traj, n_seg,
-                       b.seg_traj.p, b.seg_wp.p);
+                       ws.seg_traj.p, ws.seg_wp.p);
     RTCD_LAUNCH_CHECK();
-    SweepArgs a = base_args(c, sc, d_out, d_seg);
+    SweepArgs a = base_args(c, ws, sc, d_out, d_seg);
     a.n_traj = (int) n_traj;
     a.n_wp_total = (int) nw;
-    a.seg_traj = b.seg_traj.p;
-    a.seg_wp = b.seg_wp.p;
+    a.seg_traj = ws.seg_traj.p;
+    a.seg_wp = ws.seg_wp.p;
     a.n_seg = n_seg;
     a.n_ctrl = 2;
     const int depth = sc.edge_bvh.depth;
-    if (n_seg > 0) launch_sweep(a, 1, depth + 2 <= 32 ? 32 : 64, c->counting, s, b.c32.p, nullptr);
-    sweep_end(c, s, depth);
+    if (n_seg > 0) launch_sweep(a, 1, depth + 2 <= 32 ? 32 : 64, ws.counting, s, ws.c32.p, nullptr);
+    sweep_end(ws);
 }
 
-void enqueue_quadratic(rtcd_ctx* c, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw, uint32_t nc,
-                       const double* W, uint32_t* d_out, uint32_t* d_seg, hipStream_t s) {
-    SweepBuffers& b = c->sweep;
-    const int S = b.n_spheres;
+void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
+                       uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget) {
+    const hipStream_t s = ws.stream;
+    const int S = c->n_spheres;
     const int npc = (int) nc - 2;
     std::vector<double> Wh((size_t) nc * nw);
     if (W) std::copy(W, W + Wh.size(), Wh.begin());
     else quadratic_weights((int) nc, (int) nw, Wh.data());
-    sweep_begin(c, s);
-    b.W64.reserve(Wh.size());
-    RTCD_HIP(hipMemcpyAsync(b.W64.p, Wh.data(), sizeof(double) * Wh.size(), hipMemcpyHostToDevice, s));
+    sweep_begin(ws);
+    if (ws.W_host != Wh) {
+        ws.W_host = Wh;  // kept alive in the workspace for the asynchronous copy
+        ws.W64.upload(ws.W_host, s);
+    }
     fill_u32(d_out, n_traj, 0u, s);
     if (d_seg) fill_u32(d_seg, (size_t) n_traj * npc, 0u, s);
-    enqueue_centers(c, d_wp, (int) (n_traj * nw), s);
-    b.P64.reserve((size_t) n_traj * S * nc * 3);
-    b.P32.reserve((size_t) n_traj * S * nc);
+    enqueue_centers(c, ws, d_wp, (int) (n_traj * nw));
+    ws.P64.reserve((size_t) n_traj * S * nc * 3);
+    ws.P32.reserve((size_t) n_traj * S * nc);
+    budget_check_ws(c, ws, budget);
     if (n_traj * S > 0) {
-        hipLaunchKernelGGL(k_fit, dim3(blocks_for((size_t) n_traj * S, kBS)), dim3(kBS), 0, s, b.c64.p, b.c32.p,
-                           b.W64.p, (int) n_traj, (int) nw, (int) nc, S, b.P64.p, b.P32.p);
+        hipLaunchKernelGGL(k_fit, dim3(blocks_for((size_t) n_traj * S, kBS)), dim3(kBS), 0, s, ws.c64.p, ws.c32.p,
+                           ws.W64.p, (int) n_traj, (int) nw, (int) nc, S, ws.P64.p, ws.P32.p);
         RTCD_LAUNCH_CHECK();
     }
-    SweepArgs a = base_args(c, sc, d_out, d_seg);
+    SweepArgs a = base_args(c, ws, sc, d_out, d_seg);
     a.n_traj = (int) n_traj;
     a.n_wp = (int) nw;
     a.n_ctrl = (int) nc;
-    a.W = b.W64.p;
-    a.ctrl64 = b.P64.p;
+    a.W = ws.W64.p;
+    a.ctrl64 = ws.P64.p;
     a.n_seg = (int) n_traj * npc;
     const int depth = sc.edge_bvh.depth;
-    if (a.n_seg > 0) launch_sweep(a, 2, depth + 2 <= 32 ? 32 : 64, c->counting, s, b.c32.p, b.P32.p);
-    sweep_end(c, s, depth);
+    if (a.n_seg > 0) launch_sweep(a, 2, depth + 2 <= 32 ? 32 : 64, ws.counting, s, ws.c32.p, ws.P32.p);
+    sweep_end(ws);
 }
 
-rtcd_status sweep_precheck(rtcd_ctx* ctx, uint32_t scene_id, SceneRes** sc) {
+}  // namespace rtcd
+
+namespace {
+rtcd_status sweep_precheck(rtcd_ctx* ctx, uint32_t scene_id, std::shared_ptr<SceneRes>* sc) {
     if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
     if (!ctx->robot_loaded) return fail(RTCD_ERR_NOT_READY, "robot not loaded");
     *sc = find_scene(ctx, scene_id);
@@ -500,17 +516,22 @@ rtcd_status sweep_precheck(rtcd_ctx* ctx, uint32_t scene_id, SceneRes** sc) {
     return RTCD_OK;
 }
 
-}  // namespace
-
-namespace rtcd {
-void sweep_robot_loaded(SweepBuffers& b, const float4* sph, int n, hipStream_t s) {
-    b.spheres = sph;
-    b.n_spheres = n;
-    b.d_sphere_link.alloc(std::max(n, 1));
-    if (n) b.d_sphere_link.upload(b.sphere_link, s);
-    RTCD_HIP(hipStreamSynchronize(s));
+// Runs f on the main workspace bound to `stream` (or its own stream).
+template <typename F>
+void on_main(rtcd_ctx* ctx, void* stream, F&& f) {
+    Workspace& ws = ctx->main;
+    hipStream_t own = ws.stream;
+    if (stream) ws.stream = (hipStream_t) stream;
+    ws.counting = ctx->counting;
+    try {
+        f(ws);
+    } catch (...) {
+        ws.stream = own;
+        throw;
+    }
+    ws.stream = own;
 }
-}  // namespace rtcd
+}  // namespace
 
 extern "C" {
 
@@ -526,23 +547,24 @@ rtcd_status rtcd_quadratic_weights(uint32_t n_ctrl, uint32_t n_wp, double* W) {
 rtcd_status rtcd_check_linear_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_traj, const uint32_t* d_offsets,
                                      uint32_t n_traj, uint32_t n_wp_total, uint32_t* d_out, uint32_t* d_out_seg,
                                      void* stream) {
-    SceneRes* sc = nullptr;
+    std::shared_ptr<SceneRes> sc;
     if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
     if (n_traj && (!d_traj || !d_offsets || !d_out)) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
     if (n_wp_total < n_traj) return fail(RTCD_ERR_INVALID_ARG, "every trajectory needs at least one waypoint");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = stream ? (hipStream_t) stream : ctx->stream;
         if (n_traj == 0) return RTCD_OK;
-        enqueue_linear(ctx, *sc, d_traj, d_offsets, n_traj, n_wp_total, d_out, d_out_seg, s);
+        on_main(ctx, stream, [&](Workspace& ws) {
+            enqueue_linear(ctx, ws, *sc, d_traj, d_offsets, n_traj, n_wp_total, d_out, d_out_seg, 0);
+        });
         return RTCD_OK;
     });
 }
 
 rtcd_status rtcd_check_linear(rtcd_ctx* ctx, uint32_t scene_id, const float* traj, const uint32_t* offsets,
                               uint32_t n_traj, uint32_t* out, uint32_t* out_seg) {
-    SceneRes* sc = nullptr;
+    std::shared_ptr<SceneRes> sc;
     if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
     if (n_traj == 0) return RTCD_OK;
     if (!traj || !offsets || !out) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
@@ -554,18 +576,19 @@ rtcd_status rtcd_check_linear(rtcd_ctx* ctx, uint32_t scene_id, const float* tra
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = ctx->stream;
-        SweepBuffers& b = ctx->sweep;
+        Workspace& b = ctx->main;
+        b.counting = ctx->counting;
+        const hipStream_t s = b.stream;
         const uint32_t n_seg = nw - n_traj;
         b.h_poses.upload(traj, (size_t) nw * ctx->robot.dof, s);
         b.h_offsets.upload(offsets, n_traj + 1, s);
         b.h_out.reserve(n_traj);
         b.h_seg.reserve(std::max<uint32_t>(n_seg, 1));
-        enqueue_linear(ctx, *sc, b.h_poses.p, b.h_offsets.p, n_traj, nw, b.h_out.p, out_seg ? b.h_seg.p : nullptr, s);
+        enqueue_linear(ctx, b, *sc, b.h_poses.p, b.h_offsets.p, n_traj, nw, b.h_out.p, out_seg ? b.h_seg.p : nullptr, 0);
         RTCD_HIP(hipMemcpyAsync(out, b.h_out.p, sizeof(uint32_t) * n_traj, hipMemcpyDeviceToHost, s));
         if (out_seg && n_seg)
             RTCD_HIP(hipMemcpyAsync(out_seg, b.h_seg.p, sizeof(uint32_t) * n_seg, hipMemcpyDeviceToHost, s));
-        sweep_finish(ctx, s);
+        finish_sweep(ctx, b, &ctx->last);
         return RTCD_OK;
     });
 }
@@ -573,7 +596,7 @@ rtcd_status rtcd_check_linear(rtcd_ctx* ctx, uint32_t scene_id, const float* tra
 rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_wp, uint32_t n_traj,
                                         uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
                                         uint32_t* d_out_seg, void* stream) {
-    SceneRes* sc = nullptr;
+    std::shared_ptr<SceneRes> sc;
     if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
     if (n_ctrl < 3 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)
         return fail(RTCD_ERR_INVALID_ARG, "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
@@ -581,16 +604,17 @@ rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const 
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = stream ? (hipStream_t) stream : ctx->stream;
         if (n_traj == 0) return RTCD_OK;
-        enqueue_quadratic(ctx, *sc, d_wp, n_traj, n_wp, n_ctrl, W, d_out, d_out_seg, s);
+        on_main(ctx, stream, [&](Workspace& ws) {
+            enqueue_quadratic(ctx, ws, *sc, d_wp, n_traj, n_wp, n_ctrl, W, d_out, d_out_seg, 0);
+        });
         return RTCD_OK;
     });
 }
 
 rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* wp, uint32_t n_traj, uint32_t n_wp,
                                  uint32_t n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
-    SceneRes* sc = nullptr;
+    std::shared_ptr<SceneRes> sc;
     if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
     if (n_ctrl < 3 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)
         return fail(RTCD_ERR_INVALID_ARG, "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
@@ -600,16 +624,17 @@ rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* 
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = ctx->stream;
-        SweepBuffers& b = ctx->sweep;
+        Workspace& b = ctx->main;
+        b.counting = ctx->counting;
+        const hipStream_t s = b.stream;
         const size_t npc = n_ctrl - 2;
         b.h_poses.upload(wp, (size_t) n_traj * n_wp * ctx->robot.dof, s);
         b.h_out.reserve(n_traj);
         b.h_seg.reserve(n_traj * npc);
-        enqueue_quadratic(ctx, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, out_seg ? b.h_seg.p : nullptr, s);
+        enqueue_quadratic(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, out_seg ? b.h_seg.p : nullptr, 0);
         RTCD_HIP(hipMemcpyAsync(out, b.h_out.p, sizeof(uint32_t) * n_traj, hipMemcpyDeviceToHost, s));
         if (out_seg) RTCD_HIP(hipMemcpyAsync(out_seg, b.h_seg.p, sizeof(uint32_t) * n_traj * npc, hipMemcpyDeviceToHost, s));
-        sweep_finish(ctx, s);
+        finish_sweep(ctx, b, &ctx->last);
         return RTCD_OK;
     });
 }
