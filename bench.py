@@ -25,14 +25,51 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes ("Measurement"): node record 64 B, triangle record 48 B, edge
-# record 32 B, per candidate 8 B (pose/link key + triangle id) + 48 B (scene triangle) + 48 B
-# (inverse link transform), per pose 28 B (7 fp32 joints) + 4 B (flag).
-S_NODE, S_TRI, S_EDGE, S_CAND, S_POSE = 64, 48, 32, 8 + 48 + 48, 28 + 4
+# Algorithmic bytes of the narrow kernels ("Measurement"): node record 64 B, triangle
+# record 48 B, edge record 32 B, per leaf candidate 8 B (record) + 48 B (link transform rows), per
+# candidate triangle 48 B (scene triangle), per direction pass.
+S_NODE, S_TRI, S_EDGE, S_LEAF, S_CTRI = 64, 48, 32, 8 + 48, 48
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def _boxes8():
+    from rtcollisiondetection_amd import scenes
+    s = scenes.scene_boxes8()
+    return s.verts, s.tris
+
+
+def serve_bench(ctx, rb, scene_ids, rate=2000.0, duration=2.0):
+    """Serving stream: Poisson arrivals at `rate` req/s, 50/30/20 discrete/linear/quadratic, sizes
+    log-uniform 1..8,192, three resident scenes; submit->result latency, and the same requests run
+    one by one through the blocking entry points (the plain per-request launch)."""
+    from rtcollisiondetection_amd import _lib
+    from rtcollisiondetection_amd.serving import Server, make_stream, run_alone, run_stream
+    reqs = make_stream(rb, scene_ids, rate_hz=rate, duration_s=duration, seed=11)
+    items = sum(r.n for r in reqs)
+    srv = Server(ctx, slots=4)
+    try:
+        res, wall = run_stream(srv, reqs, realtime=True)
+        st = srv.stats()
+    finally:
+        srv.stop()
+    lat = np.array([x[2] for x in res])
+    d1k = np.array([x[2] for x in res if x[0] == _lib.REQ_DISCRETE and x[1] <= 1024])
+    served_out = [r.out.copy() for r in reqs]
+    t = time.perf_counter()
+    mism = 0
+    for r, o in zip(reqs, served_out):
+        mism += int(np.sum(run_alone(ctx, r).astype(bool) != o.astype(bool)))
+    plain_wall = time.perf_counter() - t
+    return {'requests': len(reqs), 'items': int(items), 'rate_hz': rate, 'duration_s': duration, 'slots': 4,
+            'p50_ms': round(float(np.percentile(lat, 50)), 3), 'p99_ms': round(float(np.percentile(lat, 99)), 3),
+            'p99_ms_discrete_le_1024': round(float(np.percentile(d1k, 99)), 3) if len(d1k) else None,
+            'served_items_per_s': round(items / wall, 1), 'wall_s': round(wall, 3),
+            'plain_per_request_items_per_s': round(items / plain_wall, 1),
+            'served_over_plain': round(plain_wall / wall, 2), 'mismatches_alone_vs_mix': mism,
+            'queued_max': int(st['queued_max'])}
 
 
 def main():
@@ -93,7 +130,7 @@ def main():
     ctx.set_counting(False)
     prim_tri = cst['prim_tests'] - cst['edge_tests']
     bytes_launch = (S_NODE * cst['node_fetches'] + S_TRI * prim_tri + S_EDGE * cst['edge_tests'] +
-                    S_CAND * cst['candidates'])
+                    2 * (S_LEAF * cst['leaf_candidates'] + S_CTRI * cst['candidates']))
     nar = float(np.mean(narrow_ms))
     achieved = bytes_launch / (nar * 1e-3) / 1e9
     traffic = None
@@ -108,7 +145,8 @@ def main():
                 'kernel': 'k_narrow (narrow phase)', 'kernel_ms': round(nar, 4),
                 'algorithmic_bytes': int(bytes_launch), 'node_fetches': int(cst['node_fetches']),
                 'prim_tests': int(cst['prim_tests']), 'edge_tests': int(cst['edge_tests']),
-                'candidates': int(cst['candidates']), 'fp64_redecisions': int(cst['fp64_redecisions']),
+                'candidate_tris': int(cst['candidates']), 'leaf_candidates': int(cst['leaf_candidates']),
+                'fp64_redecisions': int(cst['fp64_redecisions']),
                 'stage_ms': [round(x, 4) for x in st['stage_ms']]}
     log(f'[bench] roofline {roofline}')
 
@@ -148,6 +186,8 @@ def main():
         extra['quadratic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
         extra['quadratic_ms_device'] = round(ctx.stats()['last_ms'], 3)
         log(f'[bench] extra sweeps {extra}')
+        extra['serving'] = serve_bench(ctx, rb, [ctx.create_scene(*_boxes8()), sid2, sid])
+        log(f'[bench] serving {extra["serving"]}')
 
     cpu = None
     if not args.no_cpu:

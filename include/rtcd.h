@@ -65,12 +65,13 @@ typedef struct {
     uint64_t node_fetches;   /* BVH nodes visited (counting launches only) */
     uint64_t prim_tests;     /* primitive tests (segment-triangle, segment-tube) */
     uint64_t fp64_redecisions; /* tests whose FP32 result fell in the error band */
-    uint64_t candidates;     /* broad-phase (pose, link, triangle) / (segment, edge) candidates */
+    uint64_t candidates;     /* discrete: triangles in the broad-phase leaves; sweeps: (segment, sphere) tubes checked */
     double last_ms;          /* device time of the last check call, ms */
     uint32_t max_stack;      /* traversal stack size used by the last call */
     uint32_t max_depth;      /* deepest BVH involved in the last call */
     uint64_t edge_tests;     /* of prim_tests: tests that read a 32 B edge record (the rest read 48 B) */
-    double stage_ms[5];      /* discrete: FK, broad count, scan, broad emit, narrow (device time, ms) */
+    double stage_ms[5];      /* discrete: FK, broad, (unused), (unused), narrow + FP64 re-decision (device ms) */
+    uint64_t leaf_candidates; /* discrete: (pose, link, scene leaf) records emitted by the broad phase */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */

@@ -24,11 +24,53 @@ void budget_check(const rtcd_ctx* c, const Workspace& ws, size_t budget, size_t 
 
 namespace rtcd {
 
+// Frontier levels of a BVH: level 0 = the root's children, each next level expands every inner
+// entry into its two children (leaves carry over), up to 1024 entries.
+static void build_frontiers(const std::vector<BNode>& nodes, std::vector<FEntry>& all,
+                            std::vector<std::pair<int, int>>& levels) {
+    all.clear();
+    levels.clear();
+    if (nodes.empty()) return;
+    auto child = [&](const BNode& nd, int k) {
+        FEntry e;
+        if (k == 0) {
+            e.lo[0] = nd.lo0x; e.lo[1] = nd.lo0y; e.lo[2] = nd.lo0z; e.hi[0] = nd.hi0x; e.hi[1] = nd.hi0y; e.hi[2] = nd.hi0z;
+            e.c = nd.c0; e.n = nd.n0;
+        } else {
+            e.lo[0] = nd.lo1x; e.lo[1] = nd.lo1y; e.lo[2] = nd.lo1z; e.hi[0] = nd.hi1x; e.hi[1] = nd.hi1y; e.hi[2] = nd.hi1z;
+            e.c = nd.c1; e.n = nd.n1;
+        }
+        return e;
+    };
+    std::vector<FEntry> cur;
+    for (int k = 0; k < 2; ++k) {
+        FEntry e = child(nodes[0], k);
+        if (e.c >= 0) cur.push_back(e);
+    }
+    for (;;) {
+        levels.push_back({(int) all.size(), (int) cur.size()});
+        all.insert(all.end(), cur.begin(), cur.end());
+        std::vector<FEntry> nxt;
+        bool inner = false;
+        for (const FEntry& e : cur) {
+            if (e.n > 0) { nxt.push_back(e); continue; }
+            inner = true;
+            for (int k = 0; k < 2; ++k) {
+                FEntry c = child(nodes[e.c], k);
+                if (c.c >= 0) nxt.push_back(c);
+            }
+        }
+        if (!inner || nxt.size() > 1024) break;
+        cur.swap(nxt);
+    }
+}
+
 size_t discrete_bytes(const rtcd_ctx* c, uint32_t n) {
     const size_t NL = (size_t) c->robot.n_links, tasks = (NL - (c->cfg.skip_base_link ? 1 : 0)) * n;
-    const size_t cap = std::max<size_t>((size_t) 1 << 22, tasks * 64);
-    return n * NL * (12 * 8 + 3 * 16 + 2 * 16) + 2 * (tasks + 1) * 4 + scan_tmp_words(tasks + 1) * 4 + cap * 8 +
-           (size_t) n * (4 * c->robot.dof + 4) + 64;
+    const size_t cap = std::max<size_t>((size_t) 1 << 20, tasks * 64);
+    const size_t unsure = std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
+    return n * NL * (12 * 8 + 3 * 16 + 2 * 16) + cap * 8 +
+           unsure * 16 + (size_t) n * (4 * c->robot.dof + 4) + 64;
 }
 
 // Enqueue FK -> broad(count) -> scan -> broad(emit) -> narrow on ws.stream.  No host synchronisation.
@@ -39,25 +81,42 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     const int l0 = c->cfg.skip_base_link ? 1 : 0;
     const int n_active = NL - l0;
     const size_t tasks = (size_t) n_active * n;
-    const size_t need_cap = std::max<size_t>((size_t) 1 << 22, tasks * 64);
+    // split each (pose, link) task over F frontier entries so that small batches still fill the GPU
+    // (large batches already fill the device: one thread per task walks the whole tree)
+    int F = 1, foff = -1;
+    if (tasks < ((size_t) 1 << 17)) {
+        for (const auto& lv : sc.front_levels)
+            if (tasks * (size_t) lv.second <= ((size_t) 1 << 23) || foff < 0) { F = lv.second; foff = lv.first; }
+    } else if (!sc.front_levels.empty()) {
+        F = sc.front_levels[0].second;  // the root's children
+        foff = sc.front_levels[0].first;
+    }
+    if (foff < 0) { F = 1; foff = 0; }
+    const size_t items = tasks * (size_t) F;
+    const size_t need_cap = std::max<size_t>((size_t) 1 << 20, tasks * 64);
     if (ws.cand.n < need_cap) {
         budget_check(c, ws, budget, (need_cap - ws.cand.n) * sizeof(uint2));
         ws.cand.reserve(need_cap);
     }
     ws.cand_cap = (uint32_t) std::min<size_t>(ws.cand.n, 0xffffffffu);
+    const size_t need_unsure = std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
+    if (ws.unsure.n < need_unsure) {
+        budget_check(c, ws, budget, (need_unsure - ws.unsure.n) * sizeof(uint4));
+        ws.unsure.reserve(need_unsure);
+    }
     const size_t npl = (size_t) n * NL;
     budget_check(c, ws, budget, 0);
     ws.T64.reserve(npl * 12);
     ws.Ti.reserve(npl * 3);
     ws.wbox.reserve(npl * 2);
-    ws.counts.reserve(tasks + 1);
-    ws.offsets.reserve(tasks + 1);
-    ws.scan_tmp.reserve(scan_tmp_words(tasks + 1));
-    ws.scalars.reserve(4);
+    (void) items;
+    ws.scalars.reserve(8);
     ws.stats.reserve(1);
 
+    ws.seg_count.reserve(kCandSegments);
     fill_u32(d_out, n, 0u, s);
-    fill_u32(ws.scalars.p, 4, 0u, s);
+    fill_u32(ws.scalars.p, 8, 0u, s);
+    fill_u32(ws.seg_count.p, kCandSegments, 0u, s);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, s);
     RTCD_HIP(hipEventRecord(ws.ev[0], s));
     launch_fk(c->d_robot.p, c->robot.dof, NL, d_poses, (int) n, ws.T64.p, ws.Ti.p, ws.wbox.p, s);
@@ -73,25 +132,27 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.T64 = ws.T64.p;
     a.Ti = ws.Ti.p;
     a.wbox = ws.wbox.p;
-    a.counts = ws.counts.p;
-    a.offsets = ws.offsets.p;
     a.cand = ws.cand.p;
-    a.cap = ws.cand_cap;
-    a.total = ws.scalars.p;
+    a.seg_cap = (uint32_t) (ws.cand_cap / kCandSegments);
+    a.seg_count = ws.seg_count.p;
+    a.tri_candidates = reinterpret_cast<unsigned long long*>(ws.scalars.p + 4);
     a.overflow = ws.scalars.p + 1;
     a.flags = d_out;
     a.stats = ws.stats.p;
     a.broad_pad = 1e-5f * (1.f + sc.scale);
-    const int depth = std::max(c->robot_depth, std::max(sc.tri_bvh.depth, sc.edge_bvh.depth));
-    const int stack = stack_for(depth);
+    a.leaf_lanes = (int) c->cfg.leaf_size;
+    a.front = sc.front.p + foff;
+    a.F = sc.front_levels.empty() ? 0 : F;
+    a.unsure = ws.unsure.p;
+    a.unsure_cap = (uint32_t) std::min<size_t>(ws.unsure.n, 0xffffffffu);
+    a.unsure_count = ws.scalars.p + 2;
+    const int broad_stack = stack_for(sc.tri_bvh.depth);
     RTCD_HIP(hipEventRecord(ws.ev[1], s));
-    launch_broad(a, stack, false, s);
+    if (a.F > 0) launch_broad(a, broad_stack, s);
     RTCD_HIP(hipEventRecord(ws.ev[2], s));
-    exclusive_scan(ws.counts.p, ws.offsets.p, tasks, ws.scalars.p, ws.scan_tmp.p, s);
     RTCD_HIP(hipEventRecord(ws.ev[3], s));
-    launch_broad(a, stack, true, s);
     RTCD_HIP(hipEventRecord(ws.ev[4], s));
-    launch_narrow(a, stack, ws.counting, s);
+    launch_narrow(a, c->robot_depth + 1, ws.counting, s);
     RTCD_HIP(hipEventRecord(ws.ev[5], s));
     ws.timing_pending = true;
 }
@@ -100,9 +161,19 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                      uint32_t* d_out, size_t budget, rtcd_stats* st) {
     const hipStream_t s = ws.stream;
     for (int attempt = 0; attempt < 4; ++attempt) {
-        std::vector<uint32_t> sv = ws.scalars.download(2, s);
+        std::vector<uint32_t> sv = ws.scalars.download(6, s);
+        {
+            // total leaf candidates and the fullest segment (sizes the re-run on overflow)
+            std::vector<uint32_t> segc = ws.seg_count.download(kCandSegments, s);
+            uint64_t tot = 0;
+            uint32_t mx = 0;
+            for (uint32_t v : segc) { tot += v; mx = std::max(mx, v); }
+            sv[0] = (uint32_t) std::min<uint64_t>(tot, 0xffffffffu);
+            if (mx > ws.cand_cap / kCandSegments) sv[1] |= 1u;
+            sv.push_back(mx);
+        }
         if (sv[1] & 2u) throw std::runtime_error("traversal stack overflow (BVH deeper than the stack)");
-        if (!(sv[1] & 1u)) {
+        if (!(sv[1] & 5u)) {
             if (st) {
                 if (ws.timing_pending) {
                     float ms = 0.f;
@@ -114,10 +185,10 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                         st->stage_ms[k] = hipEventElapsedTime(&sm, b[k], b[k + 1]) == hipSuccess ? sm : -1.0;
                     }
                 }
-                st->candidates = sv[0];
-                const int depth = std::max(c->robot_depth, std::max(sc.tri_bvh.depth, sc.edge_bvh.depth));
-                st->max_depth = (uint32_t) depth;
-                st->max_stack = (uint32_t) stack_for(depth);
+                st->candidates = (uint64_t) sv[4] | ((uint64_t) sv[5] << 32);  // triangles in emitted leaves
+                st->leaf_candidates = sv[0];
+                st->max_depth = (uint32_t) std::max(c->robot_depth, sc.tri_bvh.depth);
+                st->max_stack = (uint32_t) stack_for(sc.tri_bvh.depth);
                 if (ws.counting) {
                     Stats k = ws.stats.download(1, s)[0];
                     st->node_fetches = k.node_fetches;
@@ -129,10 +200,17 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
             ws.timing_pending = false;
             return;
         }
-        const size_t need = (size_t) sv[0] + sv[0] / 4 + 1024;
-        if (need > 0xfffffff0ull) throw std::runtime_error("candidate count exceeds 2^32");
-        budget_check(c, ws, budget, need > ws.cand.n ? (need - ws.cand.n) * sizeof(uint2) : 0);
-        ws.cand.alloc(need);
+        if (sv[1] & 1u) {
+            const size_t need = ((size_t) sv[6] + sv[6] / 2 + 64) * kCandSegments;
+            if (need > 0xfffffff0ull) throw std::runtime_error("candidate count exceeds 2^32");
+            budget_check(c, ws, budget, need > ws.cand.n ? (need - ws.cand.n) * sizeof(uint2) : 0);
+            ws.cand.alloc(need);
+        }
+        if (sv[1] & 4u) {
+            const size_t need = (size_t) sv[2] + sv[2] / 4 + 1024;
+            budget_check(c, ws, budget, need > ws.unsure.n ? (need - ws.unsure.n) * sizeof(uint4) : 0);
+            ws.unsure.alloc(need);
+        }
         enqueue_discrete(c, ws, sc, dir, d_poses, n, d_out, budget);
         RTCD_HIP(hipStreamSynchronize(s));
     }
@@ -324,6 +402,7 @@ rtcd_status rtcd_scene_create(rtcd_ctx* ctx, const float* verts, uint32_t n_vert
                               uint32_t n_tris, uint32_t* scene_id) {
     if (!ctx || !scene_id || (n_verts && !verts) || (n_tris && !tris)) return fail(RTCD_ERR_INVALID_ARG, "null argument");
     if (!finite_all(verts, 3 * (size_t) n_verts)) return fail(RTCD_ERR_INVALID_ARG, "non-finite scene vertex");
+    if (n_tris >= (1u << 27)) return fail(RTCD_ERR_INVALID_ARG, "scene too large (max 2^27 - 1 triangles)");
     for (size_t i = 0; i < 3 * (size_t) n_tris; ++i)
         if (tris[i] < 0 || (uint32_t) tris[i] >= n_verts) return fail(RTCD_ERR_INVALID_ARG, "scene triangle index out of range");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
@@ -359,6 +438,13 @@ rtcd_status rtcd_scene_create(rtcd_ctx* ctx, const float* verts, uint32_t n_vert
         }
         std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->cfg.leaf_size, s, sc->tri_bvh);
         std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->cfg.leaf_size, s, sc->edge_bvh);
+        {
+            std::vector<BNode> hn = sc->tri_bvh.nodes.download(sc->tri_bvh.n_nodes, s);
+            std::vector<FEntry> fr;
+            build_frontiers(hn, fr, sc->front_levels);
+            sc->front.alloc(std::max<size_t>(fr.size(), 1));
+            if (!fr.empty()) sc->front.upload(fr, s);
+        }
         std::vector<Tri> tri_s(nt);
         for (int i = 0; i < nt; ++i) tri_s[i] = tri[pt[i]];
         std::vector<Edge> edg_s(ne);

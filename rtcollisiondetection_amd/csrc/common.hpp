@@ -40,6 +40,14 @@ struct alignas(16) BNode {
 };
 static_assert(sizeof(BNode) == 64, "BNode is 64 B");
 
+// One child reference with its box: an entry of a BVH frontier (a cut of the tree that covers every
+// leaf exactly once).  n > 0: leaf with primitives [c, c + n); n == 0: inner node c.
+struct alignas(16) FEntry {
+    float lo[3], hi[3];
+    int32_t c, n;
+};
+static_assert(sizeof(FEntry) == 32, "FEntry is 32 B");
+
 // Triangle, exact fp32 vertices (48 B).  w of v0 carries the original triangle id (as int bits),
 // w of v1 the owned-edge mask (bit j: edge j of the loop v0->v1, v1->v2, v2->v0 is a unique edge
 // first seen on this triangle), w of v2 is unused.

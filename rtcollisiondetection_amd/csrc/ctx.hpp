@@ -52,6 +52,8 @@ inline std::vector<int32_t> build_bvh(const std::vector<float4>& boxes, int n, i
 
 struct SceneRes {
     BvhOut tri_bvh, edge_bvh;
+    DevBuf<FEntry> front;                         // frontier levels of the triangle BVH, concatenated
+    std::vector<std::pair<int, int>> front_levels; // (offset, size): sizes 2, 4, ... up to 1024
     DevBuf<Tri> tris;
     DevBuf<Edge> edges;
     int n_tri = 0, n_edge = 0;
@@ -83,9 +85,11 @@ struct Workspace {
     // discrete
     DevBuf<double> T64;
     DevBuf<float4> Ti, wbox;
-    DevBuf<uint32_t> counts, offsets, scan_tmp, scalars;  // scalars: [0] total, [1] overflow
+    DevBuf<uint32_t> counts, offsets, scan_tmp, scalars;  // scalars: [0] candidates, [1] overflow bits, [2] queued FP64 tests
     DevBuf<uint2> cand;
     uint32_t cand_cap = 0;
+    DevBuf<uint32_t> seg_count;  // [kCandSegments]
+    DevBuf<uint4> unsure;  // FP64 re-decision queue
     // host staging
     DevBuf<float> h_poses;
     DevBuf<uint32_t> h_offsets, h_out, h_seg;
@@ -104,7 +108,7 @@ struct Workspace {
 
     size_t bytes() const {
         return T64.bytes() + Ti.bytes() + wbox.bytes() + counts.bytes() + offsets.bytes() + scan_tmp.bytes() +
-               scalars.bytes() + cand.bytes() + h_poses.bytes() + h_offsets.bytes() + h_out.bytes() + h_seg.bytes() +
+               scalars.bytes() + cand.bytes() + unsure.bytes() + h_poses.bytes() + h_offsets.bytes() + h_out.bytes() + h_seg.bytes() +
                c64.bytes() + c32.bytes() + seg_traj.bytes() + seg_wp.bytes() + P64.bytes() + W64.bytes() + P32.bytes();
     }
     void init(hipStream_t s, bool own) {

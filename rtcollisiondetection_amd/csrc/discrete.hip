@@ -16,6 +16,7 @@ namespace rtcd {
 namespace {
 
 constexpr int kBS = 256;
+constexpr int kSeg = kCandSegments;
 
 __device__ __forceinline__ float3 f3(float4 v) { return make_float3(v.x, v.y, v.z); }
 __device__ __forceinline__ float3 sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
@@ -96,81 +97,134 @@ __device__ __forceinline__ bool broad_test(float lx, float ly, float lz, float h
            fabsf(c.z - lc.z) <= hz2 + lh.z + pad;
 }
 
-template <int STACK, bool EMIT>
+// Appends `want` (> 0) records for every calling lane: the lanes of one wave that reach this call
+// together add to the wave's LDS counter and the first of them reserves the wave total with one
+// global atomic.  Returns this lane's first slot.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* wcnt, uint32_t* gcount, uint32_t want) {
+    const uint64_t act = __ballot(1);
+    const int leader = __ffsll((unsigned long long) act) - 1;
+    const int lane = (int) (threadIdx.x & 63);
+    const uint32_t local = atomicAdd(wcnt, want);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(gcount, atomicExch(wcnt, 0u));
+    base = (uint32_t) __shfl((int) base, leader, 64);
+    return base + local;
+}
+
+// Leaf candidate record: (pose * NL + link, first triangle | (count - 1) << 27).
+__device__ __forceinline__ uint32_t pack_leaf(int start, int n) { return (uint32_t) start | ((uint32_t) (n - 1) << 27); }
+
+// One pass: every (pose, link) task, split over F frontier entries, walks the scene triangle LBVH
+// and appends each leaf that passes both box tests to the candidate list.
+template <int STACK>
 __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a) {
     __shared__ int stk[STACK * kBS];
+    __shared__ uint32_t wcnt[kBS / 64];
+    __shared__ uint32_t tris_block;
     int* st = stk + threadIdx.x;
-    const int task = blockIdx.x * kBS + threadIdx.x;
-    if (task >= a.n_active * a.n_poses) return;
-    const int li = task / a.n_poses, p = task - li * a.n_poses, l = a.l0 + li;
-    const size_t pl = (size_t) p * a.n_links + l;
-    const float4 wlo = a.wbox[pl * 2], whi = a.wbox[pl * 2 + 1];
-    const float4 r0 = a.Ti[pl * 3], r1 = a.Ti[pl * 3 + 1], r2 = a.Ti[pl * 3 + 2];
-    const LinkInfo& L = a.robot->link[l];
-    const float3 lc = make_float3(L.c[0], L.c[1], L.c[2]);
-    const float3 lh = make_float3(0.5f * (L.hi[0] - L.lo[0]), 0.5f * (L.hi[1] - L.lo[1]), 0.5f * (L.hi[2] - L.lo[2]));
-    const float pad = a.broad_pad;
-    uint32_t count = 0;
-    uint32_t base = 0;
-    if (EMIT) base = a.offsets[task];
-    const uint32_t key = (uint32_t) pl;
-    if (a.sc.tri_has_root && (L.n_tri > 0 || L.n_edge > 0)) {
-        int sp = 0, ni = 0;
-        for (;;) {
-            const BNode nd = a.sc.tri_nodes[ni];
-            bool h0 = nd.c0 >= 0 && broad_test(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, wlo, whi, r0, r1,
-                                               r2, lc, lh, pad);
-            bool h1 = nd.c1 >= 0 && broad_test(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, wlo, whi, r0, r1,
-                                               r2, lc, lh, pad);
-            if (h0 && nd.n0 > 0) {
-                if (EMIT)
-                    for (int k = 0; k < nd.n0; ++k) {
-                        uint32_t o = base + count + k;
-                        if (o < a.cap) a.cand[o] = make_uint2(key, (uint32_t) (nd.c0 + k));
+    if (threadIdx.x < kBS / 64) wcnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) tris_block = 0;
+    __syncthreads();
+    uint32_t* my_wcnt = wcnt + (threadIdx.x >> 6);
+    // Block b owns candidate segment b and walks one contiguous chunk of the (task, frontier entry)
+    // items, so the segments in order keep the link-major task order (poses flagged through an
+    // earlier link skip the later ones in the narrow phase) and each counter is shared by one block.
+    uint32_t* seg_count = a.seg_count + blockIdx.x;
+    uint2* seg_cand = a.cand + (size_t) blockIdx.x * a.seg_cap;
+    const size_t n_items = (size_t) a.n_active * a.n_poses * a.F;
+    const size_t chunk = ((n_items + gridDim.x - 1) / gridDim.x + kBS - 1) / kBS * kBS;
+    const size_t beg = (size_t) blockIdx.x * chunk, end = min(beg + chunk, n_items);
+    uint32_t ntris = 0;
+    for (size_t id = beg + threadIdx.x; id < end; id += kBS) {
+        // item = (task, frontier entry); task = (link-major) (link, pose)
+        const int task = (int) (id / (size_t) a.F), f = (int) (id - (size_t) task * a.F);
+        const int li = task / a.n_poses, p = task - li * a.n_poses, l = a.l0 + li;
+        const size_t pl = (size_t) p * a.n_links + l;
+        const float4 wlo = a.wbox[pl * 2], whi = a.wbox[pl * 2 + 1];
+        const float4 r0 = a.Ti[pl * 3], r1 = a.Ti[pl * 3 + 1], r2 = a.Ti[pl * 3 + 2];
+        const LinkInfo& L = a.robot->link[l];
+        const float3 lc = make_float3(L.c[0], L.c[1], L.c[2]);
+        const float3 lh =
+            make_float3(0.5f * (L.hi[0] - L.lo[0]), 0.5f * (L.hi[1] - L.lo[1]), 0.5f * (L.hi[2] - L.lo[2]));
+        const float pad = a.broad_pad;
+        const uint32_t key = (uint32_t) pl;
+        const FEntry fe = a.front[f];
+        bool go = a.sc.tri_has_root && (L.n_tri > 0 || L.n_edge > 0) &&
+                  broad_test(fe.lo[0], fe.lo[1], fe.lo[2], fe.hi[0], fe.hi[1], fe.hi[2], wlo, whi, r0, r1, r2, lc, lh,
+                             pad);
+        if (go && fe.n > 0) {
+            const uint32_t o = wave_alloc(my_wcnt, seg_count, 1u);
+            if (o < a.seg_cap) seg_cand[o] = make_uint2(key, pack_leaf(fe.c, fe.n));
+            else atomicOr(a.overflow, 1u);
+            ntris += fe.n;
+            go = false;
+        }
+        if (go) {
+            int sp = 0, ni = fe.c;
+            for (;;) {
+                const BNode nd = a.sc.tri_nodes[ni];
+                bool h0 = nd.c0 >= 0 && broad_test(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, wlo, whi, r0,
+                                                   r1, r2, lc, lh, pad);
+                bool h1 = nd.c1 >= 0 && broad_test(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, wlo, whi, r0,
+                                                   r1, r2, lc, lh, pad);
+                const bool e0 = h0 && nd.n0 > 0, e1 = h1 && nd.n1 > 0;
+                if (e0 || e1) {
+                    uint32_t o = wave_alloc(my_wcnt, seg_count, (uint32_t) e0 + (uint32_t) e1);
+                    if (e0) {
+                        if (o < a.seg_cap) seg_cand[o] = make_uint2(key, pack_leaf(nd.c0, nd.n0));
                         else atomicOr(a.overflow, 1u);
+                        ++o;
+                        ntris += nd.n0;
+                        h0 = false;
                     }
-                count += nd.n0;
-                h0 = false;
-            }
-            if (h1 && nd.n1 > 0) {
-                if (EMIT)
-                    for (int k = 0; k < nd.n1; ++k) {
-                        uint32_t o = base + count + k;
-                        if (o < a.cap) a.cand[o] = make_uint2(key, (uint32_t) (nd.c1 + k));
+                    if (e1) {
+                        if (o < a.seg_cap) seg_cand[o] = make_uint2(key, pack_leaf(nd.c1, nd.n1));
                         else atomicOr(a.overflow, 1u);
+                        ntris += nd.n1;
+                        h1 = false;
                     }
-                count += nd.n1;
-                h1 = false;
-            }
-            if (h0 && h1) {
-                if (sp >= STACK) { atomicOr(a.overflow, 2u); break; }
-                st[sp * kBS] = nd.c1;
-                ++sp;
-                ni = nd.c0;
-            } else if (h0) {
-                ni = nd.c0;
-            } else if (h1) {
-                ni = nd.c1;
-            } else {
-                if (sp == 0) break;
-                --sp;
-                ni = st[sp * kBS];
+                }
+                if (h0 && h1) {
+                    if (sp >= STACK) { atomicOr(a.overflow, 2u); break; }
+                    st[sp * kBS] = nd.c1;
+                    ++sp;
+                    ni = nd.c0;
+                } else if (h0) {
+                    ni = nd.c0;
+                } else if (h1) {
+                    ni = nd.c1;
+                } else {
+                    if (sp == 0) break;
+                    --sp;
+                    ni = st[sp * kBS];
+                }
             }
         }
     }
-    if (!EMIT) a.counts[task] = count;
+    if (ntris) atomicAdd(&tris_block, ntris);
+    __syncthreads();
+    if (threadIdx.x == 0 && tris_block) atomicAdd(a.tri_candidates, (unsigned long long) tris_block);
 }
 
 // ------------------------------------------------------------------------------------------ narrow
 struct Ctr {
-    uint32_t nodes = 0, prims = 0, fp64 = 0, edges = 0;
+    uint32_t nodes = 0, prims = 0, fp64 = 0, edges = 0, tris = 0;
 };
 
-// Obstacle edge (link frame P -> Q, world endpoints wa -> wb) against the link-triangle LBVH.
-// P: link-frame origin (abs. error <= delta per component), d: R^T (Q - P) (rel. error <= 6 eps).
+// A test whose FP32 signs fall inside their band is queued for the FP64 kernel: (pose*NL + link,
+// scene triangle, robot primitive, kind) with kind 0..2 = obstacle edge j of the triangle (O2R),
+// 3 = robot edge (R2O).
+__device__ __forceinline__ void defer(const DiscreteArgs& a, uint32_t key, uint32_t tri, uint32_t prim, uint32_t kind) {
+    const uint32_t slot = atomicAdd(a.unsure_count, 1u);
+    if (slot < a.unsure_cap) a.unsure[slot] = make_uint4(key, tri, prim, kind);
+    else atomicOr(a.overflow, 4u);
+}
+
+// Obstacle edge j of scene triangle `tri` (link-frame origin P with abs. error <= delta per
+// component, direction d = R^T (Q - P) with rel. error <= 6 eps) against the link-triangle LBVH.
 template <int STACK, bool COUNT>
-__device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, const double* T64, float3 P, float3 d,
-                        float delta, float4 wa, float4 wb, int* st, const uint32_t* flag, Ctr& ctr) {
+__device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, uint32_t j, float3 P,
+                        float3 d, float delta, int* st, const uint32_t* flag, Ctr& ctr) {
     if (!L.tri_has_root) return 0;
     const float3 inv = safe_inv(d);
     const float D = l1(d.x, d.y, d.z) * (1.f + 8.f * kEps32);
@@ -182,8 +236,6 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, const double* 
     const MTBands bd = make_bands(D, ad, S, as, L.e1max, kEps32 * L.e1max, L.e2max, kEps32 * L.e2max);
     const BNode* nodes = a.rb.tri_nodes + L.tri_node0;
     const Tri* prims = a.rb.tri_prims + L.tri_prim0;
-    bool have64 = false;
-    double P64[3], Q64[3];
     int sp = 0, ni = 0, iter = 0;
     for (;;) {
         if ((++iter & 15) == 0 && load_flag(flag)) return 0;
@@ -201,19 +253,11 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, const double* 
                     const Tri t = prims[start + k];
                     if (COUNT) ctr.prims++;
                     const float3 ta = f3(t.a);
-                    int r = mt32(P, d, ta, sub(f3(t.b), ta), sub(f3(t.c), ta), bd);
+                    const int r = mt32(P, d, ta, sub(f3(t.b), ta), sub(f3(t.c), ta), bd);
                     if (r == kHit) return 1;
                     if (r == kUnsure) {
                         if (COUNT) ctr.fp64++;
-                        if (!have64) {
-                            double A[3] = {wa.x, wa.y, wa.z}, B[3] = {wb.x, wb.y, wb.z};
-                            x64::inv_xform(T64, A, P64);
-                            x64::inv_xform(T64, B, Q64);
-                            have64 = true;
-                        }
-                        double ta[3] = {t.a.x, t.a.y, t.a.z}, tb[3] = {t.b.x, t.b.y, t.b.z},
-                               tc[3] = {t.c.x, t.c.y, t.c.z};
-                        if (x64::seg_tri(P64, Q64, ta, tb, tc)) return 1;
+                        defer(a, key, tri, (uint32_t) (L.tri_prim0 + start + k), j);
                     }
                 }
                 if (side) h1 = false; else h0 = false;
@@ -236,11 +280,11 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, const double* 
     }
 }
 
-// Scene triangle (link frame A', B', C'; world wa, wb, wc) against the link-edge LBVH.
-// A: link-frame vertex (abs. error <= delta), e1 = R^T (B - A), e2 = R^T (C - A) (rel. error <= 6 eps).
+// Scene triangle (link frame A with abs. error <= delta, e1 = R^T (B - A), e2 = R^T (C - A) with
+// rel. error <= 6 eps) against the link-edge LBVH.
 template <int STACK, bool COUNT>
-__device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, const double* T64, float3 A, float3 e1, float3 e2,
-                       float delta, float4 wa, float4 wb, float4 wc, int* st, const uint32_t* flag, Ctr& ctr) {
+__device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, float3 A, float3 e1,
+                       float3 e2, float delta, int* st, const uint32_t* flag, Ctr& ctr) {
     if (!L.edge_has_root) return 0;
     const float3 B = add(A, e1), C = add(A, e2);
     const float pad = 2.f * delta + 12.f * kEps32 * (l1(e1.x, e1.y, e1.z) + l1(e2.x, e2.y, e2.z)) + 1e-6f * L.scale;
@@ -256,15 +300,32 @@ __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, const double* T
                                   6.1f * kEps32 * E2);
     const BNode* nodes = a.rb.edge_nodes + L.edge_node0;
     const Edge* prims = a.rb.edge_prims + L.edge_prim0;
+    // plane of the triangle: a node box that lies strictly on one side cannot hold a crossing edge.
+    // n = e1 x e2 and d0 = n.A carry errors bounded by 14 eps E1 E2 (n) and |n| delta (A); the
+    // margin below covers them for any box centre c: |Delta(n.(c - A))| <= 14 eps E1 E2 |c - A|_1 + |n|_1 delta.
+    const float3 nrm = make_float3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+    const float nl = l1(nrm.x, nrm.y, nrm.z);
+    const float ne = 32.f * kEps32 * E1 * E2;
+    auto straddles = [&](float bx0, float by0, float bz0, float bx1, float by1, float bz1) {
+        const float cx = 0.5f * (bx0 + bx1) - A.x, cy = 0.5f * (by0 + by1) - A.y, cz = 0.5f * (bz0 + bz1) - A.z;
+        const float hx = 0.5f * (bx1 - bx0), hy = 0.5f * (by1 - by0), hz = 0.5f * (bz1 - bz0);
+        const float sd = nrm.x * cx + nrm.y * cy + nrm.z * cz;
+        const float rr = fabsf(nrm.x) * hx + fabsf(nrm.y) * hy + fabsf(nrm.z) * hz;
+        const float m = ne * (fabsf(cx) + fabsf(cy) + fabsf(cz) + hx + hy + hz) + (nl + ne) * 2.f * delta +
+                        4.f * kEps32 * (fabsf(sd) + rr);
+        return fabsf(sd) <= rr + m;
+    };
     int sp = 0, ni = 0, iter = 0;
     for (;;) {
         if ((++iter & 15) == 0 && load_flag(flag)) return 0;
         const BNode nd = nodes[ni];
         if (COUNT) ctr.nodes++;
         bool h0 = nd.c0 >= 0 && !(lx > nd.hi0x || hx < nd.lo0x || ly > nd.hi0y || hy < nd.lo0y || lz > nd.hi0z ||
-                                  hz < nd.lo0z);
+                                  hz < nd.lo0z) &&
+                  straddles(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
         bool h1 = nd.c1 >= 0 && !(lx > nd.hi1x || hx < nd.lo1x || ly > nd.hi1y || hy < nd.lo1y || lz > nd.hi1z ||
-                                  hz < nd.lo1z);
+                                  hz < nd.lo1z) &&
+                  straddles(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
             const bool hit = side ? h1 : h0;
@@ -275,15 +336,11 @@ __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, const double* T
                     const Edge e = prims[start + k];
                     if (COUNT) { ctr.prims++; ctr.edges++; }
                     const float3 ep = f3(e.p);
-                    int r = mt32(ep, sub(f3(e.q), ep), A, e1, e2, bd);
+                    const int r = mt32(ep, sub(f3(e.q), ep), A, e1, e2, bd);
                     if (r == kHit) return 1;
                     if (r == kUnsure) {
                         if (COUNT) ctr.fp64++;
-                        double p0[3] = {e.p.x, e.p.y, e.p.z}, q0[3] = {e.q.x, e.q.y, e.q.z}, P64[3], Q64[3];
-                        x64::xform(T64, p0, P64);
-                        x64::xform(T64, q0, Q64);
-                        double ta[3] = {wa.x, wa.y, wa.z}, tb[3] = {wb.x, wb.y, wb.z}, tc[3] = {wc.x, wc.y, wc.z};
-                        if (x64::seg_tri(P64, Q64, ta, tb, tc)) return 1;
+                        defer(a, key, tri, (uint32_t) (L.edge_prim0 + start + k), 3u);
                     }
                 }
                 if (side) h1 = false; else h0 = false;
@@ -306,44 +363,87 @@ __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, const double* T
     }
 }
 
-template <int STACK, bool COUNT>
+// One direction over all candidates (grid-stride).  DIR 1: obstacle->robot, 2: robot->obstacle.
+template <int DIR, int STACK, bool COUNT>
 __global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
     __shared__ int stk[STACK * kBS];
+    __shared__ uint32_t seg_pre[kSeg + 1];
     int* st = stk + threadIdx.x;
-    const uint32_t total = min(*a.total, a.cap);
+    // prefix over the candidate segments (clamped to their capacity): each thread scans kSeg / kBS
+    // consecutive counts, then the per-thread sums are scanned by thread 0
+    {
+        constexpr int per = kSeg / kBS;
+        __shared__ uint32_t part[kBS];
+        uint32_t loc[per], acc = 0;
+        for (int k = 0; k < per; ++k) {
+            loc[k] = min(a.seg_count[threadIdx.x * per + k], a.seg_cap);
+            acc += loc[k];
+        }
+        part[threadIdx.x] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < kBS; ++k) {
+                const uint32_t x = part[k];
+                part[k] = run;
+                run += x;
+            }
+            seg_pre[kSeg] = run;
+        }
+        __syncthreads();
+        uint32_t run = part[threadIdx.x];
+        for (int k = 0; k < per; ++k) {
+            seg_pre[threadIdx.x * per + k] = run;
+            run += loc[k];
+        }
+        __syncthreads();
+    }
+    // each leaf record is spread over `lanes` adjacent threads, one triangle per thread
+    const uint32_t lanes = (uint32_t) a.leaf_lanes;
+    const uint64_t total = (uint64_t) seg_pre[kSeg] * lanes;
     const uint32_t stride = gridDim.x * kBS;
     Ctr ctr;
-    for (uint32_t i = blockIdx.x * kBS + threadIdx.x; i < total; i += stride) {
-        const uint2 c = a.cand[i];
-        const int p = (int) (c.x / (uint32_t) a.n_links), l = (int) (c.x - (uint32_t) p * a.n_links);
+    for (uint64_t v = blockIdx.x * kBS + threadIdx.x; v < total; v += stride) {
+        const uint32_t i = (uint32_t) (v / lanes), sub_k = (uint32_t) (v - (uint64_t) i * lanes);
+        int lo = 0, hi = kSeg;  // largest segment with seg_pre[lo] <= i
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (seg_pre[mid] <= i) lo = mid; else hi = mid;
+        }
+        const uint2 c = a.cand[(size_t) lo * a.seg_cap + (i - seg_pre[lo])];
+        if (sub_k > (c.y >> 27)) continue;
+        const uint32_t p = c.x / (uint32_t) a.n_links, l = c.x - p * (uint32_t) a.n_links;
         uint32_t* flag = a.flags + p;
         if (load_flag(flag)) continue;
         const LinkInfo& L = a.robot->link[l];
         const float4 r0 = a.Ti[(size_t) c.x * 3], r1 = a.Ti[(size_t) c.x * 3 + 1], r2 = a.Ti[(size_t) c.x * 3 + 2];
-        const double* T64 = a.T64 + (size_t) c.x * 12;
-        const Tri t = a.sc.tri_prims[c.y];
-        // vertex A moved as R^T (A - t); edges rotated from world differences, so that their error is
-        // relative to their own length:  |A' - A'_exact| <= eps |t| + 5 eps |A - t|_1 per component
-        const float3 wA = f3(t.a), wB = f3(t.b), wC = f3(t.c);
-        const float3 rel = make_float3(wA.x - r0.w, wA.y - r1.w, wA.z - r2.w);
-        const float3 A = rot(r0, r1, r2, rel);
-        const float3 e1 = rot(r0, r1, r2, sub(wB, wA)), e2 = rot(r0, r1, r2, sub(wC, wA));
         const float tinf = fmaxf(fmaxf(fabsf(r0.w), fabsf(r1.w)), fabsf(r2.w));
-        const float en = fmaxf(l1(e1.x, e1.y, e1.z), l1(e2.x, e2.y, e2.z));
-        // B' = A' + e1', C' = A' + e2' add their own rounding: 8 eps |e|
-        const float delta = 1.05f * kEps32 * (tinf + 5.f * l1(rel.x, rel.y, rel.z) + 8.f * en) + 1e-30f;
         int hit = 0;
-        if (a.dir & 1) {
-            const uint32_t owned = __float_as_uint(t.b.w);
-            if (owned & 1u)
-                hit = o2r_edge<STACK, COUNT>(a, L, T64, A, e1, delta, t.a, t.b, st, flag, ctr);
-            if (!hit && (owned & 2u))
-                hit = o2r_edge<STACK, COUNT>(a, L, T64, add(A, e1), rot(r0, r1, r2, sub(wC, wB)), delta, t.b, t.c,
-                                             st, flag, ctr);
-            if (!hit && (owned & 4u))
-                hit = o2r_edge<STACK, COUNT>(a, L, T64, add(A, e2), neg(e2), delta, t.c, t.a, st, flag, ctr);
+        {
+            const uint32_t tri = (c.y & 0x7ffffffu) + sub_k;
+            const Tri t = a.sc.tri_prims[tri];
+            // vertex A moved as R^T (A - t); edges rotated from world differences, so that their
+            // error is relative to their own length: |A' - A'_exact| <= eps |t| + 5 eps |A - t|_1
+            const float3 wA = f3(t.a), wB = f3(t.b), wC = f3(t.c);
+            const float3 rel = make_float3(wA.x - r0.w, wA.y - r1.w, wA.z - r2.w);
+            const float3 A = rot(r0, r1, r2, rel);
+            const float3 e1 = rot(r0, r1, r2, sub(wB, wA)), e2 = rot(r0, r1, r2, sub(wC, wA));
+            const float en = fmaxf(l1(e1.x, e1.y, e1.z), l1(e2.x, e2.y, e2.z));
+            // B' = A' + e1', C' = A' + e2' add their own rounding: 8 eps |e|
+            const float delta = 1.05f * kEps32 * (tinf + 5.f * l1(rel.x, rel.y, rel.z) + 8.f * en) + 1e-30f;
+            if (COUNT) ctr.tris++;
+            if (DIR == 1) {
+                const uint32_t owned = __float_as_uint(t.b.w);
+                if (owned & 1u) hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 0u, A, e1, delta, st, flag, ctr);
+                if (!hit && (owned & 2u))
+                    hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 1u, add(A, e1), rot(r0, r1, r2, sub(wC, wB)), delta, st,
+                                                 flag, ctr);
+                if (!hit && (owned & 4u))
+                    hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 2u, add(A, e2), neg(e2), delta, st, flag, ctr);
+            } else {
+                hit = r2o_tri<STACK, COUNT>(a, L, c.x, tri, A, e1, e2, delta, st, flag, ctr);
+            }
         }
-        if (!hit && (a.dir & 2)) hit = r2o_tri<STACK, COUNT>(a, L, T64, A, e1, e2, delta, t.a, t.b, t.c, st, flag, ctr);
         if (hit) atomicOr(flag, 1u);
     }
     if (COUNT) {
@@ -351,6 +451,40 @@ __global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
         atomicAdd(&a.stats->prim_tests, (unsigned long long) ctr.prims);
         atomicAdd(&a.stats->fp64, (unsigned long long) ctr.fp64);
         atomicAdd(&a.stats->edge_tests, (unsigned long long) ctr.edges);
+    }
+}
+
+// FP64 re-decision of the queued tests, exactly as the oracle computes them.
+__global__ __launch_bounds__(kBS) void k_redecide(DiscreteArgs a) {
+    const uint32_t total = min(*a.unsure_count, a.unsure_cap);
+    for (uint32_t i = blockIdx.x * kBS + threadIdx.x; i < total; i += gridDim.x * kBS) {
+        const uint4 u = a.unsure[i];
+        const uint32_t p = u.x / (uint32_t) a.n_links;
+        uint32_t* flag = a.flags + p;
+        if (load_flag(flag)) continue;
+        const double* T64 = a.T64 + (size_t) u.x * 12;
+        const Tri t = a.sc.tri_prims[u.y];
+        const float4 w[3] = {t.a, t.b, t.c};
+        double P[3], Q[3], A[3], B[3], C[3];
+        if (u.w < 3u) {
+            const float4 wa = w[u.w], wb = w[(u.w + 1) % 3];
+            double pa[3] = {wa.x, wa.y, wa.z}, pb[3] = {wb.x, wb.y, wb.z};
+            x64::inv_xform(T64, pa, P);
+            x64::inv_xform(T64, pb, Q);
+            const Tri lt = a.rb.tri_prims[u.z];
+            A[0] = lt.a.x; A[1] = lt.a.y; A[2] = lt.a.z;
+            B[0] = lt.b.x; B[1] = lt.b.y; B[2] = lt.b.z;
+            C[0] = lt.c.x; C[1] = lt.c.y; C[2] = lt.c.z;
+        } else {
+            const Edge e = a.rb.edge_prims[u.z];
+            double p0[3] = {e.p.x, e.p.y, e.p.z}, q0[3] = {e.q.x, e.q.y, e.q.z};
+            x64::xform(T64, p0, P);
+            x64::xform(T64, q0, Q);
+            A[0] = t.a.x; A[1] = t.a.y; A[2] = t.a.z;
+            B[0] = t.b.x; B[1] = t.b.y; B[2] = t.b.z;
+            C[0] = t.c.x; C[1] = t.c.y; C[2] = t.c.z;
+        }
+        if (x64::seg_tri(P, Q, A, B, C)) atomicOr(flag, 1u);
     }
 }
 
@@ -374,32 +508,43 @@ void launch_fk(const RobotDev* robot, int, int, const float* poses, int n, doubl
 }
 
 template <int STACK>
-static void broad_t(const DiscreteArgs& a, bool emit, hipStream_t s) {
-    unsigned nb = blocks_for((size_t) a.n_active * a.n_poses, kBS);
+static void broad_t(const DiscreteArgs& a, hipStream_t s) {
+    const unsigned nb = (unsigned) std::min<size_t>(blocks_for((size_t) a.n_active * a.n_poses * a.F, kBS), kSeg);
     if (!nb) return;
-    if (emit) hipLaunchKernelGGL((k_broad<STACK, true>), dim3(nb), dim3(kBS), 0, s, a);
-    else hipLaunchKernelGGL((k_broad<STACK, false>), dim3(nb), dim3(kBS), 0, s, a);
+    hipLaunchKernelGGL((k_broad<STACK>), dim3(nb), dim3(kBS), 0, s, a);
     RTCD_LAUNCH_CHECK();
 }
 
-void launch_broad(const DiscreteArgs& a, int stack, bool emit, hipStream_t s) {
-    if (stack <= 32) broad_t<32>(a, emit, s);
-    else if (stack <= 64) broad_t<64>(a, emit, s);
+void launch_broad(const DiscreteArgs& a, int stack, hipStream_t s) {
+    if (stack <= 32) broad_t<32>(a, s);
+    else if (stack <= 64) broad_t<64>(a, s);
     else throw std::runtime_error("BVH deeper than 64 levels");
 }
 
-template <int STACK>
+template <int DIR, int STACK>
 static void narrow_t(const DiscreteArgs& a, bool count, hipStream_t s) {
-    unsigned nb = (unsigned) device_cus() * (STACK <= 32 ? 8u : 4u);
-    if (count) hipLaunchKernelGGL((k_narrow<STACK, true>), dim3(nb), dim3(kBS), 0, s, a);
-    else hipLaunchKernelGGL((k_narrow<STACK, false>), dim3(nb), dim3(kBS), 0, s, a);
+    const unsigned nb = (unsigned) device_cus() * 8u;
+    if (count) hipLaunchKernelGGL((k_narrow<DIR, STACK, true>), dim3(nb), dim3(kBS), 0, s, a);
+    else hipLaunchKernelGGL((k_narrow<DIR, STACK, false>), dim3(nb), dim3(kBS), 0, s, a);
     RTCD_LAUNCH_CHECK();
+}
+
+template <int DIR>
+static void narrow_dir(const DiscreteArgs& a, int stack, bool count, hipStream_t s) {
+    if (stack <= 16) narrow_t<DIR, 16>(a, count, s);
+    else if (stack <= 24) narrow_t<DIR, 24>(a, count, s);
+    else if (stack <= 32) narrow_t<DIR, 32>(a, count, s);
+    else if (stack <= 64) narrow_t<DIR, 64>(a, count, s);
+    else throw std::runtime_error("BVH deeper than 64 levels");
 }
 
 void launch_narrow(const DiscreteArgs& a, int stack, bool count, hipStream_t s) {
-    if (stack <= 32) narrow_t<32>(a, count, s);
-    else if (stack <= 64) narrow_t<64>(a, count, s);
-    else throw std::runtime_error("BVH deeper than 64 levels");
+    // obstacle->robot first: it is cheaper and flags most colliding poses, which the
+    // robot->obstacle pass then skips
+    if (a.dir & 1) narrow_dir<1>(a, stack, count, s);
+    if (a.dir & 2) narrow_dir<2>(a, stack, count, s);
+    hipLaunchKernelGGL(k_redecide, dim3((unsigned) device_cus() * 2u), dim3(kBS), 0, s, a);
+    RTCD_LAUNCH_CHECK();
 }
 
 }  // namespace rtcd

@@ -4,6 +4,10 @@
 
 namespace rtcd {
 
+// The broad phase appends candidates into this many independent segments, one per broad block (at
+// most this many blocks), so that its atomics do not contend on one address.
+constexpr int kCandSegments = 2048;
+
 struct RobotBvh {
     const BNode* tri_nodes;
     const Tri* tri_prims;
@@ -33,23 +37,30 @@ struct DiscreteArgs {
     const double* T64;     // [n][NL][12]
     const float4* Ti;      // [n][NL][3]  rows of [R^T | -R^T t], fp32
     const float4* wbox;    // [n][NL][2]  world AABB of each link
-    uint32_t* counts;      // [n_active * n]
-    const uint32_t* offsets;
-    uint2* cand;           // (pose * NL + link, scene triangle)
-    uint32_t cap;
-    const uint32_t* total;
+    uint2* cand;           // (pose * NL + link, first triangle | (count - 1) << 27) per scene leaf,
+                           // in kCandSegments segments of seg_cap records
+    uint32_t seg_cap;
+    uint32_t* seg_count;   // [kCandSegments] records appended per segment (may exceed seg_cap: overflow bit 1)
+    unsigned long long* tri_candidates;  // triangles in the emitted leaves
     uint32_t* overflow;
     uint32_t* flags;       // [n] output
     Stats* stats;
     float broad_pad;
+    int leaf_lanes;        // threads per leaf record in the narrow phase (= scene BVH leaf size)
+    const FEntry* front;   // scene-BVH frontier: each (pose, link) task is split over its F entries
+    int F;
+    uint4* unsure;         // FP64 re-decision queue
+    uint32_t unsure_cap;
+    uint32_t* unsure_count;
 };
 
 // FK for n poses: T64, Ti, wbox for every link (exact FP64 sequence shared with the oracle).
 void launch_fk(const RobotDev* robot, int dof, int n_links, const float* poses, int n, double* T64, float4* Ti,
                float4* wbox, hipStream_t s);
-// Broad phase: emit == false writes counts[task], emit == true writes candidates at offsets[task].
-void launch_broad(const DiscreteArgs& a, int stack, bool emit, hipStream_t s);
-// Narrow phase over *total candidates (grid-stride, sized for the device).
+// Broad phase: one pass appending leaf candidates (wave-aggregated atomics).
+void launch_broad(const DiscreteArgs& a, int stack, hipStream_t s);
+// Narrow phase over *total candidates (grid-stride, sized for the device): one pass per direction
+// in a.dir, then the FP64 re-decision of the queued tests.  `stack` covers the robot BVH depth.
 void launch_narrow(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
 
 // ---- swept volumes ----

@@ -142,26 +142,50 @@ __device__ int cap_cert(float3 C0, float3 C1, float3 A, float3 B, float r_hit, f
     return kUnsure;
 }
 
-// Quadratic Bezier (q0, q1, q2) vs segment: split into 4 sub-pieces (de Casteljau); each sub-piece
-// lies within h = |q1' - (q0' + q2')/2| / 2 of its chord.
+// Quadratic Bezier (q0, q1, q2) vs segment, adaptive: the dyadic sub-pieces [k/2^d, (k+1)/2^d] are
+// visited depth first without a stack; a sub-piece with controls (a, m, b) lies within
+// h = |m - (a + b)/2| / 2 of its chord, so the chord capsule with radii r - h (hit) and r + h (miss)
+// decides it; undecided sub-pieces are refined down to depth kQuadDepth.  Sub-piece controls come
+// from the blossom B(u, v) = (1-u)(1-v) q0 + ((1-u) v + u (1-v)) q1 + u v q2: a = B(t0, t0),
+// m = B(t0, t1), b = B(t1, t1).
+constexpr int kQuadDepth = 4;
+constexpr int kSub = 4;  // work items per quadratic piece
+
+__device__ __forceinline__ float3 blossom(float3 q0, float3 q1, float3 q2, float u, float v) {
+    const float w0 = (1.f - u) * (1.f - v), w1 = (1.f - u) * v + u * (1.f - v), w2 = u * v;
+    return make_float3(w0 * q0.x + w1 * q1.x + w2 * q2.x, w0 * q0.y + w1 * q1.y + w2 * q2.y,
+                       w0 * q0.z + w1 * q1.z + w2 * q2.z);
+}
+
 __device__ int quad_cert(float3 q0, float3 q1, float3 q2, float3 A, float3 B, float r, float ein) {
-    float3 pts[9];
-    {
-        const float3 m01 = 0.5f * (q0 + q1), m12 = 0.5f * (q1 + q2), mid = 0.5f * (m01 + m12);
-        const float3 l1 = 0.5f * (q0 + m01), l2 = 0.5f * (m01 + mid), lm = 0.5f * (l1 + l2);
-        const float3 r1 = 0.5f * (mid + m12), r2 = 0.5f * (m12 + q2), rm = 0.5f * (r1 + r2);
-        pts[0] = q0; pts[1] = l1; pts[2] = lm; pts[3] = l2; pts[4] = mid; pts[5] = r1; pts[6] = rm; pts[7] = r2; pts[8] = q2;
-    }
     const float M = fmaxf(fmaxf(linf(q0), linf(q1)), linf(q2));
-    const float e2 = ein + 8.f * kEps32 * M;  // error of the sub-piece control points
+    // blossom weights are exact at dyadic parameters; the weighted sum adds 6 eps M per component
+    const float e2 = ein + 8.f * kEps32 * M;
     bool all_miss = true;
-    for (int k = 0; k < 4; ++k) {
-        const float3 a = pts[2 * k], m = pts[2 * k + 1], b = pts[2 * k + 2];
-        const float3 dev = m - 0.5f * (a + b);
+    int d = 0;
+    uint32_t k = 0;
+    for (;;) {
+        const float t0 = (float) k / (float) (1u << d), t1 = (float) (k + 1) / (float) (1u << d);
+        const float3 pa = blossom(q0, q1, q2, t0, t0), pm = blossom(q0, q1, q2, t0, t1), pb = blossom(q0, q1, q2, t1, t1);
+        const float3 dev = pm - 0.5f * (pa + pb);
         const float h = 0.5f * sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 2.f * 1.7321f * e2;
-        const int cls = cap_cert(a, b, A, B, r - h, r + h, e2);
+        const int cls = cap_cert(pa, pb, A, B, r - h, r + h, e2);
         if (cls == kHit) return kHit;
-        if (cls != kMiss) all_miss = false;
+        if (cls != kMiss) {
+            if (d < kQuadDepth) {  // refine: first child
+                ++d;
+                k <<= 1;
+                continue;
+            }
+            all_miss = false;
+        }
+        // next sibling, climbing while this was a right child
+        while (d > 0 && (k & 1u)) {
+            --d;
+            k >>= 1;
+        }
+        if (d == 0) break;
+        ++k;
     }
     return all_miss ? kMiss : kUnsure;
 }
@@ -173,12 +197,16 @@ __global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, const float4* __rest
     int* st = stk + threadIdx.x;
     const int S = a.robot->n_spheres;
     const int npc = a.n_ctrl - 2;
-    const size_t n_items = (size_t) a.n_seg * S;
+    // quadratic pieces are split into kSub work items, one per sub-piece [k/kSub, (k+1)/kSub]
+    constexpr int SUB = DEG == 1 ? 1 : kSub;
+    const size_t n_items = (size_t) a.n_seg * SUB * S;
     const size_t stride = (size_t) gridDim.x * kBS;
     uint32_t c_nodes = 0, c_prims = 0, c_fp64 = 0, c_items = 0;
     for (size_t it = (size_t) blockIdx.x * kBS + threadIdx.x; it < n_items; it += stride) {
-        const uint32_t g = (uint32_t) (it / S);
-        const int s = (int) (it - (size_t) g * S);
+        const uint32_t gk = (uint32_t) (it / S);
+        const int s = (int) (it - (size_t) gk * S);
+        const uint32_t g = gk / (uint32_t) SUB;
+        const int sub_k = (int) (gk - g * (uint32_t) SUB);
         uint32_t traj;
         float3 q[3];
         float r;
@@ -202,8 +230,19 @@ __global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, const float4* __rest
         uint32_t* flag = a.out_seg ? a.out_seg + g : a.out + traj;
         if (load_flag(flag)) continue;
         if (COUNT) c_items++;
-        const float M = fmaxf(fmaxf(linf(q[0]), linf(q[1])), DEG == 2 ? linf(q[2]) : 0.f);
-        const float ein = (DEG == 1 ? 1.01f : 3.f) * kEps32 * M;
+        float M = fmaxf(fmaxf(linf(q[0]), linf(q[1])), DEG == 2 ? linf(q[2]) : 0.f);
+        float ein = (DEG == 1 ? 1.01f : 3.f) * kEps32 * M;
+        if (DEG == 2) {
+            // this item's sub-piece, controls from the blossom (exact weights at dyadic parameters)
+            const float t0 = (float) sub_k / (float) SUB, t1 = (float) (sub_k + 1) / (float) SUB;
+            const float3 sa = blossom(q[0], q[1], q[2], t0, t0), sm = blossom(q[0], q[1], q[2], t0, t1),
+                         sb = blossom(q[0], q[1], q[2], t1, t1);
+            q[0] = sa;
+            q[1] = sm;
+            q[2] = sb;
+            ein += 8.f * kEps32 * M;
+            M = fmaxf(fmaxf(linf(q[0]), linf(q[1])), linf(q[2]));
+        }
         const float pad = r + 4.f * ein + 1e-6f * (1.f + a.sc.scale);
         float lx = fminf(q[0].x, q[1].x), ly = fminf(q[0].y, q[1].y), lz = fminf(q[0].z, q[1].z);
         float hx = fmaxf(q[0].x, q[1].x), hy = fmaxf(q[0].y, q[1].y), hz = fmaxf(q[0].z, q[1].z);
@@ -212,17 +251,28 @@ __global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, const float4* __rest
             hx = fmaxf(hx, q[2].x); hy = fmaxf(hy, q[2].y); hz = fmaxf(hz, q[2].z);
         }
         lx -= pad; ly -= pad; lz -= pad; hx += pad; hy += pad; hz += pad;
+        // Tighter node test: the tube lies within r (+ h for a curved piece) of the chord, so a node box
+        // can hold a candidate edge only if the chord meets the box grown by that margin.
+        const float3 cp = q[0];
+        const float3 cinv = safe_inv((DEG == 1 ? q[1] : q[2]) - q[0]);
+        float cpad = pad;
+        if (DEG == 2) {
+            const float3 dev = q[1] - 0.5f * (q[0] + q[2]);
+            cpad += 0.5f * sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 4.f * 1.7321f * (ein + 8.f * kEps32 * M);
+        }
+        auto node_ok = [&](float bx0, float by0, float bz0, float bx1, float by1, float bz1) {
+            if (lx > bx1 || hx < bx0 || ly > by1 || hy < by0 || lz > bz1 || hz < bz0) return false;
+            return slab(cp, cinv, bx0, by0, bz0, bx1, by1, bz1, cpad);
+        };
         int hit = 0;
         if (a.sc.edge_has_root) {
             int sp = 0, ni = 0, iter = 0;
             for (;;) {
-                if ((++iter & 15) == 0 && load_flag(flag)) break;
+                if ((++iter & 7) == 0 && load_flag(flag)) break;
                 const BNode nd = a.sc.edge_nodes[ni];
                 if (COUNT) c_nodes++;
-                bool h0 = nd.c0 >= 0 && !(lx > nd.hi0x || hx < nd.lo0x || ly > nd.hi0y || hy < nd.lo0y ||
-                                          lz > nd.hi0z || hz < nd.lo0z);
-                bool h1 = nd.c1 >= 0 && !(lx > nd.hi1x || hx < nd.lo1x || ly > nd.hi1y || hy < nd.lo1y ||
-                                          lz > nd.hi1z || hz < nd.lo1z);
+                bool h0 = nd.c0 >= 0 && node_ok(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
+                bool h1 = nd.c1 >= 0 && node_ok(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
 #pragma unroll
                 for (int side = 0; side < 2 && !hit; ++side) {
                     const bool hb = side ? h1 : h0;
@@ -305,7 +355,7 @@ int device_cus() {
 
 template <int DEG, int STACK>
 void sweep_t(const SweepArgs& a, const float4* c32, const float4* P32, bool count, hipStream_t s) {
-    const size_t items = (size_t) a.n_seg * a.robot_n_spheres_host;
+    const size_t items = (size_t) a.n_seg * a.robot_n_spheres_host * (DEG == 1 ? 1 : kSub);
     unsigned nb = (unsigned) std::min<size_t>(blocks_for(items, kBS), (size_t) device_cus() * 8);
     if (!nb) return;
     if (count) hipLaunchKernelGGL((k_sweep<DEG, STACK, true>), dim3(nb), dim3(kBS), 0, s, a, c32, P32);

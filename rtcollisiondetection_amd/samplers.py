@@ -40,22 +40,34 @@ def random_walk_trajectories(n, lower, upper, lengths=32, max_step=0.1, seed=4):
     rng = np.random.default_rng(seed)
     lower = np.asarray(lower, dtype=np.float64)
     upper = np.asarray(upper, dtype=np.float64)
+    dof = len(lower)
     if isinstance(lengths, (tuple, list)):
         L = rng.integers(lengths[0], lengths[1] + 1, size=n)
     else:
         L = np.full(n, int(lengths))
     offs = np.zeros(n + 1, dtype=np.uint32)
     offs[1:] = np.cumsum(L)
-    out = np.empty((int(offs[-1]), len(lower)))
-    for i in range(n):
-        q = lower + (upper - lower) * rng.random(len(lower))
-        steps = rng.uniform(-max_step, max_step, size=(L[i] - 1, len(lower)))
-        traj = [q]
-        for s in steps:
-            q = np.clip(q + s, lower, upper)
-            traj.append(q)
-        out[offs[i]:offs[i + 1]] = traj
+    Lmax = int(L.max()) if n else 0
+    q = lower + (upper - lower) * rng.random((n, dof))
+    steps = rng.uniform(-max_step, max_step, size=(max(Lmax - 1, 0), n, dof))
+    walk = np.empty((Lmax, n, dof))
+    if Lmax:
+        walk[0] = q
+    for k in range(1, Lmax):
+        q = np.clip(q + steps[k - 1], lower, upper)
+        walk[k] = q
+    # gather the first L_i waypoints of walk i, trajectories concatenated
+    idx_t = np.repeat(np.arange(n), L)
+    idx_k = np.arange(int(offs[-1])) - np.repeat(offs[:-1].astype(np.int64), L)
+    out = walk[idx_k, idx_t]
     return np.ascontiguousarray(out.astype(np.float32)), offs
+
+
+def clamped_quadratic_basis(n_ctrl, n_samples):
+    """Basis matrix [n_samples, n_ctrl] of the clamped uniform quadratic B-spline (knots
+    0,0,0,1,..,m,m,m with m = n_ctrl - 2) at n_samples parameters uniform on [0, m] (de Boor)."""
+    eye = np.eye(n_ctrl)
+    return clamped_quadratic_bspline(eye, n_samples)
 
 
 def clamped_quadratic_bspline(ctrl, n_samples):
@@ -86,7 +98,8 @@ def bspline_trajectories(n, lower, upper, n_ctrl=8, n_wp=16, seed=3):
     lower = np.asarray(lower, dtype=np.float64)
     upper = np.asarray(upper, dtype=np.float64)
     ctrl = lower + (upper - lower) * rng.random((n, n_ctrl, len(lower)))
-    wp = np.stack([clamped_quadratic_bspline(c, n_wp) for c in ctrl])
+    B = clamped_quadratic_basis(n_ctrl, n_wp)
+    wp = np.einsum('wc,ncd->nwd', B, ctrl)
     return np.ascontiguousarray(wp.astype(np.float32)), np.ascontiguousarray(ctrl.astype(np.float32))
 
 

@@ -1,0 +1,61 @@
+"""Serving loop: concurrent requests on several slots give the same flags as each request alone."""
+import numpy as np
+import pytest
+
+from rtcollisiondetection_amd import _lib, scenes
+from rtcollisiondetection_amd.serving import Server, make_stream, run_alone, run_stream
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope='module')
+def env(panda):
+    from rtcollisiondetection_amd.detector import Context
+    ctx = Context()
+    ctx.load_robot(panda)
+    s8 = scenes.scene_boxes8()
+    sh = scenes.scene_shelf()
+    ids = [ctx.create_scene(s8.verts, s8.tris), ctx.create_scene(sh.verts, sh.tris)]
+    yield ctx, ids
+    ctx.close()
+
+
+def test_mixed_stream_matches_alone(env, panda):
+    ctx, ids = env
+    reqs = make_stream(panda, ids, rate_hz=400.0, duration_s=0.25, seed=5, max_size=512)
+    assert len(reqs) > 20
+    kinds = {r.kind for r in reqs}
+    assert kinds == {1, 2, 3}
+    srv = Server(ctx, slots=4)
+    try:
+        res, wall = run_stream(srv, reqs, realtime=False)
+        st = srv.stats()
+    finally:
+        srv.stop()
+    assert st['completed'] == len(reqs) and st['failed'] == 0
+    for r in reqs:
+        alone = run_alone(ctx, r)
+        assert np.array_equal(r.out, alone), (r.kind, r.n)
+    print(f"served {len(reqs)} requests, p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms")
+
+
+def test_admission_rejects_oversized_and_bad_requests(env, panda):
+    ctx, ids = env
+    from rtcollisiondetection_amd.serving import Request
+    from rtcollisiondetection_amd import samplers
+    srv = Server(ctx, slots=2, mem_budget=64 << 20)
+    try:
+        big = Request(_lib.REQ_DISCRETE, ids[0], 65536, samplers.uniform_poses(65536, panda.lower, panda.upper, 1))
+        with pytest.raises(_lib.RtcdError) as e:
+            srv.submit(big)
+        assert e.value.status == 4
+        bad = Request(_lib.REQ_DISCRETE, 9999, 4, samplers.uniform_poses(4, panda.lower, panda.upper, 1))
+        with pytest.raises(_lib.RtcdError) as e:
+            srv.submit(bad)
+        assert e.value.status == 6
+        ok = Request(_lib.REQ_DISCRETE, ids[0], 64, samplers.uniform_poses(64, panda.lower, panda.upper, 2))
+        rid = srv.submit(ok)
+        out, lat = srv.wait(rid)
+        assert np.array_equal(out, ctx.check_discrete(ids[0], ok.data, 3)) and lat > 0
+    finally:
+        srv.stop()
