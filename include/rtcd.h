@@ -65,13 +65,14 @@ typedef struct {
     uint64_t node_fetches;   /* BVH nodes visited (counting launches only) */
     uint64_t prim_tests;     /* primitive tests (segment-triangle, segment-tube) */
     uint64_t fp64_redecisions; /* tests whose FP32 result fell in the error band */
-    uint64_t candidates;     /* discrete: triangles in the broad-phase leaves; sweeps: (segment, sphere) tubes checked */
+    uint64_t candidates;     /* discrete: triangles in the broad-phase leaves; sweeps: (segment or sub-piece, sphere) tubes traversed */
     double last_ms;          /* device time of the last check call, ms */
     uint32_t max_stack;      /* traversal stack size used by the last call */
     uint32_t max_depth;      /* deepest BVH involved in the last call */
     uint64_t edge_tests;     /* of prim_tests: tests that read a 32 B edge record (the rest read 48 B) */
     double stage_ms[5];      /* discrete: FK, broad, (unused), (unused), narrow + FP64 re-decision (device ms) */
-    uint64_t leaf_candidates; /* discrete: (pose, link, scene leaf) records emitted by the broad phase */
+    uint64_t leaf_candidates; /* (pose, link, scene leaf) or (tube, scene leaf) records emitted by the broad phase */
+    uint64_t cert_calls;     /* sweeps: FP32 capsule certificates evaluated (quadratic: one per refined sub-piece) */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */
@@ -155,6 +156,18 @@ rtcd_status rtcd_poll(rtcd_ctx* ctx, uint64_t id, int32_t* done);
 /* Blocks until request `id` is done; returns its status and frees its record. */
 rtcd_status rtcd_wait(rtcd_ctx* ctx, uint64_t id, double* latency_ms);
 rtcd_status rtcd_serve_get_stats(rtcd_ctx* ctx, rtcd_serve_stats* stats, int32_t reset);
+
+/* Launch-shape parameters of the kernels.  No counterpart in the reference, whose launch shapes are
+ * fixed at compile time (CollisionDetector.h:396-433 batch sizes); the defaults were measured on
+ * MI355X and rtcd_set_param lets a caller (or a tuning sweep) change them per context. */
+typedef enum {
+    RTCD_PARAM_SWEEP_BUDGET = 1,       /* sweep: work units (node visits + certificates) a task spends
+                                          before handing its pending subtrees to the next pass (>= 1) */
+    RTCD_PARAM_FRONTIER_WORK = 2,      /* discrete: target number of (task, frontier entry) threads */
+    RTCD_PARAM_FRONTIER_MAX_TASKS = 3  /* discrete: batches with fewer (pose, link) tasks use the split */
+} rtcd_param;
+rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
+rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value);
 
 /* Counting mode: when enabled, check calls also count node fetches / primitive tests (slower). */
 rtcd_status rtcd_set_counting(rtcd_ctx* ctx, int32_t enable);

@@ -20,7 +20,8 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
            'rtcd_scene_destroy', 'rtcd_scene_info', 'rtcd_check_discrete', 'rtcd_check_discrete_device', 'rtcd_sync',
            'rtcd_check_linear', 'rtcd_check_linear_device', 'rtcd_check_quadratic', 'rtcd_check_quadratic_device',
            'rtcd_quadratic_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
-           'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_serve_get_stats']
+           'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param']
+PARAM_SWEEP_BUDGET, PARAM_FRONTIER_WORK, PARAM_FRONTIER_MAX_TASKS = 1, 2, 3
 REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC = 1, 2, 3
 
 
@@ -46,7 +47,7 @@ class Stats(C.Structure):
     _fields_ = [('node_fetches', C.c_uint64), ('prim_tests', C.c_uint64), ('fp64_redecisions', C.c_uint64),
                 ('candidates', C.c_uint64), ('last_ms', C.c_double), ('max_stack', C.c_uint32),
                 ('max_depth', C.c_uint32), ('edge_tests', C.c_uint64), ('stage_ms', C.c_double * 5),
-                ('leaf_candidates', C.c_uint64)]
+                ('leaf_candidates', C.c_uint64), ('cert_calls', C.c_uint64)]
 
 
 class Request(C.Structure):
@@ -101,6 +102,8 @@ def lib():
         L.rtcd_poll.argtypes = [vp, C.c_uint64, C.POINTER(i32)]
         L.rtcd_wait.argtypes = [vp, C.c_uint64, C.POINTER(C.c_double)]
         L.rtcd_serve_get_stats.argtypes = [vp, C.POINTER(ServeStats), i32]
+        L.rtcd_set_param.argtypes = [vp, u32, C.c_uint64]
+        L.rtcd_get_param.argtypes = [vp, u32, C.POINTER(C.c_uint64)]
         for name in EXPORTS:
             if name not in ('rtcd_destroy', 'rtcd_last_error'):
                 getattr(L, name).restype = C.c_int32

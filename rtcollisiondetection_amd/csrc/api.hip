@@ -84,9 +84,9 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     // split each (pose, link) task over F frontier entries so that small batches still fill the GPU
     // (large batches already fill the device: one thread per task walks the whole tree)
     int F = 1, foff = -1;
-    if (tasks < ((size_t) 1 << 17)) {
+    if (tasks < c->frontier_max_tasks) {
         for (const auto& lv : sc.front_levels)
-            if (tasks * (size_t) lv.second <= ((size_t) 1 << 23) || foff < 0) { F = lv.second; foff = lv.first; }
+            if (tasks * (size_t) lv.second <= c->frontier_work || foff < 0) { F = lv.second; foff = lv.first; }
     } else if (!sc.front_levels.empty()) {
         F = sc.front_levels[0].second;  // the root's children
         foff = sc.front_levels[0].first;
@@ -196,6 +196,7 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                     st->fp64_redecisions = k.fp64;
                     st->edge_tests = k.edge_tests;
                 }
+                st->cert_calls = 0;
             }
             ws.timing_pending = false;
             return;
@@ -388,6 +389,10 @@ rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
         ctx->r_sphere_link.alloc(std::max<size_t>(sl.size(), 1));
         ctx->r_sphere_link.upload(sl, s);
         ctx->n_spheres = (int) d->n_spheres;
+        std::vector<float> rad(std::max<size_t>(d->n_spheres, 1), 0.f);
+        for (uint32_t i = 0; i < d->n_spheres; ++i) rad[i] = d->spheres[4 * i + 3];
+        ctx->sphere_radius.alloc(rad.size());
+        ctx->sphere_radius.upload(rad, s);
         ctx->robot = R;
         ctx->d_robot.alloc(1);
         ctx->d_robot.upload(&ctx->robot, 1, s);
@@ -508,7 +513,7 @@ rtcd_status rtcd_check_discrete_device(rtcd_ctx* ctx, uint32_t scene_id, int32_t
             throw;
         }
         ws.stream = own;
-        ctx->pending = {true, sc, direction, d_poses, n, d_out, stream ? (hipStream_t) stream : own};
+        ctx->pending = {true, nullptr, sc, direction, d_poses, n, d_out, stream ? (hipStream_t) stream : own};
         return RTCD_OK;
     });
 }
@@ -519,7 +524,22 @@ rtcd_status rtcd_sync(rtcd_ctx* ctx, void* stream) {
     return guarded([&]() -> rtcd_status {
         hipStream_t s = stream ? (hipStream_t) stream : ctx->main.stream;
         RTCD_HIP(hipStreamSynchronize(s));
-        if (ctx->pending.active && ctx->pending.stream == s) {
+        if (ctx->pending.active && ctx->pending.stream == s && ctx->pending.sweep_rerun) {
+            auto rerun = ctx->pending.sweep_rerun;
+            ctx->pending.active = false;
+            ctx->pending.sweep_rerun = nullptr;
+            ctx->pending.scene.reset();
+            Workspace& ws = ctx->main;
+            hipStream_t own = ws.stream;
+            ws.stream = s;
+            try {
+                finish_sweep(ctx, ws, &ctx->last, rerun);
+            } catch (...) {
+                ws.stream = own;
+                throw;
+            }
+            ws.stream = own;
+        } else if (ctx->pending.active && ctx->pending.stream == s) {
             auto p = ctx->pending;
             ctx->pending.active = false;
             ctx->pending.scene.reset();
@@ -565,6 +585,37 @@ rtcd_status rtcd_check_discrete(rtcd_ctx* ctx, uint32_t scene_id, int32_t direct
         RTCD_HIP(hipStreamSynchronize(s));
         return RTCD_OK;
     });
+}
+
+rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
+    if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    switch (param) {
+    case RTCD_PARAM_SWEEP_BUDGET:
+        if (value < 1 || value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "sweep budget must be 1..2^32-1");
+        ctx->sweep_budget = (uint32_t) value;
+        return RTCD_OK;
+    case RTCD_PARAM_FRONTIER_WORK:
+        if (value < 1) return fail(RTCD_ERR_INVALID_ARG, "frontier work must be >= 1");
+        ctx->frontier_work = value;
+        return RTCD_OK;
+    case RTCD_PARAM_FRONTIER_MAX_TASKS:
+        ctx->frontier_max_tasks = value;
+        return RTCD_OK;
+    default:
+        return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
+    }
+}
+
+rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
+    if (!ctx || !value) return fail(RTCD_ERR_INVALID_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    switch (param) {
+    case RTCD_PARAM_SWEEP_BUDGET: *value = ctx->sweep_budget; return RTCD_OK;
+    case RTCD_PARAM_FRONTIER_WORK: *value = ctx->frontier_work; return RTCD_OK;
+    case RTCD_PARAM_FRONTIER_MAX_TASKS: *value = ctx->frontier_max_tasks; return RTCD_OK;
+    default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
+    }
 }
 
 rtcd_status rtcd_set_counting(rtcd_ctx* ctx, int32_t enable) {

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -140,6 +141,11 @@ struct rtcd_ctx {
     DevBuf<Edge> r_edges;
     DevBuf<float4> r_spheres;  // x y z r (link frame)
     DevBuf<int32_t> r_sphere_link;
+    DevBuf<float> sphere_radius;
+    // launch-shape parameters (rtcd_set_param)
+    uint32_t sweep_budget = 32;                 // work units per sweep task before it hands off its subtrees
+    uint64_t frontier_work = (uint64_t) 1 << 23;     // discrete: target (task, frontier entry) threads
+    uint64_t frontier_max_tasks = (uint64_t) 1 << 17; // discrete: split batches with fewer tasks
     int n_spheres = 0;
     int robot_depth = 1;
     // scenes (shared so that in-flight served requests keep a scene alive)
@@ -152,6 +158,7 @@ struct rtcd_ctx {
     // async discrete call waiting for rtcd_sync (for the capacity re-run)
     struct Pending {
         bool active = false;
+        std::function<void()> sweep_rerun;  // set for a pending *_device sweep
         std::shared_ptr<SceneRes> scene;
         int dir;
         const float* poses;
@@ -190,7 +197,8 @@ void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_tra
                     uint32_t n_traj, uint32_t nw, uint32_t* d_out, uint32_t* d_seg, size_t budget);
 void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
                        uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget);
-void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st);
+// After ws.stream finished: re-runs (through `rerun`) when the FP64 queue overflowed.
+void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::function<void()>& rerun);
 // Device bytes a request of this shape needs in a fresh workspace (admission control).
 size_t discrete_bytes(const rtcd_ctx* c, uint32_t n);
 size_t sweep_bytes(const rtcd_ctx* c, uint32_t n_traj, uint32_t n_wp_total, uint32_t n_ctrl);

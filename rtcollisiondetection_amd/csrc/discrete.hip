@@ -9,6 +9,7 @@
 // where its owned edges trace the static link-triangle LBVH (obstacle->robot) and the triangle
 // itself queries the static link-edge LBVH (robot->obstacle).
 #include "band32.hpp"
+#include "cand.hpp"
 #include "exact64.hpp"
 #include "kernels.hpp"
 
@@ -96,23 +97,6 @@ __device__ __forceinline__ bool broad_test(float lx, float ly, float lz, float h
     return fabsf(c.x - lc.x) <= hx2 + lh.x + pad && fabsf(c.y - lc.y) <= hy2 + lh.y + pad &&
            fabsf(c.z - lc.z) <= hz2 + lh.z + pad;
 }
-
-// Appends `want` (> 0) records for every calling lane: the lanes of one wave that reach this call
-// together add to the wave's LDS counter and the first of them reserves the wave total with one
-// global atomic.  Returns this lane's first slot.
-__device__ __forceinline__ uint32_t wave_alloc(uint32_t* wcnt, uint32_t* gcount, uint32_t want) {
-    const uint64_t act = __ballot(1);
-    const int leader = __ffsll((unsigned long long) act) - 1;
-    const int lane = (int) (threadIdx.x & 63);
-    const uint32_t local = atomicAdd(wcnt, want);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(gcount, atomicExch(wcnt, 0u));
-    base = (uint32_t) __shfl((int) base, leader, 64);
-    return base + local;
-}
-
-// Leaf candidate record: (pose * NL + link, first triangle | (count - 1) << 27).
-__device__ __forceinline__ uint32_t pack_leaf(int start, int n) { return (uint32_t) start | ((uint32_t) (n - 1) << 27); }
 
 // One pass: every (pose, link) task, split over F frontier entries, walks the scene triangle LBVH
 // and appends each leaf that passes both box tests to the candidate list.
@@ -369,35 +353,8 @@ __global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
     __shared__ int stk[STACK * kBS];
     __shared__ uint32_t seg_pre[kSeg + 1];
     int* st = stk + threadIdx.x;
-    // prefix over the candidate segments (clamped to their capacity): each thread scans kSeg / kBS
-    // consecutive counts, then the per-thread sums are scanned by thread 0
-    {
-        constexpr int per = kSeg / kBS;
-        __shared__ uint32_t part[kBS];
-        uint32_t loc[per], acc = 0;
-        for (int k = 0; k < per; ++k) {
-            loc[k] = min(a.seg_count[threadIdx.x * per + k], a.seg_cap);
-            acc += loc[k];
-        }
-        part[threadIdx.x] = acc;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t run = 0;
-            for (int k = 0; k < kBS; ++k) {
-                const uint32_t x = part[k];
-                part[k] = run;
-                run += x;
-            }
-            seg_pre[kSeg] = run;
-        }
-        __syncthreads();
-        uint32_t run = part[threadIdx.x];
-        for (int k = 0; k < per; ++k) {
-            seg_pre[threadIdx.x * per + k] = run;
-            run += loc[k];
-        }
-        __syncthreads();
-    }
+    __shared__ uint32_t part[kBS];
+    seg_prefix<kBS>(a.seg_count, a.seg_cap, seg_pre, part);
     // each leaf record is spread over `lanes` adjacent threads, one triangle per thread
     const uint32_t lanes = (uint32_t) a.leaf_lanes;
     const uint64_t total = (uint64_t) seg_pre[kSeg] * lanes;
@@ -405,13 +362,8 @@ __global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
     Ctr ctr;
     for (uint64_t v = blockIdx.x * kBS + threadIdx.x; v < total; v += stride) {
         const uint32_t i = (uint32_t) (v / lanes), sub_k = (uint32_t) (v - (uint64_t) i * lanes);
-        int lo = 0, hi = kSeg;  // largest segment with seg_pre[lo] <= i
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (seg_pre[mid] <= i) lo = mid; else hi = mid;
-        }
-        const uint2 c = a.cand[(size_t) lo * a.seg_cap + (i - seg_pre[lo])];
-        if (sub_k > (c.y >> 27)) continue;
+        const uint2 c = seg_record(a.cand, a.seg_cap, seg_pre, i);
+        if (sub_k > leaf_last(c.y)) continue;
         const uint32_t p = c.x / (uint32_t) a.n_links, l = c.x - p * (uint32_t) a.n_links;
         uint32_t* flag = a.flags + p;
         if (load_flag(flag)) continue;
@@ -420,7 +372,7 @@ __global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
         const float tinf = fmaxf(fmaxf(fabsf(r0.w), fabsf(r1.w)), fabsf(r2.w));
         int hit = 0;
         {
-            const uint32_t tri = (c.y & 0x7ffffffu) + sub_k;
+            const uint32_t tri = leaf_start(c.y) + sub_k;
             const Tri t = a.sc.tri_prims[tri];
             // vertex A moved as R^T (A - t); edges rotated from world differences, so that their
             // error is relative to their own length: |A' - A'_exact| <= eps |t| + 5 eps |A - t|_1

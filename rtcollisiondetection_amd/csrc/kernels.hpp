@@ -26,7 +26,7 @@ struct SceneDev {
 };
 
 struct Stats {
-    unsigned long long node_fetches, prim_tests, fp64, candidates, edge_tests;
+    unsigned long long node_fetches, prim_tests, fp64, candidates, edge_tests, certs;
 };
 
 struct DiscreteArgs {
@@ -81,10 +81,27 @@ struct SweepArgs {
     uint32_t* out;             // [n_traj]
     uint32_t* out_seg;         // [n_seg] or nullptr
     Stats* stats;
+    const float* radius;       // [S] sphere radii
+    const float4* c32;         // sphere centres [n_wp_total][S] (x, y, z, r), FP32
+    const float4* P32;         // quadratic control points [n_traj][S][n_ctrl] (x, y, z, r), FP32
+    uint32_t n_items;          // n_seg * (1 or the quadratic sub-pieces per piece) * S
+    uint2* spill[2];           // hand-off queues of (item, edge-BVH node), used by the passes in turn
+    uint32_t spill_cap;        // records per queue
+    uint32_t* spill_count;     // [kSweepPasses - 1] records handed off by each pass
+    uint32_t budget;           // work units (node visits + certificates) before a task hands off
+    uint4* unsure;             // FP64 re-decision queue: (segment or piece, sphere, scene edge, 0)
+    uint32_t unsure_cap;
+    uint32_t* unsure_count;
+    uint32_t* overflow;        // bit 2: stack overflow, 4: FP64 queue overflow
 };
-// Tubes of degree 1 (capsules between waypoints) or 2 (quadratic B-spline pieces) vs scene edges.
-void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s, const float4* c32,
-                  const float4* P32);
+// Sweep passes: the first starts one task per item, each later one runs the subtrees handed off by
+// the previous pass; the last never hands off.
+constexpr int kSweepPasses = 4;
+// Tubes of degree 1 (capsules between waypoints) or 2 (quadratic B-spline pieces) vs scene edges:
+// kSweepPasses any-hit passes with FP32 certificates, then the FP64 re-decision of the queue.
+void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s);
+// Quadratic pieces are split into this many sub-pieces (work items) each.
+constexpr int kSweepSub = 4;
 void quadratic_weights(int nc, int nw, double* W);
 
 }  // namespace rtcd

@@ -134,6 +134,23 @@ void complete(ServeState* S, Slot& sl) {
                                     sl.ws.stream));
             RTCD_HIP(hipStreamSynchronize(sl.ws.stream));
         }
+    } else {
+        // re-runs synchronously if a candidate segment or the FP64 queue overflowed
+        std::vector<uint32_t> sv = sl.ws.scalars.download(3, sl.ws.stream);
+        if (sv[1] & 7u) {
+            auto rerun = [&]() {
+                if (r.kind == RTCD_REQ_LINEAR)
+                    enqueue_linear(c, sl.ws, *q->scene, sl.ws.h_poses.p, sl.ws.h_offsets.p, r.n, q->n_wp_total,
+                                   sl.ws.h_out.p, nullptr, 0);
+                else
+                    enqueue_quadratic(c, sl.ws, *q->scene, sl.ws.h_poses.p, r.n, r.n_wp, r.n_ctrl, nullptr, sl.ws.h_out.p,
+                                      nullptr, 0);
+            };
+            finish_sweep(c, sl.ws, nullptr, rerun);
+            RTCD_HIP(hipMemcpyAsync(sl.out.p, sl.ws.h_out.p, sizeof(uint32_t) * r.n, hipMemcpyDeviceToHost,
+                                    sl.ws.stream));
+            RTCD_HIP(hipStreamSynchronize(sl.ws.stream));
+        }
     }
     std::memcpy(r.out, sl.out.p, sizeof(uint32_t) * r.n);
 }

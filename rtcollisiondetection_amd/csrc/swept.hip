@@ -148,8 +148,8 @@ __device__ int cap_cert(float3 C0, float3 C1, float3 A, float3 B, float r_hit, f
 // decides it; undecided sub-pieces are refined down to depth kQuadDepth.  Sub-piece controls come
 // from the blossom B(u, v) = (1-u)(1-v) q0 + ((1-u) v + u (1-v)) q1 + u v q2: a = B(t0, t0),
 // m = B(t0, t1), b = B(t1, t1).
-constexpr int kQuadDepth = 4;
-constexpr int kSub = 4;  // work items per quadratic piece
+constexpr int kQuadDepth = 8;
+constexpr int kSub = kSweepSub;  // work items per quadratic piece
 
 __device__ __forceinline__ float3 blossom(float3 q0, float3 q1, float3 q2, float u, float v) {
     const float w0 = (1.f - u) * (1.f - v), w1 = (1.f - u) * v + u * (1.f - v), w2 = u * v;
@@ -157,7 +157,7 @@ __device__ __forceinline__ float3 blossom(float3 q0, float3 q1, float3 q2, float
                        w0 * q0.z + w1 * q1.z + w2 * q2.z);
 }
 
-__device__ int quad_cert(float3 q0, float3 q1, float3 q2, float3 A, float3 B, float r, float ein) {
+__device__ int quad_cert(float3 q0, float3 q1, float3 q2, float3 A, float3 B, float r, float ein, uint32_t& n_cert) {
     const float M = fmaxf(fmaxf(linf(q0), linf(q1)), linf(q2));
     // blossom weights are exact at dyadic parameters; the weighted sum adds 6 eps M per component
     const float e2 = ein + 8.f * kEps32 * M;
@@ -170,6 +170,7 @@ __device__ int quad_cert(float3 q0, float3 q1, float3 q2, float3 A, float3 B, fl
         const float3 dev = pm - 0.5f * (pa + pb);
         const float h = 0.5f * sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 2.f * 1.7321f * e2;
         const int cls = cap_cert(pa, pb, A, B, r - h, r + h, e2);
+        ++n_cert;
         if (cls == kHit) return kHit;
         if (cls != kMiss) {
             if (d < kQuadDepth) {  // refine: first child
@@ -190,156 +191,254 @@ __device__ int quad_cert(float3 q0, float3 q1, float3 q2, float3 A, float3 B, fl
     return all_miss ? kMiss : kUnsure;
 }
 
+// ---------------------------------------------------------------------------------- sweep items
+// Item it = (g * SUB + k) * S + s: sub-piece k of segment / quadratic piece g, sphere s.  Spheres run
+// fastest, so the lanes of a wave hold the spheres of one sub-piece and traverse coherently.
+struct Item {
+    uint32_t g, traj;
+    int s;
+    float3 q0, q1, q2;  // capsule axis (q0, q1) or Bezier sub-piece controls (q0, q1, q2)
+    float r;            // sphere radius
+    float ein;          // max abs. error of the FP32 controls per component
+    float M;            // max |coordinate| of the controls
+};
+
+template <int DEG>
+__device__ __forceinline__ Item make_item(const SweepArgs& a, uint32_t it, int S) {
+    constexpr int SUB = DEG == 1 ? 1 : kSub;
+    Item m;
+    const uint32_t gk = it / (uint32_t) S;
+    m.s = (int) (it - gk * (uint32_t) S);
+    m.g = gk / (uint32_t) SUB;
+    if (DEG == 1) {
+        m.traj = a.seg_traj[m.g];
+        const uint32_t w = a.seg_wp[m.g];
+        const float4 x0 = a.c32[(size_t) w * S + m.s], x1 = a.c32[(size_t) (w + 1) * S + m.s];
+        m.q0 = f3(x0);
+        m.q1 = f3(x1);
+        m.q2 = m.q1;
+        m.r = x0.w;
+        m.M = fmaxf(linf(m.q0), linf(m.q1));
+        m.ein = 1.01f * kEps32 * m.M;
+    } else {
+        const int sub_k = (int) (gk - m.g * (uint32_t) SUB);
+        const int npc = a.n_ctrl - 2;
+        m.traj = m.g / (uint32_t) npc;
+        const int piece = (int) (m.g - m.traj * (uint32_t) npc);
+        const size_t pb = ((size_t) m.traj * S + m.s) * a.n_ctrl + piece;
+        const float4 p0 = a.P32[pb], p1 = a.P32[pb + 1], p2 = a.P32[pb + 2];
+        // Bezier controls of the piece ((P_i + P_i+1) / 2, P_i+1, (P_i+1 + P_i+2) / 2): 3 eps M error
+        const float3 b0 = 0.5f * (f3(p0) + f3(p1)), b1 = f3(p1), b2 = 0.5f * (f3(p1) + f3(p2));
+        const float Mp = fmaxf(fmaxf(linf(b0), linf(b1)), linf(b2));
+        // this item's sub-piece from the blossom (exact weights at dyadic parameters): + 8 eps M
+        const float t0 = (float) sub_k / (float) SUB, t1 = (float) (sub_k + 1) / (float) SUB;
+        m.q0 = blossom(b0, b1, b2, t0, t0);
+        m.q1 = blossom(b0, b1, b2, t0, t1);
+        m.q2 = blossom(b0, b1, b2, t1, t1);
+        m.ein = 3.f * kEps32 * Mp + 8.f * kEps32 * Mp;
+        m.M = fmaxf(fmaxf(linf(m.q0), linf(m.q1)), linf(m.q2));
+        m.r = p0.w;
+    }
+    return m;
+}
+
+// Node culling volume of an item: the box of its controls grown by the radius and error pad, and the
+// chord q0 -> q_end grown by the same pad plus, for a curved piece, its deviation from the chord.
+struct Cull {
+    float lx, ly, lz, hx, hy, hz;
+    float3 cp, cinv;
+    float cpad;
+};
+
+template <int DEG>
+__device__ __forceinline__ Cull make_cull(const Item& m, float scale) {
+    Cull c;
+    const float pad = m.r + 4.f * m.ein + 1e-6f * (1.f + scale);
+    c.lx = fminf(m.q0.x, m.q1.x); c.ly = fminf(m.q0.y, m.q1.y); c.lz = fminf(m.q0.z, m.q1.z);
+    c.hx = fmaxf(m.q0.x, m.q1.x); c.hy = fmaxf(m.q0.y, m.q1.y); c.hz = fmaxf(m.q0.z, m.q1.z);
+    if (DEG == 2) {
+        c.lx = fminf(c.lx, m.q2.x); c.ly = fminf(c.ly, m.q2.y); c.lz = fminf(c.lz, m.q2.z);
+        c.hx = fmaxf(c.hx, m.q2.x); c.hy = fmaxf(c.hy, m.q2.y); c.hz = fmaxf(c.hz, m.q2.z);
+    }
+    c.lx -= pad; c.ly -= pad; c.lz -= pad; c.hx += pad; c.hy += pad; c.hz += pad;
+    c.cp = m.q0;
+    c.cinv = safe_inv((DEG == 1 ? m.q1 : m.q2) - m.q0);
+    c.cpad = pad;
+    if (DEG == 2) {
+        // a quadratic sub-piece lies within |q1 - (q0 + q2) / 2| / 2 of its chord
+        const float3 dev = m.q1 - 0.5f * (m.q0 + m.q2);
+        c.cpad += 0.5f * sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 4.f * 1.7321f * (m.ein + 8.f * kEps32 * m.M);
+    }
+    return c;
+}
+
+__device__ __forceinline__ bool cull_ok(const Cull& c, float bx0, float by0, float bz0, float bx1, float by1, float bz1) {
+    if (c.lx > bx1 || c.hx < bx0 || c.ly > by1 || c.hy < by0 || c.lz > bz1 || c.hz < bz0) return false;
+    return slab(c.cp, c.cinv, bx0, by0, bz0, bx1, by1, bz1, c.cpad);
+}
+
 // ---------------------------------------------------------------------------------- sweep kernel
+// One task = (item, subtree of the scene edge LBVH).  Pass 0 starts one task per item at the root;
+// each task walks its subtree with the culling volume and tests the edges of every leaf it reaches
+// with the FP32 certificate, stopping at the first hit (any-hit) or once the item's flag is set by
+// another task.  A task that has spent a.budget units of work (node visits + certificates) in a
+// pass that may hand off work appends its pending subtrees (the next node and its stack) to the
+// queue of the next pass and ends, so that a tube near much geometry is spread over many threads
+// instead of holding one wave; the last pass never hands off.  Undecided (item, edge) pairs go to
+// the FP64 queue.
 template <int DEG, int STACK, bool COUNT>
-__global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, const float4* __restrict__ c32, const float4* __restrict__ P32) {
+__global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, int pass) {
     __shared__ int stk[STACK * kBS];
     int* st = stk + threadIdx.x;
-    const int S = a.robot->n_spheres;
-    const int npc = a.n_ctrl - 2;
-    // quadratic pieces are split into kSub work items, one per sub-piece [k/kSub, (k+1)/kSub]
-    constexpr int SUB = DEG == 1 ? 1 : kSub;
-    const size_t n_items = (size_t) a.n_seg * SUB * S;
-    const size_t stride = (size_t) gridDim.x * kBS;
-    uint32_t c_nodes = 0, c_prims = 0, c_fp64 = 0, c_items = 0;
-    for (size_t it = (size_t) blockIdx.x * kBS + threadIdx.x; it < n_items; it += stride) {
-        const uint32_t gk = (uint32_t) (it / S);
-        const int s = (int) (it - (size_t) gk * S);
-        const uint32_t g = gk / (uint32_t) SUB;
-        const int sub_k = (int) (gk - g * (uint32_t) SUB);
-        uint32_t traj;
-        float3 q[3];
-        float r;
-        if (DEG == 1) {
-            traj = a.seg_traj[g];
-            const uint32_t w = a.seg_wp[g];
-            const float4 x0 = c32[(size_t) w * S + s], x1 = c32[(size_t) (w + 1) * S + s];
-            q[0] = f3(x0);
-            q[1] = f3(x1);
-            r = x0.w;
-        } else {
-            traj = g / (uint32_t) npc;
-            const int piece = (int) (g - traj * npc);
-            const size_t pb = ((size_t) traj * S + s) * a.n_ctrl + piece;
-            const float4 p0 = P32[pb], p1 = P32[pb + 1], p2 = P32[pb + 2];
-            q[0] = 0.5f * (f3(p0) + f3(p1));
-            q[1] = f3(p1);
-            q[2] = 0.5f * (f3(p1) + f3(p2));
-            r = p0.w;
+    const int S = a.robot_n_spheres_host;
+    const uint2* in_q = pass > 0 ? a.spill[(pass - 1) & 1] : nullptr;
+    uint2* out_q = a.spill[pass & 1];
+    uint32_t* out_cnt = a.spill_count + pass;
+    const bool may_spill = pass < kSweepPasses - 1;
+    const uint32_t n_tasks = pass == 0 ? a.n_items : min(a.spill_count[pass - 1], a.spill_cap);
+    const uint32_t stride = gridDim.x * kBS;
+    uint32_t c_nodes = 0, c_prims = 0, c_fp64 = 0, c_items = 0, c_certs = 0;
+    for (uint32_t t = blockIdx.x * kBS + threadIdx.x; t < n_tasks; t += stride) {
+        uint32_t it = t;
+        int ni = 0;
+        if (pass > 0) {
+            const uint2 r = in_q[t];
+            if (r.x == 0xffffffffu) continue;  // slot of a hand-off that did not fit
+            it = r.x;
+            ni = (int) r.y;
         }
-        uint32_t* flag = a.out_seg ? a.out_seg + g : a.out + traj;
+        const Item m = make_item<DEG>(a, it, S);
+        uint32_t* flag = a.out_seg ? a.out_seg + m.g : a.out + m.traj;
         if (load_flag(flag)) continue;
-        if (COUNT) c_items++;
-        float M = fmaxf(fmaxf(linf(q[0]), linf(q[1])), DEG == 2 ? linf(q[2]) : 0.f);
-        float ein = (DEG == 1 ? 1.01f : 3.f) * kEps32 * M;
-        if (DEG == 2) {
-            // this item's sub-piece, controls from the blossom (exact weights at dyadic parameters)
-            const float t0 = (float) sub_k / (float) SUB, t1 = (float) (sub_k + 1) / (float) SUB;
-            const float3 sa = blossom(q[0], q[1], q[2], t0, t0), sm = blossom(q[0], q[1], q[2], t0, t1),
-                         sb = blossom(q[0], q[1], q[2], t1, t1);
-            q[0] = sa;
-            q[1] = sm;
-            q[2] = sb;
-            ein += 8.f * kEps32 * M;
-            M = fmaxf(fmaxf(linf(q[0]), linf(q[1])), linf(q[2]));
-        }
-        const float pad = r + 4.f * ein + 1e-6f * (1.f + a.sc.scale);
-        float lx = fminf(q[0].x, q[1].x), ly = fminf(q[0].y, q[1].y), lz = fminf(q[0].z, q[1].z);
-        float hx = fmaxf(q[0].x, q[1].x), hy = fmaxf(q[0].y, q[1].y), hz = fmaxf(q[0].z, q[1].z);
-        if (DEG == 2) {
-            lx = fminf(lx, q[2].x); ly = fminf(ly, q[2].y); lz = fminf(lz, q[2].z);
-            hx = fmaxf(hx, q[2].x); hy = fmaxf(hy, q[2].y); hz = fmaxf(hz, q[2].z);
-        }
-        lx -= pad; ly -= pad; lz -= pad; hx += pad; hy += pad; hz += pad;
-        // Tighter node test: the tube lies within r (+ h for a curved piece) of the chord, so a node box
-        // can hold a candidate edge only if the chord meets the box grown by that margin.
-        const float3 cp = q[0];
-        const float3 cinv = safe_inv((DEG == 1 ? q[1] : q[2]) - q[0]);
-        float cpad = pad;
-        if (DEG == 2) {
-            const float3 dev = q[1] - 0.5f * (q[0] + q[2]);
-            cpad += 0.5f * sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 4.f * 1.7321f * (ein + 8.f * kEps32 * M);
-        }
-        auto node_ok = [&](float bx0, float by0, float bz0, float bx1, float by1, float bz1) {
-            if (lx > bx1 || hx < bx0 || ly > by1 || hy < by0 || lz > bz1 || hz < bz0) return false;
-            return slab(cp, cinv, bx0, by0, bz0, bx1, by1, bz1, cpad);
-        };
-        int hit = 0;
-        if (a.sc.edge_has_root) {
-            int sp = 0, ni = 0, iter = 0;
-            for (;;) {
-                if ((++iter & 7) == 0 && load_flag(flag)) break;
-                const BNode nd = a.sc.edge_nodes[ni];
-                if (COUNT) c_nodes++;
-                bool h0 = nd.c0 >= 0 && node_ok(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
-                bool h1 = nd.c1 >= 0 && node_ok(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
+        if (COUNT && pass == 0) c_items++;
+        const Cull cu = make_cull<DEG>(m, a.sc.scale);
+        bool spill_ok = may_spill;
+        int hit = 0, sp = 0, iter = 0;
+        uint32_t work = 0;
+        for (;;) {
+            if ((++iter & 7) == 0 && load_flag(flag)) break;
+            if (spill_ok && work > a.budget) {
+                const uint32_t want = (uint32_t) sp + 1u;
+                const uint32_t o = atomicAdd(out_cnt, want);
+                if ((uint64_t) o + want <= a.spill_cap) {
+                    out_q[o] = make_uint2(it, (uint32_t) ni);
+                    for (int j = 0; j < sp; ++j) out_q[o + 1 + j] = make_uint2(it, (uint32_t) st[j * kBS]);
+                    break;
+                }
+                // queue full: mark the reserved slots empty and finish this subtree here
+                for (uint64_t j = o; j < a.spill_cap && j < (uint64_t) o + want; ++j)
+                    out_q[j] = make_uint2(0xffffffffu, 0u);
+                spill_ok = false;
+            }
+            const BNode nd = a.sc.edge_nodes[ni];
+            ++work;
+            if (COUNT) c_nodes++;
+            bool h0 = nd.c0 >= 0 && cull_ok(cu, nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
+            bool h1 = nd.c1 >= 0 && cull_ok(cu, nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
 #pragma unroll
-                for (int side = 0; side < 2 && !hit; ++side) {
-                    const bool hb = side ? h1 : h0;
-                    const int cnt = side ? nd.n1 : nd.n0;
-                    const int start = side ? nd.c1 : nd.c0;
-                    if (hb && cnt > 0) {
-                        for (int k = 0; k < cnt && !hit; ++k) {
-                            const Edge e = a.sc.edge_prims[start + k];
-                            if (COUNT) c_prims++;
-                            const float3 A = f3(e.p), B = f3(e.q);
-                            int cls = DEG == 1 ? cap_cert(q[0], q[1], A, B, r, r, ein) : quad_cert(q[0], q[1], q[2], A, B, r, ein);
-                            if (cls == kHit) {
-                                hit = 1;
-                            } else if (cls == kUnsure) {
-                                if (COUNT) c_fp64++;
-                                double ctrl[9];
-                                if (DEG == 1) {
-                                    const uint32_t w = a.seg_wp[g];
-                                    for (int c = 0; c < 3; ++c) {
-                                        ctrl[c] = a.centers64[((size_t) w * S + s) * 3 + c];
-                                        ctrl[3 + c] = a.centers64[((size_t) (w + 1) * S + s) * 3 + c];
-                                    }
-                                } else {
-                                    const int piece = (int) (g - traj * npc);
-                                    const double* P = a.ctrl64 + (((size_t) traj * S + s) * a.n_ctrl + piece) * 3;
-                                    for (int c = 0; c < 3; ++c) {
-#pragma clang fp contract(off)
-                                        ctrl[c] = (P[c] + P[3 + c]) * 0.5;
-                                        ctrl[3 + c] = P[3 + c];
-                                        ctrl[6 + c] = (P[3 + c] + P[6 + c]) * 0.5;
-                                    }
-                                }
-                                double pa[3] = {e.p.x, e.p.y, e.p.z}, pb2[3] = {e.q.x, e.q.y, e.q.z};
-                                hit = x64::tube(DEG, ctrl, (double) r, pa, pb2);
-                            }
+            for (int side = 0; side < 2 && !hit; ++side) {
+                const bool hb = side ? h1 : h0;
+                const int cnt = side ? nd.n1 : nd.n0;
+                const int start = side ? nd.c1 : nd.c0;
+                if (hb && cnt > 0) {
+                    for (int k = 0; k < cnt && !hit; ++k) {
+                        const Edge e = a.sc.edge_prims[start + k];
+                        if (COUNT) c_prims++;
+                        const float3 A = f3(e.p), B = f3(e.q);
+                        uint32_t nc = 0;
+                        int cls;
+                        if (DEG == 1) {
+                            cls = cap_cert(m.q0, m.q1, A, B, m.r, m.r, m.ein);
+                            nc = 1;
+                        } else {
+                            cls = quad_cert(m.q0, m.q1, m.q2, A, B, m.r, m.ein, nc);
                         }
-                        if (side) h1 = false; else h0 = false;
+                        work += nc;
+                        if (COUNT) c_certs += nc;
+                        if (cls == kHit) {
+                            hit = 1;
+                        } else if (cls == kUnsure) {
+                            if (COUNT) c_fp64++;
+                            const uint32_t slot = atomicAdd(a.unsure_count, 1u);
+                            if (slot < a.unsure_cap) a.unsure[slot] = make_uint4(m.g, (uint32_t) m.s, (uint32_t) (start + k), 0u);
+                            else atomicOr(a.overflow, 4u);
+                        }
                     }
+                    if (side) h1 = false; else h0 = false;
                 }
-                if (hit) break;
-                if (h0 && h1) {
-                    if (sp >= STACK) break;
-                    st[sp * kBS] = nd.c1;
-                    ++sp;
-                    ni = nd.c0;
-                } else if (h0) {
-                    ni = nd.c0;
-                } else if (h1) {
-                    ni = nd.c1;
-                } else {
-                    if (sp == 0) break;
-                    --sp;
-                    ni = st[sp * kBS];
-                }
+            }
+            if (hit) break;
+            if (h0 && h1) {
+                if (sp >= STACK) { atomicOr(a.overflow, 2u); break; }
+                st[sp * kBS] = nd.c1;
+                ++sp;
+                ni = nd.c0;
+            } else if (h0) {
+                ni = nd.c0;
+            } else if (h1) {
+                ni = nd.c1;
+            } else {
+                if (sp == 0) break;
+                --sp;
+                ni = st[sp * kBS];
             }
         }
         if (hit) {
-            atomicOr(a.out + traj, 1u);
-            if (a.out_seg) atomicOr(a.out_seg + g, 1u);
+            atomicOr(a.out + m.traj, 1u);
+            if (a.out_seg) atomicOr(a.out_seg + m.g, 1u);
         }
     }
     if (COUNT) {
         atomicAdd(&a.stats->node_fetches, (unsigned long long) c_nodes);
         atomicAdd(&a.stats->prim_tests, (unsigned long long) c_prims);
-        atomicAdd(&a.stats->fp64, (unsigned long long) c_fp64);
-        atomicAdd(&a.stats->candidates, (unsigned long long) c_items);
         atomicAdd(&a.stats->edge_tests, (unsigned long long) c_prims);
+        atomicAdd(&a.stats->fp64, (unsigned long long) c_fp64);
+        atomicAdd(&a.stats->certs, (unsigned long long) c_certs);
+        atomicAdd(&a.stats->candidates, (unsigned long long) c_items);
+    }
+}
+
+// FP64 re-decision of the queued (segment or piece, sphere, scene edge) tests, exactly as the oracle
+// computes them: the capsule between the FP64 sphere centres, or the Bezier piece
+// ((P_i + P_{i+1}) / 2, P_{i+1}, (P_{i+1} + P_{i+2}) / 2) of the FP64 control points.
+template <int DEG>
+__global__ __launch_bounds__(kBS) void k_sweep_redecide(SweepArgs a) {
+    const uint32_t total = min(*a.unsure_count, a.unsure_cap);
+    const int S = a.robot->n_spheres;
+    const int npc = a.n_ctrl - 2;
+    for (uint32_t i = blockIdx.x * kBS + threadIdx.x; i < total; i += gridDim.x * kBS) {
+        const uint4 u = a.unsure[i];
+        const uint32_t g = u.x;
+        const int s = (int) u.y;
+        const uint32_t traj = DEG == 1 ? a.seg_traj[g] : g / (uint32_t) npc;
+        uint32_t* flag = a.out_seg ? a.out_seg + g : a.out + traj;
+        if (load_flag(flag)) continue;
+        double ctrl[9];
+        float r;
+        if (DEG == 1) {
+            const uint32_t w = a.seg_wp[g];
+            for (int c = 0; c < 3; ++c) {
+                ctrl[c] = a.centers64[((size_t) w * S + s) * 3 + c];
+                ctrl[3 + c] = a.centers64[((size_t) (w + 1) * S + s) * 3 + c];
+            }
+            r = a.radius[s];
+        } else {
+            const int piece = (int) (g - traj * npc);
+            const double* P = a.ctrl64 + (((size_t) traj * S + s) * a.n_ctrl + piece) * 3;
+            for (int c = 0; c < 3; ++c) {
+#pragma clang fp contract(off)
+                ctrl[c] = (P[c] + P[3 + c]) * 0.5;
+                ctrl[3 + c] = P[3 + c];
+                ctrl[6 + c] = (P[3 + c] + P[6 + c]) * 0.5;
+            }
+            r = a.radius[s];
+        }
+        const Edge e = a.sc.edge_prims[u.z];
+        double pa[3] = {e.p.x, e.p.y, e.p.z}, pb[3] = {e.q.x, e.q.y, e.q.z};
+        if (x64::tube(DEG, ctrl, (double) r, pa, pb)) {
+            atomicOr(a.out + traj, 1u);
+            if (a.out_seg) atomicOr(a.out_seg + g, 1u);
+        }
     }
 }
 
@@ -354,12 +453,15 @@ int device_cus() {
 }
 
 template <int DEG, int STACK>
-void sweep_t(const SweepArgs& a, const float4* c32, const float4* P32, bool count, hipStream_t s) {
-    const size_t items = (size_t) a.n_seg * a.robot_n_spheres_host * (DEG == 1 ? 1 : kSub);
-    unsigned nb = (unsigned) std::min<size_t>(blocks_for(items, kBS), (size_t) device_cus() * 8);
-    if (!nb) return;
-    if (count) hipLaunchKernelGGL((k_sweep<DEG, STACK, true>), dim3(nb), dim3(kBS), 0, s, a, c32, P32);
-    else hipLaunchKernelGGL((k_sweep<DEG, STACK, false>), dim3(nb), dim3(kBS), 0, s, a, c32, P32);
+void sweep_t(const SweepArgs& a, bool count, hipStream_t s) {
+    const unsigned cap_blocks = (unsigned) device_cus() * 8u;
+    for (int pass = 0; pass < kSweepPasses; ++pass) {
+        const unsigned nb = pass == 0 ? std::min<unsigned>(blocks_for(a.n_items, kBS), cap_blocks) : cap_blocks;
+        if (count) hipLaunchKernelGGL((k_sweep<DEG, STACK, true>), dim3(nb), dim3(kBS), 0, s, a, pass);
+        else hipLaunchKernelGGL((k_sweep<DEG, STACK, false>), dim3(nb), dim3(kBS), 0, s, a, pass);
+        RTCD_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL((k_sweep_redecide<DEG>), dim3((unsigned) device_cus() * 4u), dim3(kBS), 0, s, a);
     RTCD_LAUNCH_CHECK();
 }
 
@@ -408,14 +510,15 @@ void quadratic_weights(int nc, int nw, double* W) {
         for (int k = 0; k < nw; ++k) W[(size_t) r * nw + k] = M[(size_t) r * cols + nc + k];
 }
 
-void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s, const float4* c32,
-                  const float4* P32) {
+void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s) {
+    if (!a.sc.edge_has_root || a.n_items == 0) return;
+    if (stack > 64) throw std::runtime_error("BVH deeper than 64 levels");
     if (degree == 1) {
-        if (stack <= 32) sweep_t<1, 32>(a, c32, P32, count, s);
-        else sweep_t<1, 64>(a, c32, P32, count, s);
+        if (stack <= 32) sweep_t<1, 32>(a, count, s);
+        else sweep_t<1, 64>(a, count, s);
     } else {
-        if (stack <= 32) sweep_t<2, 32>(a, c32, P32, count, s);
-        else sweep_t<2, 64>(a, c32, P32, count, s);
+        if (stack <= 32) sweep_t<2, 32>(a, count, s);
+        else sweep_t<2, 64>(a, count, s);
     }
 }
 
@@ -424,11 +527,20 @@ void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStre
 // ============================================================================================ host side
 namespace rtcd {
 
+// Work items of a sweep: n_seg segments (linear) or pieces (quadratic, kSweepSub sub-pieces each)
+// times the spheres.
+static size_t sweep_items(const rtcd_ctx* c, size_t n_seg, bool quadratic) {
+    return n_seg * (quadratic ? kSweepSub : 1) * (size_t) c->n_spheres;
+}
+
 size_t sweep_bytes(const rtcd_ctx* c, uint32_t n_traj, uint32_t nw, uint32_t nc) {
     const size_t S = (size_t) c->n_spheres;
     size_t b = (size_t) nw * S * (24 + 16) + (size_t) nw * (4 * c->robot.dof) + (size_t) n_traj * 8 + 64;
+    const size_t n_seg = nc ? (size_t) n_traj * (nc - 2) : (size_t) (nw - std::min(nw, n_traj));
     if (nc) b += (size_t) n_traj * S * nc * (24 + 16) + (size_t) nc * (nw / std::max<uint32_t>(n_traj, 1)) * 8;
     else b += (size_t) nw * 8;
+    b += ((size_t) 2 << 20) * sizeof(uint2) + ((size_t) 1 << 16) * 16;
+    (void) n_seg;
     return b;
 }
 
@@ -452,6 +564,12 @@ static void enqueue_centers(rtcd_ctx* c, Workspace& ws, const float* d_poses, in
 
 static SweepArgs base_args(rtcd_ctx* c, Workspace& ws, SceneRes& sc, uint32_t* d_out, uint32_t* d_seg) {
     SweepArgs a{};
+    a.c32 = ws.c32.p;
+    a.unsure = ws.unsure.p;
+    a.unsure_cap = (uint32_t) std::min<size_t>(ws.unsure.n, 0xffffffffu);
+    a.unsure_count = ws.scalars.p + 2;
+    a.overflow = ws.scalars.p + 1;
+    a.radius = c->sphere_radius.p;
     a.robot = c->d_robot.p;
     a.robot_n_spheres_host = c->n_spheres;
     a.sc = sc.dev();
@@ -464,8 +582,29 @@ static SweepArgs base_args(rtcd_ctx* c, Workspace& ws, SceneRes& sc, uint32_t* d
 
 static void sweep_begin(Workspace& ws) {
     ws.stats.reserve(1);
+    ws.scalars.reserve(8);
+    if (ws.unsure.n < ((size_t) 1 << 16)) ws.unsure.reserve((size_t) 1 << 16);
+    fill_u32(ws.scalars.p, 8, 0u, ws.stream);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, ws.stream);
     RTCD_HIP(hipEventRecord(ws.ev[0], ws.stream));
+}
+
+// Hand-off queues of a sweep launch: two halves of the workspace candidate buffer, used in turn by
+// the passes.  A full queue only stops hand-offs, so its size is a tuning choice.
+static void sweep_work(rtcd_ctx* c, Workspace& ws, SweepArgs& a, size_t items, size_t budget) {
+    if (items > 0xffffffffull) throw std::invalid_argument("too many (segment, sphere) items in one call");
+    a.n_items = (uint32_t) items;
+    const size_t need = (size_t) 2 << 20;
+    if (ws.cand.n < need) {
+        if (budget && ws.bytes() + (need - ws.cand.n) * sizeof(uint2) > budget)
+            throw HipError(hipErrorOutOfMemory, "request needs more than its memory budget");
+        ws.cand.reserve(need);
+    }
+    a.spill_cap = (uint32_t) std::min<size_t>(ws.cand.n / 2, 0x7fffffffu);
+    a.spill[0] = ws.cand.p;
+    a.spill[1] = ws.cand.p + a.spill_cap;
+    a.spill_count = ws.scalars.p + 3;
+    a.budget = c->sweep_budget;
 }
 
 static void sweep_end(Workspace& ws) {
@@ -473,8 +612,19 @@ static void sweep_end(Workspace& ws) {
     ws.timing_pending = true;
 }
 
-void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st) {
+void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::function<void()>& rerun) {
+    (void) c;
     RTCD_HIP(hipStreamSynchronize(ws.stream));
+    std::vector<uint32_t> sv;
+    for (int attempt = 0;; ++attempt) {
+        sv = ws.scalars.download(3, ws.stream);
+        if (sv[1] & 2u) throw std::runtime_error("traversal stack overflow (BVH deeper than the stack)");
+        if (!(sv[1] & 4u)) break;
+        if (attempt == 3 || !rerun) throw std::runtime_error("FP64 re-decision queue overflow persisted");
+        ws.unsure.alloc((size_t) sv[2] + sv[2] / 4 + 1024);
+        rerun();
+        RTCD_HIP(hipStreamSynchronize(ws.stream));
+    }
     if (st) {
         if (ws.timing_pending) {
             float ms = 0.f;
@@ -488,6 +638,11 @@ void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st) {
             st->fp64_redecisions = k.fp64;
             st->candidates = k.candidates;
             st->edge_tests = k.edge_tests;
+            st->cert_calls = k.certs;
+            std::vector<uint32_t> sc = ws.scalars.download(3 + kSweepPasses - 1, ws.stream);
+            uint64_t tot = 0;
+            for (int p = 0; p + 1 < kSweepPasses; ++p) tot += sc[3 + p];
+            st->leaf_candidates = tot;
         }
     }
     ws.timing_pending = false;
@@ -514,8 +669,9 @@ void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_tra
     a.seg_wp = ws.seg_wp.p;
     a.n_seg = n_seg;
     a.n_ctrl = 2;
+    sweep_work(c, ws, a, sweep_items(c, (size_t) std::max(n_seg, 0), false), budget);
     const int depth = sc.edge_bvh.depth;
-    if (n_seg > 0) launch_sweep(a, 1, depth + 2 <= 32 ? 32 : 64, ws.counting, s, ws.c32.p, nullptr);
+    if (n_seg > 0) launch_sweep(a, 1, depth + 2 <= 32 ? 32 : 64, ws.counting, s);
     sweep_end(ws);
 }
 
@@ -550,8 +706,10 @@ void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_
     a.W = ws.W64.p;
     a.ctrl64 = ws.P64.p;
     a.n_seg = (int) n_traj * npc;
+    a.P32 = ws.P32.p;
+    sweep_work(c, ws, a, sweep_items(c, (size_t) a.n_seg, true), budget);
     const int depth = sc.edge_bvh.depth;
-    if (a.n_seg > 0) launch_sweep(a, 2, depth + 2 <= 32 ? 32 : 64, ws.counting, s, ws.c32.p, ws.P32.p);
+    if (a.n_seg > 0) launch_sweep(a, 2, depth + 2 <= 32 ? 32 : 64, ws.counting, s);
     sweep_end(ws);
 }
 
@@ -608,6 +766,15 @@ rtcd_status rtcd_check_linear_device(rtcd_ctx* ctx, uint32_t scene_id, const flo
         on_main(ctx, stream, [&](Workspace& ws) {
             enqueue_linear(ctx, ws, *sc, d_traj, d_offsets, n_traj, n_wp_total, d_out, d_out_seg, 0);
         });
+        hipStream_t st = stream ? (hipStream_t) stream : ctx->main.stream;
+        auto scp = sc;
+        ctx->pending = {};
+        ctx->pending.active = true;
+        ctx->pending.stream = st;
+        ctx->pending.scene = sc;
+        ctx->pending.sweep_rerun = [ctx, scp, d_traj, d_offsets, n_traj, n_wp_total, d_out, d_out_seg]() {
+            enqueue_linear(ctx, ctx->main, *scp, d_traj, d_offsets, n_traj, n_wp_total, d_out, d_out_seg, 0);
+        };
         return RTCD_OK;
     });
 }
@@ -634,11 +801,16 @@ rtcd_status rtcd_check_linear(rtcd_ctx* ctx, uint32_t scene_id, const float* tra
         b.h_offsets.upload(offsets, n_traj + 1, s);
         b.h_out.reserve(n_traj);
         b.h_seg.reserve(std::max<uint32_t>(n_seg, 1));
-        enqueue_linear(ctx, b, *sc, b.h_poses.p, b.h_offsets.p, n_traj, nw, b.h_out.p, out_seg ? b.h_seg.p : nullptr, 0);
+        auto run = [&]() {
+            enqueue_linear(ctx, b, *sc, b.h_poses.p, b.h_offsets.p, n_traj, nw, b.h_out.p, out_seg ? b.h_seg.p : nullptr,
+                           0);
+        };
+        run();
+        finish_sweep(ctx, b, &ctx->last, run);
         RTCD_HIP(hipMemcpyAsync(out, b.h_out.p, sizeof(uint32_t) * n_traj, hipMemcpyDeviceToHost, s));
         if (out_seg && n_seg)
             RTCD_HIP(hipMemcpyAsync(out_seg, b.h_seg.p, sizeof(uint32_t) * n_seg, hipMemcpyDeviceToHost, s));
-        finish_sweep(ctx, b, &ctx->last);
+        RTCD_HIP(hipStreamSynchronize(s));
         return RTCD_OK;
     });
 }
@@ -658,6 +830,18 @@ rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const 
         on_main(ctx, stream, [&](Workspace& ws) {
             enqueue_quadratic(ctx, ws, *sc, d_wp, n_traj, n_wp, n_ctrl, W, d_out, d_out_seg, 0);
         });
+        hipStream_t st = stream ? (hipStream_t) stream : ctx->main.stream;
+        auto scp = sc;
+        std::vector<double> Wc;
+        if (W) Wc.assign(W, W + (size_t) n_ctrl * n_wp);
+        ctx->pending = {};
+        ctx->pending.active = true;
+        ctx->pending.stream = st;
+        ctx->pending.scene = sc;
+        ctx->pending.sweep_rerun = [ctx, scp, d_wp, n_traj, n_wp, n_ctrl, Wc, d_out, d_out_seg]() {
+            enqueue_quadratic(ctx, ctx->main, *scp, d_wp, n_traj, n_wp, n_ctrl, Wc.empty() ? nullptr : Wc.data(), d_out,
+                              d_out_seg, 0);
+        };
         return RTCD_OK;
     });
 }
@@ -681,10 +865,15 @@ rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* 
         b.h_poses.upload(wp, (size_t) n_traj * n_wp * ctx->robot.dof, s);
         b.h_out.reserve(n_traj);
         b.h_seg.reserve(n_traj * npc);
-        enqueue_quadratic(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, out_seg ? b.h_seg.p : nullptr, 0);
+        auto run = [&]() {
+            enqueue_quadratic(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, out_seg ? b.h_seg.p : nullptr,
+                              0);
+        };
+        run();
+        finish_sweep(ctx, b, &ctx->last, run);
         RTCD_HIP(hipMemcpyAsync(out, b.h_out.p, sizeof(uint32_t) * n_traj, hipMemcpyDeviceToHost, s));
         if (out_seg) RTCD_HIP(hipMemcpyAsync(out_seg, b.h_seg.p, sizeof(uint32_t) * n_traj * npc, hipMemcpyDeviceToHost, s));
-        finish_sweep(ctx, b, &ctx->last);
+        RTCD_HIP(hipStreamSynchronize(s));
         return RTCD_OK;
     });
 }

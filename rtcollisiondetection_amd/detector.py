@@ -117,6 +117,18 @@ class Context:
     def set_counting(self, enable):
         check(lib().rtcd_set_counting(self.h, int(bool(enable))))
 
+    PARAMS = {'sweep_budget': _lib.PARAM_SWEEP_BUDGET, 'frontier_work': _lib.PARAM_FRONTIER_WORK,
+              'frontier_max_tasks': _lib.PARAM_FRONTIER_MAX_TASKS}
+
+    def set_param(self, name, value):
+        """Set a launch-shape parameter (see rtcd_param in include/rtcd.h)."""
+        check(lib().rtcd_set_param(self.h, self.PARAMS[name], int(value)))
+
+    def get_param(self, name):
+        v = C.c_uint64()
+        check(lib().rtcd_get_param(self.h, self.PARAMS[name], C.byref(v)))
+        return v.value
+
     def stats(self):
         st = _lib.Stats()
         check(lib().rtcd_get_stats(self.h, C.byref(st)))
