@@ -71,8 +71,10 @@ typedef struct {
     uint32_t max_depth;      /* deepest BVH involved in the last call */
     uint64_t edge_tests;     /* of prim_tests: tests that read a 32 B edge record (the rest read 48 B) */
     double stage_ms[5];      /* discrete: FK, broad, (unused), (unused), narrow + FP64 re-decision (device ms) */
-    uint64_t leaf_candidates; /* (pose, link, scene leaf) or (tube, scene leaf) records emitted by the broad phase */
+    uint64_t leaf_candidates; /* counting launches: discrete (pose, link, scene leaf) records of the broad phase;
+                                 sweeps: subtrees handed off between passes */
     uint64_t cert_calls;     /* sweeps: FP32 capsule certificates evaluated (quadratic: one per refined sub-piece) */
+    uint64_t broad_node_fetches; /* discrete: scene BVH nodes visited by the broad phase */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */
@@ -164,7 +166,10 @@ typedef enum {
     RTCD_PARAM_SWEEP_BUDGET = 1,       /* sweep: work units (node visits + certificates) a task spends
                                           before handing its pending subtrees to the next pass (>= 1) */
     RTCD_PARAM_FRONTIER_WORK = 2,      /* discrete: target number of (task, frontier entry) threads */
-    RTCD_PARAM_FRONTIER_MAX_TASKS = 3  /* discrete: batches with fewer (pose, link) tasks use the split */
+    RTCD_PARAM_FRONTIER_MAX_TASKS = 3, /* discrete: batches with fewer (pose, link) tasks use the split */
+    RTCD_PARAM_BROAD_BUDGET = 4,       /* discrete: scene-BVH nodes a broad-phase walk visits before
+                                          handing its pending subtrees to the next pass (>= 1) */
+    RTCD_PARAM_BROAD_BUDGET_SMALL = 5  /* the same for batches below RTCD_PARAM_FRONTIER_MAX_TASKS tasks */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value);

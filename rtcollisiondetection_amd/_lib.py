@@ -21,7 +21,8 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
            'rtcd_check_linear', 'rtcd_check_linear_device', 'rtcd_check_quadratic', 'rtcd_check_quadratic_device',
            'rtcd_quadratic_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
            'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param']
-PARAM_SWEEP_BUDGET, PARAM_FRONTIER_WORK, PARAM_FRONTIER_MAX_TASKS = 1, 2, 3
+PARAM_SWEEP_BUDGET, PARAM_FRONTIER_WORK, PARAM_FRONTIER_MAX_TASKS, PARAM_BROAD_BUDGET, PARAM_BROAD_BUDGET_SMALL = \
+    1, 2, 3, 4, 5
 REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC = 1, 2, 3
 
 
@@ -47,7 +48,7 @@ class Stats(C.Structure):
     _fields_ = [('node_fetches', C.c_uint64), ('prim_tests', C.c_uint64), ('fp64_redecisions', C.c_uint64),
                 ('candidates', C.c_uint64), ('last_ms', C.c_double), ('max_stack', C.c_uint32),
                 ('max_depth', C.c_uint32), ('edge_tests', C.c_uint64), ('stage_ms', C.c_double * 5),
-                ('leaf_candidates', C.c_uint64), ('cert_calls', C.c_uint64)]
+                ('leaf_candidates', C.c_uint64), ('cert_calls', C.c_uint64), ('broad_node_fetches', C.c_uint64)]
 
 
 class Request(C.Structure):

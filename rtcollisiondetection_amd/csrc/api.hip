@@ -69,7 +69,7 @@ size_t discrete_bytes(const rtcd_ctx* c, uint32_t n) {
     const size_t NL = (size_t) c->robot.n_links, tasks = (NL - (c->cfg.skip_base_link ? 1 : 0)) * n;
     const size_t cap = std::max<size_t>((size_t) 1 << 20, tasks * 64);
     const size_t unsure = std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
-    return n * NL * (12 * 8 + 3 * 16 + 2 * 16) + cap * 8 +
+    return n * NL * (12 * 8 + 3 * 16 + 2 * 16) + cap * 8 + ((size_t) 2 << 20) * 8 +
            unsure * 16 + (size_t) n * (4 * c->robot.dof + 4) + 64;
 }
 
@@ -110,13 +110,14 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     ws.Ti.reserve(npl * 3);
     ws.wbox.reserve(npl * 2);
     (void) items;
-    ws.scalars.reserve(8);
+    ws.scalars.reserve(kScalars);
     ws.stats.reserve(1);
+    if (ws.spill.n < ((size_t) 2 << 20)) ws.spill.reserve((size_t) 2 << 20);
 
-    ws.seg_count.reserve(kCandSegments);
+    ws.seg_count.reserve((size_t) kCandSegments * kSegStride);
     fill_u32(d_out, n, 0u, s);
-    fill_u32(ws.scalars.p, 8, 0u, s);
-    fill_u32(ws.seg_count.p, kCandSegments, 0u, s);
+    fill_u32(ws.scalars.p, kScalars, 0u, s);
+    fill_u32(ws.seg_count.p, (size_t) kCandSegments * kSegStride, 0u, s);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, s);
     RTCD_HIP(hipEventRecord(ws.ev[0], s));
     launch_fk(c->d_robot.p, c->robot.dof, NL, d_poses, (int) n, ws.T64.p, ws.Ti.p, ws.wbox.p, s);
@@ -146,9 +147,14 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.unsure = ws.unsure.p;
     a.unsure_cap = (uint32_t) std::min<size_t>(ws.unsure.n, 0xffffffffu);
     a.unsure_count = ws.scalars.p + 2;
+    a.spill_cap = (uint32_t) std::min<size_t>(ws.spill.n / 2, 0x7fffffffu);
+    a.spill[0] = ws.spill.p;
+    a.spill[1] = ws.spill.p + a.spill_cap;
+    a.spill_count = ws.scalars.p + 8;
+    a.budget = tasks < c->frontier_max_tasks ? c->broad_budget_small : c->broad_budget;
     const int broad_stack = stack_for(sc.tri_bvh.depth);
     RTCD_HIP(hipEventRecord(ws.ev[1], s));
-    if (a.F > 0) launch_broad(a, broad_stack, s);
+    if (a.F > 0) launch_broad(a, broad_stack, ws.counting, s);
     RTCD_HIP(hipEventRecord(ws.ev[2], s));
     RTCD_HIP(hipEventRecord(ws.ev[3], s));
     RTCD_HIP(hipEventRecord(ws.ev[4], s));
@@ -162,15 +168,18 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
     const hipStream_t s = ws.stream;
     for (int attempt = 0; attempt < 4; ++attempt) {
         std::vector<uint32_t> sv = ws.scalars.download(6, s);
-        {
+        sv.push_back(0);
+        if ((sv[1] & 1u) || ws.counting) {
             // total leaf candidates and the fullest segment (sizes the re-run on overflow)
-            std::vector<uint32_t> segc = ws.seg_count.download(kCandSegments, s);
+            std::vector<uint32_t> segc = ws.seg_count.download((size_t) kCandSegments * kSegStride, s);
             uint64_t tot = 0;
             uint32_t mx = 0;
-            for (uint32_t v : segc) { tot += v; mx = std::max(mx, v); }
+            for (int b = 0; b < kCandSegments; ++b) {
+                tot += segc[(size_t) b * kSegStride];
+                mx = std::max(mx, segc[(size_t) b * kSegStride]);
+            }
             sv[0] = (uint32_t) std::min<uint64_t>(tot, 0xffffffffu);
-            if (mx > ws.cand_cap / kCandSegments) sv[1] |= 1u;
-            sv.push_back(mx);
+            sv[6] = mx;
         }
         if (sv[1] & 2u) throw std::runtime_error("traversal stack overflow (BVH deeper than the stack)");
         if (!(sv[1] & 5u)) {
@@ -186,7 +195,7 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                     }
                 }
                 st->candidates = (uint64_t) sv[4] | ((uint64_t) sv[5] << 32);  // triangles in emitted leaves
-                st->leaf_candidates = sv[0];
+                st->leaf_candidates = ws.counting ? sv[0] : 0;
                 st->max_depth = (uint32_t) std::max(c->robot_depth, sc.tri_bvh.depth);
                 st->max_stack = (uint32_t) stack_for(sc.tri_bvh.depth);
                 if (ws.counting) {
@@ -195,6 +204,7 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                     st->prim_tests = k.prim_tests;
                     st->fp64_redecisions = k.fp64;
                     st->edge_tests = k.edge_tests;
+                    st->broad_node_fetches = k.broad_nodes;
                 }
                 st->cert_calls = 0;
             }
@@ -602,6 +612,11 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
     case RTCD_PARAM_FRONTIER_MAX_TASKS:
         ctx->frontier_max_tasks = value;
         return RTCD_OK;
+    case RTCD_PARAM_BROAD_BUDGET:
+    case RTCD_PARAM_BROAD_BUDGET_SMALL:
+        if (value < 1 || value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "broad budget must be 1..2^32-1");
+        (param == RTCD_PARAM_BROAD_BUDGET ? ctx->broad_budget : ctx->broad_budget_small) = (uint32_t) value;
+        return RTCD_OK;
     default:
         return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
@@ -614,6 +629,8 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_SWEEP_BUDGET: *value = ctx->sweep_budget; return RTCD_OK;
     case RTCD_PARAM_FRONTIER_WORK: *value = ctx->frontier_work; return RTCD_OK;
     case RTCD_PARAM_FRONTIER_MAX_TASKS: *value = ctx->frontier_max_tasks; return RTCD_OK;
+    case RTCD_PARAM_BROAD_BUDGET: *value = ctx->broad_budget; return RTCD_OK;
+    case RTCD_PARAM_BROAD_BUDGET_SMALL: *value = ctx->broad_budget_small; return RTCD_OK;
     default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
 }

@@ -28,7 +28,7 @@ __device__ __forceinline__ uint32_t pack_leaf(int start, int n) { return (uint32
 __device__ __forceinline__ uint32_t leaf_start(uint32_t y) { return y & 0x7ffffffu; }
 __device__ __forceinline__ uint32_t leaf_last(uint32_t y) { return y >> 27; }
 
-// Exclusive prefix of the segment counters (each clamped to seg_cap) into seg_pre[0..kCandSegments],
+// Exclusive prefix of the segment counters (seg_count[b * kSegStride], each clamped to seg_cap) into seg_pre[0..kCandSegments],
 // seg_pre[kCandSegments] = total.  Called by all BS threads of a block; BS divides kCandSegments.
 template <int BS>
 __device__ __forceinline__ void seg_prefix(const uint32_t* __restrict__ seg_count, uint32_t seg_cap, uint32_t* seg_pre,
@@ -36,7 +36,7 @@ __device__ __forceinline__ void seg_prefix(const uint32_t* __restrict__ seg_coun
     constexpr int per = kCandSegments / BS;
     uint32_t loc[per], acc = 0;
     for (int k = 0; k < per; ++k) {
-        loc[k] = min(seg_count[threadIdx.x * per + k], seg_cap);
+        loc[k] = min(seg_count[(threadIdx.x * per + k) * kSegStride], seg_cap);
         acc += loc[k];
     }
     part[threadIdx.x] = acc;

@@ -86,11 +86,14 @@ struct Workspace {
     // discrete
     DevBuf<double> T64;
     DevBuf<float4> Ti, wbox;
-    DevBuf<uint32_t> counts, offsets, scan_tmp, scalars;  // scalars: [0] candidates, [1] overflow bits, [2] queued FP64 tests
+    // scalars (kScalars words): [0] leaf records (counting), [1] overflow bits, [2] queued FP64 tests,
+    // [4..5] discrete triangles in emitted leaves (u64), [8..] hand-offs per pass (broad or sweep)
+    DevBuf<uint32_t> counts, offsets, scan_tmp, scalars;
     DevBuf<uint2> cand;
     uint32_t cand_cap = 0;
-    DevBuf<uint32_t> seg_count;  // [kCandSegments]
+    DevBuf<uint32_t> seg_count;  // [kCandSegments * kSegStride]
     DevBuf<uint4> unsure;  // FP64 re-decision queue
+    DevBuf<uint2> spill;   // hand-off queues of the broad phase (two halves)
     // host staging
     DevBuf<float> h_poses;
     DevBuf<uint32_t> h_offsets, h_out, h_seg;
@@ -109,7 +112,7 @@ struct Workspace {
 
     size_t bytes() const {
         return T64.bytes() + Ti.bytes() + wbox.bytes() + counts.bytes() + offsets.bytes() + scan_tmp.bytes() +
-               scalars.bytes() + cand.bytes() + unsure.bytes() + h_poses.bytes() + h_offsets.bytes() + h_out.bytes() + h_seg.bytes() +
+               scalars.bytes() + cand.bytes() + unsure.bytes() + spill.bytes() + h_poses.bytes() + h_offsets.bytes() + h_out.bytes() + h_seg.bytes() +
                c64.bytes() + c32.bytes() + seg_traj.bytes() + seg_wp.bytes() + P64.bytes() + W64.bytes() + P32.bytes();
     }
     void init(hipStream_t s, bool own) {
@@ -146,6 +149,8 @@ struct rtcd_ctx {
     uint32_t sweep_budget = 32;                 // work units per sweep task before it hands off its subtrees
     uint64_t frontier_work = (uint64_t) 1 << 23;     // discrete: target (task, frontier entry) threads
     uint64_t frontier_max_tasks = (uint64_t) 1 << 17; // discrete: split batches with fewer tasks
+    uint32_t broad_budget = 64;                 // discrete: broad node visits before a walk hands off
+    uint32_t broad_budget_small = 16;           // the same for batches below frontier_max_tasks tasks
     int n_spheres = 0;
     int robot_depth = 1;
     // scenes (shared so that in-flight served requests keep a scene alive)

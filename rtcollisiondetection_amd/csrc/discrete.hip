@@ -98,10 +98,14 @@ __device__ __forceinline__ bool broad_test(float lx, float ly, float lz, float h
            fabsf(c.z - lc.z) <= hz2 + lh.z + pad;
 }
 
-// One pass: every (pose, link) task, split over F frontier entries, walks the scene triangle LBVH
-// and appends each leaf that passes both box tests to the candidate list.
-template <int STACK>
-__global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a) {
+// Broad phase, pass `pass` of kBroadPasses.  Pass 0: every (pose, link) task, split over F frontier
+// entries, walks the scene triangle LBVH and appends each leaf that passes both box tests to the
+// candidate list.  A walk that has visited a.budget nodes in a pass that may hand off work appends
+// its pending subtrees (the next node and its stack) as (pose * NL + link, node) to the queue of the
+// next pass and ends; passes 1.. run those subtrees, so that a link deep inside dense geometry is
+// walked by many threads instead of holding one wave.  The last pass never hands off.
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
     __shared__ int stk[STACK * kBS];
     __shared__ uint32_t wcnt[kBS / 64];
     __shared__ uint32_t tris_block;
@@ -110,82 +114,123 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a) {
     if (threadIdx.x == 0) tris_block = 0;
     __syncthreads();
     uint32_t* my_wcnt = wcnt + (threadIdx.x >> 6);
-    // Block b owns candidate segment b and walks one contiguous chunk of the (task, frontier entry)
-    // items, so the segments in order keep the link-major task order (poses flagged through an
-    // earlier link skip the later ones in the narrow phase) and each counter is shared by one block.
-    uint32_t* seg_count = a.seg_count + blockIdx.x;
+    // Block b owns candidate segment b.  In pass 0 it walks one contiguous chunk of the (task,
+    // frontier entry) items, so the segments in order keep the link-major task order (poses flagged
+    // through an earlier link skip the later ones in the narrow phase); later passes stride.
+    uint32_t* seg_count = a.seg_count + (size_t) blockIdx.x * kSegStride;
     uint2* seg_cand = a.cand + (size_t) blockIdx.x * a.seg_cap;
-    const size_t n_items = (size_t) a.n_active * a.n_poses * a.F;
-    const size_t chunk = ((n_items + gridDim.x - 1) / gridDim.x + kBS - 1) / kBS * kBS;
-    const size_t beg = (size_t) blockIdx.x * chunk, end = min(beg + chunk, n_items);
-    uint32_t ntris = 0;
-    for (size_t id = beg + threadIdx.x; id < end; id += kBS) {
-        // item = (task, frontier entry); task = (link-major) (link, pose)
-        const int task = (int) (id / (size_t) a.F), f = (int) (id - (size_t) task * a.F);
-        const int li = task / a.n_poses, p = task - li * a.n_poses, l = a.l0 + li;
-        const size_t pl = (size_t) p * a.n_links + l;
+    const uint2* in_q = pass > 0 ? a.spill[(pass - 1) & 1] : nullptr;
+    uint2* out_q = a.spill[pass & 1];
+    uint32_t* out_cnt = a.spill_count + pass;
+    const bool may_spill = pass < kBroadPasses - 1;
+    size_t beg, end, step;
+    if (pass == 0) {
+        const size_t n_items = (size_t) a.n_active * a.n_poses * a.F;
+        const size_t chunk = ((n_items + gridDim.x - 1) / gridDim.x + kBS - 1) / kBS * kBS;
+        beg = (size_t) blockIdx.x * chunk;
+        end = min(beg + chunk, n_items);
+        step = kBS;
+    } else {
+        beg = (size_t) blockIdx.x * kBS;
+        end = min(a.spill_count[pass - 1], a.spill_cap);
+        step = (size_t) gridDim.x * kBS;
+    }
+    const float pad = a.broad_pad;
+    uint32_t ntris = 0, c_nodes = 0;
+    auto emit = [&](uint32_t key, int start, int n, uint32_t o) {
+        if (o < a.seg_cap) seg_cand[o] = make_uint2(key, pack_leaf(start, n));
+        else atomicOr(a.overflow, 1u);
+        ntris += (uint32_t) n;
+    };
+    for (size_t id = beg + threadIdx.x; id < end; id += step) {
+        uint32_t key;
+        int f = 0, ni = 0;
+        if (pass == 0) {
+            // item = (task, frontier entry); task = (link-major) (link, pose)
+            const int task = (int) (id / (size_t) a.F);
+            f = (int) (id - (size_t) task * a.F);
+            const int li = task / a.n_poses, p = task - li * a.n_poses;
+            key = (uint32_t) ((size_t) p * a.n_links + a.l0 + li);
+        } else {
+            const uint2 r = in_q[id];
+            if (r.x == 0xffffffffu) continue;  // slot of a hand-off that did not fit
+            key = r.x;
+            ni = (int) r.y;
+        }
+        const int l = (int) (key % (uint32_t) a.n_links);
+        const size_t pl = key;
         const float4 wlo = a.wbox[pl * 2], whi = a.wbox[pl * 2 + 1];
         const float4 r0 = a.Ti[pl * 3], r1 = a.Ti[pl * 3 + 1], r2 = a.Ti[pl * 3 + 2];
         const LinkInfo& L = a.robot->link[l];
         const float3 lc = make_float3(L.c[0], L.c[1], L.c[2]);
         const float3 lh =
             make_float3(0.5f * (L.hi[0] - L.lo[0]), 0.5f * (L.hi[1] - L.lo[1]), 0.5f * (L.hi[2] - L.lo[2]));
-        const float pad = a.broad_pad;
-        const uint32_t key = (uint32_t) pl;
-        const FEntry fe = a.front[f];
-        bool go = a.sc.tri_has_root && (L.n_tri > 0 || L.n_edge > 0) &&
-                  broad_test(fe.lo[0], fe.lo[1], fe.lo[2], fe.hi[0], fe.hi[1], fe.hi[2], wlo, whi, r0, r1, r2, lc, lh,
-                             pad);
-        if (go && fe.n > 0) {
-            const uint32_t o = wave_alloc(my_wcnt, seg_count, 1u);
-            if (o < a.seg_cap) seg_cand[o] = make_uint2(key, pack_leaf(fe.c, fe.n));
-            else atomicOr(a.overflow, 1u);
-            ntris += fe.n;
-            go = false;
+        if (pass == 0) {
+            const FEntry fe = a.front[f];
+            const bool go = a.sc.tri_has_root && (L.n_tri > 0 || L.n_edge > 0) &&
+                            broad_test(fe.lo[0], fe.lo[1], fe.lo[2], fe.hi[0], fe.hi[1], fe.hi[2], wlo, whi, r0, r1,
+                                       r2, lc, lh, pad);
+            if (!go) continue;
+            if (fe.n > 0) {
+                emit(key, fe.c, fe.n, wave_alloc(my_wcnt, seg_count, 1u));
+                continue;
+            }
+            ni = fe.c;
         }
-        if (go) {
-            int sp = 0, ni = fe.c;
-            for (;;) {
-                const BNode nd = a.sc.tri_nodes[ni];
-                bool h0 = nd.c0 >= 0 && broad_test(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, wlo, whi, r0,
-                                                   r1, r2, lc, lh, pad);
-                bool h1 = nd.c1 >= 0 && broad_test(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, wlo, whi, r0,
-                                                   r1, r2, lc, lh, pad);
-                const bool e0 = h0 && nd.n0 > 0, e1 = h1 && nd.n1 > 0;
-                if (e0 || e1) {
-                    uint32_t o = wave_alloc(my_wcnt, seg_count, (uint32_t) e0 + (uint32_t) e1);
-                    if (e0) {
-                        if (o < a.seg_cap) seg_cand[o] = make_uint2(key, pack_leaf(nd.c0, nd.n0));
-                        else atomicOr(a.overflow, 1u);
-                        ++o;
-                        ntris += nd.n0;
-                        h0 = false;
-                    }
-                    if (e1) {
-                        if (o < a.seg_cap) seg_cand[o] = make_uint2(key, pack_leaf(nd.c1, nd.n1));
-                        else atomicOr(a.overflow, 1u);
-                        ntris += nd.n1;
-                        h1 = false;
-                    }
+        bool spill_ok = may_spill;
+        uint32_t work = 0;
+        int sp = 0;
+        for (;;) {
+            if (spill_ok && work >= a.budget) {
+                const uint32_t want = (uint32_t) sp + 1u;
+                const uint32_t o = atomicAdd(out_cnt, want);
+                if ((uint64_t) o + want <= a.spill_cap) {
+                    out_q[o] = make_uint2(key, (uint32_t) ni);
+                    for (int j = 0; j < sp; ++j) out_q[o + 1 + j] = make_uint2(key, (uint32_t) st[j * kBS]);
+                    break;
                 }
-                if (h0 && h1) {
-                    if (sp >= STACK) { atomicOr(a.overflow, 2u); break; }
-                    st[sp * kBS] = nd.c1;
-                    ++sp;
-                    ni = nd.c0;
-                } else if (h0) {
-                    ni = nd.c0;
-                } else if (h1) {
-                    ni = nd.c1;
-                } else {
-                    if (sp == 0) break;
-                    --sp;
-                    ni = st[sp * kBS];
+                // queue full: mark the reserved slots empty and finish this subtree here
+                for (uint64_t j = o; j < a.spill_cap && j < (uint64_t) o + want; ++j)
+                    out_q[j] = make_uint2(0xffffffffu, 0u);
+                spill_ok = false;
+            }
+            const BNode nd = a.sc.tri_nodes[ni];
+            ++work;
+            if (COUNT) c_nodes++;
+            bool h0 = nd.c0 >= 0 && broad_test(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, wlo, whi, r0, r1,
+                                               r2, lc, lh, pad);
+            bool h1 = nd.c1 >= 0 && broad_test(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, wlo, whi, r0, r1,
+                                               r2, lc, lh, pad);
+            const bool e0 = h0 && nd.n0 > 0, e1 = h1 && nd.n1 > 0;
+            if (e0 || e1) {
+                uint32_t o = wave_alloc(my_wcnt, seg_count, (uint32_t) e0 + (uint32_t) e1);
+                if (e0) {
+                    emit(key, nd.c0, nd.n0, o++);
+                    h0 = false;
                 }
+                if (e1) {
+                    emit(key, nd.c1, nd.n1, o);
+                    h1 = false;
+                }
+            }
+            if (h0 && h1) {
+                if (sp >= STACK) { atomicOr(a.overflow, 2u); break; }
+                st[sp * kBS] = nd.c1;
+                ++sp;
+                ni = nd.c0;
+            } else if (h0) {
+                ni = nd.c0;
+            } else if (h1) {
+                ni = nd.c1;
+            } else {
+                if (sp == 0) break;
+                --sp;
+                ni = st[sp * kBS];
             }
         }
     }
     if (ntris) atomicAdd(&tris_block, ntris);
+    if (COUNT) atomicAdd(&a.stats->broad_nodes, (unsigned long long) c_nodes);
     __syncthreads();
     if (threadIdx.x == 0 && tris_block) atomicAdd(a.tri_candidates, (unsigned long long) tris_block);
 }
@@ -460,16 +505,22 @@ void launch_fk(const RobotDev* robot, int, int, const float* poses, int n, doubl
 }
 
 template <int STACK>
-static void broad_t(const DiscreteArgs& a, hipStream_t s) {
-    const unsigned nb = (unsigned) std::min<size_t>(blocks_for((size_t) a.n_active * a.n_poses * a.F, kBS), kSeg);
-    if (!nb) return;
-    hipLaunchKernelGGL((k_broad<STACK>), dim3(nb), dim3(kBS), 0, s, a);
-    RTCD_LAUNCH_CHECK();
+static void broad_t(const DiscreteArgs& a, bool count, hipStream_t s) {
+    const unsigned nb0 = (unsigned) std::min<size_t>(blocks_for((size_t) a.n_active * a.n_poses * a.F, kBS), kSeg);
+    if (!nb0) return;
+    // later passes stride over the hand-off queue; they need at most kSeg blocks (one per segment)
+    const unsigned nb1 = std::min<unsigned>((unsigned) device_cus() * 8u, kSeg);
+    for (int pass = 0; pass < kBroadPasses; ++pass) {
+        const unsigned nb = pass == 0 ? nb0 : nb1;
+        if (count) hipLaunchKernelGGL((k_broad<STACK, true>), dim3(nb), dim3(kBS), 0, s, a, pass);
+        else hipLaunchKernelGGL((k_broad<STACK, false>), dim3(nb), dim3(kBS), 0, s, a, pass);
+        RTCD_LAUNCH_CHECK();
+    }
 }
 
-void launch_broad(const DiscreteArgs& a, int stack, hipStream_t s) {
-    if (stack <= 32) broad_t<32>(a, s);
-    else if (stack <= 64) broad_t<64>(a, s);
+void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s) {
+    if (stack <= 32) broad_t<32>(a, count, s);
+    else if (stack <= 64) broad_t<64>(a, count, s);
     else throw std::runtime_error("BVH deeper than 64 levels");
 }
 

@@ -7,6 +7,11 @@ namespace rtcd {
 // The broad phase appends candidates into this many independent segments, one per broad block (at
 // most this many blocks), so that its atomics do not contend on one address.
 constexpr int kCandSegments = 2048;
+// Segment counters are kept one per 128 B line (kSegStride words apart): blocks that append at the
+// same time then never contend for one line's atomics.
+constexpr int kSegStride = 32;
+// Words of the per-workspace scalar block (counters and flags of one launch).
+constexpr int kScalars = 16;
 
 struct RobotBvh {
     const BNode* tri_nodes;
@@ -26,7 +31,7 @@ struct SceneDev {
 };
 
 struct Stats {
-    unsigned long long node_fetches, prim_tests, fp64, candidates, edge_tests, certs;
+    unsigned long long node_fetches, prim_tests, fp64, candidates, edge_tests, certs, broad_nodes;
 };
 
 struct DiscreteArgs {
@@ -40,7 +45,8 @@ struct DiscreteArgs {
     uint2* cand;           // (pose * NL + link, first triangle | (count - 1) << 27) per scene leaf,
                            // in kCandSegments segments of seg_cap records
     uint32_t seg_cap;
-    uint32_t* seg_count;   // [kCandSegments] records appended per segment (may exceed seg_cap: overflow bit 1)
+    uint32_t* seg_count;   // [kCandSegments * kSegStride]: records appended to segment b at b * kSegStride
+                           // (may exceed seg_cap: overflow bit 1)
     unsigned long long* tri_candidates;  // triangles in the emitted leaves
     uint32_t* overflow;
     uint32_t* flags;       // [n] output
@@ -52,13 +58,20 @@ struct DiscreteArgs {
     uint4* unsure;         // FP64 re-decision queue
     uint32_t unsure_cap;
     uint32_t* unsure_count;
+    uint2* spill[2];       // broad hand-off queues of (pose * NL + link, scene node), used in turn
+    uint32_t spill_cap;    // records per queue
+    uint32_t* spill_count; // [kBroadPasses - 1] records handed off by each pass
+    uint32_t budget;       // node visits before a broad walk hands off its pending subtrees
 };
+// Broad-phase passes: the first walks every (pose, link) task, each later one the subtrees handed
+// off by the previous pass; the last never hands off.
+constexpr int kBroadPasses = 4;
 
 // FK for n poses: T64, Ti, wbox for every link (exact FP64 sequence shared with the oracle).
 void launch_fk(const RobotDev* robot, int dof, int n_links, const float* poses, int n, double* T64, float4* Ti,
                float4* wbox, hipStream_t s);
 // Broad phase: one pass appending leaf candidates (wave-aggregated atomics).
-void launch_broad(const DiscreteArgs& a, int stack, hipStream_t s);
+void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
 // Narrow phase over *total candidates (grid-stride, sized for the device): one pass per direction
 // in a.dir, then the FP64 re-decision of the queued tests.  `stack` covers the robot BVH depth.
 void launch_narrow(const DiscreteArgs& a, int stack, bool count, hipStream_t s);

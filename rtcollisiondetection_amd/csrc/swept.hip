@@ -582,9 +582,9 @@ static SweepArgs base_args(rtcd_ctx* c, Workspace& ws, SceneRes& sc, uint32_t* d
 
 static void sweep_begin(Workspace& ws) {
     ws.stats.reserve(1);
-    ws.scalars.reserve(8);
+    ws.scalars.reserve(kScalars);
     if (ws.unsure.n < ((size_t) 1 << 16)) ws.unsure.reserve((size_t) 1 << 16);
-    fill_u32(ws.scalars.p, 8, 0u, ws.stream);
+    fill_u32(ws.scalars.p, kScalars, 0u, ws.stream);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, ws.stream);
     RTCD_HIP(hipEventRecord(ws.ev[0], ws.stream));
 }
@@ -603,7 +603,7 @@ static void sweep_work(rtcd_ctx* c, Workspace& ws, SweepArgs& a, size_t items, s
     a.spill_cap = (uint32_t) std::min<size_t>(ws.cand.n / 2, 0x7fffffffu);
     a.spill[0] = ws.cand.p;
     a.spill[1] = ws.cand.p + a.spill_cap;
-    a.spill_count = ws.scalars.p + 3;
+    a.spill_count = ws.scalars.p + 8;
     a.budget = c->sweep_budget;
 }
 
@@ -639,9 +639,10 @@ void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::functio
             st->candidates = k.candidates;
             st->edge_tests = k.edge_tests;
             st->cert_calls = k.certs;
-            std::vector<uint32_t> sc = ws.scalars.download(3 + kSweepPasses - 1, ws.stream);
+            st->broad_node_fetches = 0;
+            std::vector<uint32_t> sc = ws.scalars.download(8 + kSweepPasses - 1, ws.stream);
             uint64_t tot = 0;
-            for (int p = 0; p + 1 < kSweepPasses; ++p) tot += sc[3 + p];
+            for (int p = 0; p + 1 < kSweepPasses; ++p) tot += sc[8 + p];
             st->leaf_candidates = tot;
         }
     }

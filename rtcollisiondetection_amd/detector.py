@@ -118,7 +118,8 @@ class Context:
         check(lib().rtcd_set_counting(self.h, int(bool(enable))))
 
     PARAMS = {'sweep_budget': _lib.PARAM_SWEEP_BUDGET, 'frontier_work': _lib.PARAM_FRONTIER_WORK,
-              'frontier_max_tasks': _lib.PARAM_FRONTIER_MAX_TASKS}
+              'frontier_max_tasks': _lib.PARAM_FRONTIER_MAX_TASKS, 'broad_budget': _lib.PARAM_BROAD_BUDGET,
+              'broad_budget_small': _lib.PARAM_BROAD_BUDGET_SMALL}
 
     def set_param(self, name, value):
         """Set a launch-shape parameter (see rtcd_param in include/rtcd.h)."""
