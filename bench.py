@@ -80,6 +80,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-extra', action='store_true')
     ap.add_argument('--poses', type=int, default=65536)
+    ap.add_argument('--copy-calibration', action='store_true',
+                    help='only run a plain 256 MiB device-to-device copy 5 times (reference for HBM counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build measures one GPU only; the path shards by query range '
@@ -87,6 +89,14 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log('[bench] copied 256 MiB x 5')
+        return
     from rtcollisiondetection_amd import robots, samplers, scenes
     from rtcollisiondetection_amd.detector import Context
 

@@ -169,7 +169,10 @@ typedef enum {
     RTCD_PARAM_FRONTIER_MAX_TASKS = 3, /* discrete: batches with fewer (pose, link) tasks use the split */
     RTCD_PARAM_BROAD_BUDGET = 4,       /* discrete: scene-BVH nodes a broad-phase walk visits before
                                           handing its pending subtrees to the next pass (>= 1) */
-    RTCD_PARAM_BROAD_BUDGET_SMALL = 5  /* the same for batches below RTCD_PARAM_FRONTIER_MAX_TASKS tasks */
+    RTCD_PARAM_BROAD_BUDGET_SMALL = 5, /* the same for batches below RTCD_PARAM_FRONTIER_MAX_TASKS tasks */
+    RTCD_PARAM_BROAD_BUDGET_NEXT = 6,  /* the same in every broad pass after the first */
+    RTCD_PARAM_BROAD_PASSES = 7,       /* broad-phase passes, 1..16; the last never hands off */
+    RTCD_PARAM_SWEEP_PASSES = 8        /* sweep passes, 1..16; the last never hands off */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value);

@@ -21,8 +21,8 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
            'rtcd_check_linear', 'rtcd_check_linear_device', 'rtcd_check_quadratic', 'rtcd_check_quadratic_device',
            'rtcd_quadratic_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
            'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param']
-PARAM_SWEEP_BUDGET, PARAM_FRONTIER_WORK, PARAM_FRONTIER_MAX_TASKS, PARAM_BROAD_BUDGET, PARAM_BROAD_BUDGET_SMALL = \
-    1, 2, 3, 4, 5
+PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad_budget': 4,
+          'broad_budget_small': 5, 'broad_budget_next': 6, 'broad_passes': 7, 'sweep_passes': 8}
 REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC = 1, 2, 3
 
 

@@ -152,6 +152,8 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.spill[1] = ws.spill.p + a.spill_cap;
     a.spill_count = ws.scalars.p + 8;
     a.budget = tasks < c->frontier_max_tasks ? c->broad_budget_small : c->broad_budget;
+    a.budget_next = c->broad_budget_next;
+    a.n_passes = (int) c->broad_passes;
     const int broad_stack = stack_for(sc.tri_bvh.depth);
     RTCD_HIP(hipEventRecord(ws.ev[1], s));
     if (a.F > 0) launch_broad(a, broad_stack, ws.counting, s);
@@ -614,8 +616,15 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
         return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET:
     case RTCD_PARAM_BROAD_BUDGET_SMALL:
-        if (value < 1 || value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "broad budget must be 1..2^32-1");
-        (param == RTCD_PARAM_BROAD_BUDGET ? ctx->broad_budget : ctx->broad_budget_small) = (uint32_t) value;
+    case RTCD_PARAM_BROAD_BUDGET_NEXT:
+        if (value < 1 || value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "budget must be 1..2^32-1");
+        (param == RTCD_PARAM_BROAD_BUDGET ? ctx->broad_budget
+         : param == RTCD_PARAM_BROAD_BUDGET_SMALL ? ctx->broad_budget_small : ctx->broad_budget_next) = (uint32_t) value;
+        return RTCD_OK;
+    case RTCD_PARAM_BROAD_PASSES:
+    case RTCD_PARAM_SWEEP_PASSES:
+        if (value < 1 || value > (uint64_t) kMaxPasses) return fail(RTCD_ERR_INVALID_ARG, "passes must be 1..16");
+        (param == RTCD_PARAM_BROAD_PASSES ? ctx->broad_passes : ctx->sweep_passes) = (uint32_t) value;
         return RTCD_OK;
     default:
         return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
@@ -631,6 +640,9 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_FRONTIER_MAX_TASKS: *value = ctx->frontier_max_tasks; return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET: *value = ctx->broad_budget; return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET_SMALL: *value = ctx->broad_budget_small; return RTCD_OK;
+    case RTCD_PARAM_BROAD_BUDGET_NEXT: *value = ctx->broad_budget_next; return RTCD_OK;
+    case RTCD_PARAM_BROAD_PASSES: *value = ctx->broad_passes; return RTCD_OK;
+    case RTCD_PARAM_SWEEP_PASSES: *value = ctx->sweep_passes; return RTCD_OK;
     default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
 }

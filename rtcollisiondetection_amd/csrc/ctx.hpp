@@ -146,11 +146,14 @@ struct rtcd_ctx {
     DevBuf<int32_t> r_sphere_link;
     DevBuf<float> sphere_radius;
     // launch-shape parameters (rtcd_set_param)
-    uint32_t sweep_budget = 32;                 // work units per sweep task before it hands off its subtrees
+    uint32_t sweep_budget = 16;                 // work units per sweep task before it hands off its subtrees
     uint64_t frontier_work = (uint64_t) 1 << 23;     // discrete: target (task, frontier entry) threads
     uint64_t frontier_max_tasks = (uint64_t) 1 << 17; // discrete: split batches with fewer tasks
     uint32_t broad_budget = 64;                 // discrete: broad node visits before a walk hands off
     uint32_t broad_budget_small = 16;           // the same for batches below frontier_max_tasks tasks
+    uint32_t broad_budget_next = 16;            // the same in the passes after the first
+    uint32_t broad_passes = 12;                 // broad hand-off passes (the last never hands off)
+    uint32_t sweep_passes = 8;                  // sweep hand-off passes
     int n_spheres = 0;
     int robot_depth = 1;
     // scenes (shared so that in-flight served requests keep a scene alive)

@@ -98,7 +98,7 @@ __device__ __forceinline__ bool broad_test(float lx, float ly, float lz, float h
            fabsf(c.z - lc.z) <= hz2 + lh.z + pad;
 }
 
-// Broad phase, pass `pass` of kBroadPasses.  Pass 0: every (pose, link) task, split over F frontier
+// Broad phase, pass `pass` of a.n_passes.  Pass 0: every (pose, link) task, split over F frontier
 // entries, walks the scene triangle LBVH and appends each leaf that passes both box tests to the
 // candidate list.  A walk that has visited a.budget nodes in a pass that may hand off work appends
 // its pending subtrees (the next node and its stack) as (pose * NL + link, node) to the queue of the
@@ -122,7 +122,8 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
     const uint2* in_q = pass > 0 ? a.spill[(pass - 1) & 1] : nullptr;
     uint2* out_q = a.spill[pass & 1];
     uint32_t* out_cnt = a.spill_count + pass;
-    const bool may_spill = pass < kBroadPasses - 1;
+    const bool may_spill = pass < a.n_passes - 1;
+    const uint32_t budget = pass == 0 ? a.budget : a.budget_next;
     size_t beg, end, step;
     if (pass == 0) {
         const size_t n_items = (size_t) a.n_active * a.n_poses * a.F;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
         uint32_t work = 0;
         int sp = 0;
         for (;;) {
-            if (spill_ok && work >= a.budget) {
+            if (spill_ok && work >= budget) {
                 const uint32_t want = (uint32_t) sp + 1u;
                 const uint32_t o = atomicAdd(out_cnt, want);
                 if ((uint64_t) o + want <= a.spill_cap) {
@@ -510,7 +511,7 @@ static void broad_t(const DiscreteArgs& a, bool count, hipStream_t s) {
     if (!nb0) return;
     // later passes stride over the hand-off queue; they need at most kSeg blocks (one per segment)
     const unsigned nb1 = std::min<unsigned>((unsigned) device_cus() * 8u, kSeg);
-    for (int pass = 0; pass < kBroadPasses; ++pass) {
+    for (int pass = 0; pass < a.n_passes; ++pass) {
         const unsigned nb = pass == 0 ? nb0 : nb1;
         if (count) hipLaunchKernelGGL((k_broad<STACK, true>), dim3(nb), dim3(kBS), 0, s, a, pass);
         else hipLaunchKernelGGL((k_broad<STACK, false>), dim3(nb), dim3(kBS), 0, s, a, pass);

@@ -11,7 +11,7 @@ constexpr int kCandSegments = 2048;
 // same time then never contend for one line's atomics.
 constexpr int kSegStride = 32;
 // Words of the per-workspace scalar block (counters and flags of one launch).
-constexpr int kScalars = 16;
+constexpr int kScalars = 32;
 
 struct RobotBvh {
     const BNode* tri_nodes;
@@ -60,12 +60,14 @@ struct DiscreteArgs {
     uint32_t* unsure_count;
     uint2* spill[2];       // broad hand-off queues of (pose * NL + link, scene node), used in turn
     uint32_t spill_cap;    // records per queue
-    uint32_t* spill_count; // [kBroadPasses - 1] records handed off by each pass
-    uint32_t budget;       // node visits before a broad walk hands off its pending subtrees
+    uint32_t* spill_count; // [n_passes - 1] records handed off by each pass
+    uint32_t budget;       // node visits before a pass-0 walk hands off its pending subtrees
+    uint32_t budget_next;  // the same in later passes
+    int n_passes;          // broad passes (<= kMaxPasses); the last never hands off
 };
-// Broad-phase passes: the first walks every (pose, link) task, each later one the subtrees handed
-// off by the previous pass; the last never hands off.
-constexpr int kBroadPasses = 4;
+// Most hand-off passes of one broad phase or sweep: the first walks every task, each later one the
+// subtrees handed off by the previous pass.
+constexpr int kMaxPasses = 16;
 
 // FK for n poses: T64, Ti, wbox for every link (exact FP64 sequence shared with the oracle).
 void launch_fk(const RobotDev* robot, int dof, int n_links, const float* poses, int n, double* T64, float4* Ti,
@@ -100,18 +102,16 @@ struct SweepArgs {
     uint32_t n_items;          // n_seg * (1 or the quadratic sub-pieces per piece) * S
     uint2* spill[2];           // hand-off queues of (item, edge-BVH node), used by the passes in turn
     uint32_t spill_cap;        // records per queue
-    uint32_t* spill_count;     // [kSweepPasses - 1] records handed off by each pass
+    uint32_t* spill_count;     // [n_passes - 1] records handed off by each pass
     uint32_t budget;           // work units (node visits + certificates) before a task hands off
+    int n_passes;              // sweep passes (<= kMaxPasses); the last never hands off
     uint4* unsure;             // FP64 re-decision queue: (segment or piece, sphere, scene edge, 0)
     uint32_t unsure_cap;
     uint32_t* unsure_count;
     uint32_t* overflow;        // bit 2: stack overflow, 4: FP64 queue overflow
 };
-// Sweep passes: the first starts one task per item, each later one runs the subtrees handed off by
-// the previous pass; the last never hands off.
-constexpr int kSweepPasses = 4;
 // Tubes of degree 1 (capsules between waypoints) or 2 (quadratic B-spline pieces) vs scene edges:
-// kSweepPasses any-hit passes with FP32 certificates, then the FP64 re-decision of the queue.
+// n_passes any-hit passes with FP32 certificates, then the FP64 re-decision of the queue.
 void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s);
 // Quadratic pieces are split into this many sub-pieces (work items) each.
 constexpr int kSweepSub = 4;

@@ -294,7 +294,7 @@ __global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, int pass) {
     const uint2* in_q = pass > 0 ? a.spill[(pass - 1) & 1] : nullptr;
     uint2* out_q = a.spill[pass & 1];
     uint32_t* out_cnt = a.spill_count + pass;
-    const bool may_spill = pass < kSweepPasses - 1;
+    const bool may_spill = pass < a.n_passes - 1;
     const uint32_t n_tasks = pass == 0 ? a.n_items : min(a.spill_count[pass - 1], a.spill_cap);
     const uint32_t stride = gridDim.x * kBS;
     uint32_t c_nodes = 0, c_prims = 0, c_fp64 = 0, c_items = 0, c_certs = 0;
@@ -455,7 +455,7 @@ int device_cus() {
 template <int DEG, int STACK>
 void sweep_t(const SweepArgs& a, bool count, hipStream_t s) {
     const unsigned cap_blocks = (unsigned) device_cus() * 8u;
-    for (int pass = 0; pass < kSweepPasses; ++pass) {
+    for (int pass = 0; pass < a.n_passes; ++pass) {
         const unsigned nb = pass == 0 ? std::min<unsigned>(blocks_for(a.n_items, kBS), cap_blocks) : cap_blocks;
         if (count) hipLaunchKernelGGL((k_sweep<DEG, STACK, true>), dim3(nb), dim3(kBS), 0, s, a, pass);
         else hipLaunchKernelGGL((k_sweep<DEG, STACK, false>), dim3(nb), dim3(kBS), 0, s, a, pass);
@@ -605,6 +605,7 @@ static void sweep_work(rtcd_ctx* c, Workspace& ws, SweepArgs& a, size_t items, s
     a.spill[1] = ws.cand.p + a.spill_cap;
     a.spill_count = ws.scalars.p + 8;
     a.budget = c->sweep_budget;
+    a.n_passes = (int) c->sweep_passes;
 }
 
 static void sweep_end(Workspace& ws) {
@@ -613,7 +614,6 @@ static void sweep_end(Workspace& ws) {
 }
 
 void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::function<void()>& rerun) {
-    (void) c;
     RTCD_HIP(hipStreamSynchronize(ws.stream));
     std::vector<uint32_t> sv;
     for (int attempt = 0;; ++attempt) {
@@ -640,9 +640,9 @@ void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::functio
             st->edge_tests = k.edge_tests;
             st->cert_calls = k.certs;
             st->broad_node_fetches = 0;
-            std::vector<uint32_t> sc = ws.scalars.download(8 + kSweepPasses - 1, ws.stream);
+            std::vector<uint32_t> sc = ws.scalars.download(kScalars, ws.stream);
             uint64_t tot = 0;
-            for (int p = 0; p + 1 < kSweepPasses; ++p) tot += sc[8 + p];
+            for (int p = 0; p + 1 < (int) c->sweep_passes; ++p) tot += sc[8 + p];
             st->leaf_candidates = tot;
         }
     }

@@ -117,9 +117,7 @@ class Context:
     def set_counting(self, enable):
         check(lib().rtcd_set_counting(self.h, int(bool(enable))))
 
-    PARAMS = {'sweep_budget': _lib.PARAM_SWEEP_BUDGET, 'frontier_work': _lib.PARAM_FRONTIER_WORK,
-              'frontier_max_tasks': _lib.PARAM_FRONTIER_MAX_TASKS, 'broad_budget': _lib.PARAM_BROAD_BUDGET,
-              'broad_budget_small': _lib.PARAM_BROAD_BUDGET_SMALL}
+    PARAMS = _lib.PARAMS
 
     def set_param(self, name, value):
         """Set a launch-shape parameter (see rtcd_param in include/rtcd.h)."""
