@@ -65,7 +65,8 @@ typedef struct {
     uint64_t node_fetches;   /* BVH nodes visited (counting launches only) */
     uint64_t prim_tests;     /* primitive tests (segment-triangle, segment-tube) */
     uint64_t fp64_redecisions; /* tests whose FP32 result fell in the error band */
-    uint64_t candidates;     /* discrete: triangles in the broad-phase leaves; sweeps: (segment or sub-piece, sphere) tubes traversed */
+    uint64_t candidates;     /* discrete: triangles in the leaves emitted by the broad phase;
+                                sweeps: (segment or sub-piece, sphere) tubes traversed */
     double last_ms;          /* device time of the last check call, ms */
     uint32_t max_stack;      /* traversal stack size used by the last call */
     uint32_t max_depth;      /* deepest BVH involved in the last call */

@@ -69,7 +69,7 @@ size_t discrete_bytes(const rtcd_ctx* c, uint32_t n) {
     const size_t NL = (size_t) c->robot.n_links, tasks = (NL - (c->cfg.skip_base_link ? 1 : 0)) * n;
     const size_t cap = std::max<size_t>((size_t) 1 << 20, tasks * 64);
     const size_t unsure = std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
-    return n * NL * (12 * 8 + 3 * 16 + 2 * 16) + cap * 8 + ((size_t) 2 << 20) * 8 +
+    return n * NL * (12 * 8 + 3 * 16 + 2 * 16) + 2 * cap * 8 + ((size_t) 2 << 20) * 8 +
            unsure * 16 + (size_t) n * (4 * c->robot.dof + 4) + 64;
 }
 
@@ -98,6 +98,10 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
         budget_check(c, ws, budget, (need_cap - ws.cand.n) * sizeof(uint2));
         ws.cand.reserve(need_cap);
     }
+    if (ws.flat.n < ws.cand.n) {
+        budget_check(c, ws, budget, (ws.cand.n - ws.flat.n) * sizeof(uint2));
+        ws.flat.alloc(ws.cand.n);
+    }
     ws.cand_cap = (uint32_t) std::min<size_t>(ws.cand.n, 0xffffffffu);
     const size_t need_unsure = std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
     if (ws.unsure.n < need_unsure) {
@@ -115,6 +119,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     if (ws.spill.n < ((size_t) 2 << 20)) ws.spill.reserve((size_t) 2 << 20);
 
     ws.seg_count.reserve((size_t) kCandSegments * kSegStride);
+    ws.seg_pre.reserve(kCandSegments + 1);
     fill_u32(d_out, n, 0u, s);
     fill_u32(ws.scalars.p, kScalars, 0u, s);
     fill_u32(ws.seg_count.p, (size_t) kCandSegments * kSegStride, 0u, s);
@@ -136,6 +141,8 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.cand = ws.cand.p;
     a.seg_cap = (uint32_t) (ws.cand_cap / kCandSegments);
     a.seg_count = ws.seg_count.p;
+    a.seg_pre = ws.seg_pre.p;
+    a.flat = ws.flat.p;
     a.tri_candidates = reinterpret_cast<unsigned long long*>(ws.scalars.p + 4);
     a.overflow = ws.scalars.p + 1;
     a.flags = d_out;

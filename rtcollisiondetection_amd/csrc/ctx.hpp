@@ -92,6 +92,8 @@ struct Workspace {
     DevBuf<uint2> cand;
     uint32_t cand_cap = 0;
     DevBuf<uint32_t> seg_count;  // [kCandSegments * kSegStride]
+    DevBuf<uint32_t> seg_pre;    // [kCandSegments + 1]
+    DevBuf<uint2> flat;          // candidate segments gathered in order (capacity of cand)
     DevBuf<uint4> unsure;  // FP64 re-decision queue
     DevBuf<uint2> spill;   // hand-off queues of the broad phase (two halves)
     // host staging
@@ -112,7 +114,7 @@ struct Workspace {
 
     size_t bytes() const {
         return T64.bytes() + Ti.bytes() + wbox.bytes() + counts.bytes() + offsets.bytes() + scan_tmp.bytes() +
-               scalars.bytes() + cand.bytes() + unsure.bytes() + spill.bytes() + h_poses.bytes() + h_offsets.bytes() + h_out.bytes() + h_seg.bytes() +
+               scalars.bytes() + cand.bytes() + flat.bytes() + unsure.bytes() + spill.bytes() + h_poses.bytes() + h_offsets.bytes() + h_out.bytes() + h_seg.bytes() +
                c64.bytes() + c32.bytes() + seg_traj.bytes() + seg_wp.bytes() + P64.bytes() + W64.bytes() + P32.bytes();
     }
     void init(hipStream_t s, bool own) {

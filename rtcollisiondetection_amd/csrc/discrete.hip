@@ -35,6 +35,27 @@ __device__ __forceinline__ uint32_t load_flag(const uint32_t* f) {
     return __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Scene triangle t moved into the frame of a link with rows (r0, r1, r2) = [R^T | t]: vertex A as
+// R^T (A - t) and the edges rotated from world differences, so that their error is relative to
+// their own length.  delta bounds the abs. error of A per component, |A' - A'_exact| <= eps |t| +
+// 5 eps |A - t|_1, plus 8 eps |e| for B' = A' + e1', C' = A' + e2'.
+struct LinkTri {
+    float3 A, e1, e2;
+    float delta;
+};
+__device__ __forceinline__ LinkTri link_tri(const Tri& t, const float4& r0, const float4& r1, const float4& r2) {
+    LinkTri o;
+    const float tinf = fmaxf(fmaxf(fabsf(r0.w), fabsf(r1.w)), fabsf(r2.w));
+    const float3 wA = f3(t.a), wB = f3(t.b), wC = f3(t.c);
+    const float3 rel = make_float3(wA.x - r0.w, wA.y - r1.w, wA.z - r2.w);
+    o.A = rot(r0, r1, r2, rel);
+    o.e1 = rot(r0, r1, r2, sub(wB, wA));
+    o.e2 = rot(r0, r1, r2, sub(wC, wA));
+    const float en = fmaxf(l1(o.e1.x, o.e1.y, o.e1.z), l1(o.e2.x, o.e2.y, o.e2.z));
+    o.delta = 1.05f * kEps32 * (tinf + 5.f * l1(rel.x, rel.y, rel.z) + 8.f * en) + 1e-30f;
+    return o;
+}
+
 // ------------------------------------------------------------------------------------------ FK
 __global__ __launch_bounds__(kBS) void k_fk(const RobotDev* __restrict__ R, const float* __restrict__ poses, int n,
                                            double* __restrict__ T64, float4* __restrict__ Ti,
@@ -236,6 +257,46 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
     if (threadIdx.x == 0 && tris_block) atomicAdd(a.tri_candidates, (unsigned long long) tris_block);
 }
 
+// ------------------------------------------------------------------------------------------ gather
+// Exclusive prefix of the segment counts (each clamped to seg_cap): one block, seg_pre[kSeg] = total.
+__global__ __launch_bounds__(kBS) void k_seg_scan(const uint32_t* __restrict__ seg_count, uint32_t seg_cap,
+                                                 uint32_t* __restrict__ seg_pre) {
+    constexpr int per = kSeg / kBS;
+    __shared__ uint32_t part[kBS];
+    uint32_t loc[per], acc = 0;
+    for (int k = 0; k < per; ++k) {
+        loc[k] = min(seg_count[(size_t) (threadIdx.x * per + k) * kSegStride], seg_cap);
+        acc += loc[k];
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int k = 0; k < kBS; ++k) {
+            const uint32_t x = part[k];
+            part[k] = run;
+            run += x;
+        }
+        seg_pre[kSeg] = run;
+    }
+    __syncthreads();
+    uint32_t run = part[threadIdx.x];
+    for (int k = 0; k < per; ++k) {
+        seg_pre[threadIdx.x * per + k] = run;
+        run += loc[k];
+    }
+}
+
+// Block b copies segment b to flat[seg_pre[b] ..): the narrow phase then reads records by index.
+__global__ __launch_bounds__(kBS) void k_seg_gather(const uint2* __restrict__ cand, const uint32_t* __restrict__ seg_count,
+                                                   uint32_t seg_cap, const uint32_t* __restrict__ seg_pre,
+                                                   uint2* __restrict__ flat) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t n = min(seg_count[(size_t) b * kSegStride], seg_cap), base = seg_pre[b];
+    const uint2* src = cand + (size_t) b * seg_cap;
+    for (uint32_t j = threadIdx.x; j < n; j += kBS) flat[(size_t) base + j] = src[j];
+}
+
 // ------------------------------------------------------------------------------------------ narrow
 struct Ctr {
     uint32_t nodes = 0, prims = 0, fp64 = 0, edges = 0, tris = 0;
@@ -397,45 +458,35 @@ __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, u
 template <int DIR, int STACK, bool COUNT>
 __global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
     __shared__ int stk[STACK * kBS];
-    __shared__ uint32_t seg_pre[kSeg + 1];
     int* st = stk + threadIdx.x;
-    __shared__ uint32_t part[kBS];
-    seg_prefix<kBS>(a.seg_count, a.seg_cap, seg_pre, part);
     // each leaf record is spread over `lanes` adjacent threads, one triangle per thread
     const uint32_t lanes = (uint32_t) a.leaf_lanes;
-    const uint64_t total = (uint64_t) seg_pre[kSeg] * lanes;
+    const uint64_t total = (uint64_t) a.seg_pre[kSeg] * lanes;
     const uint32_t stride = gridDim.x * kBS;
     Ctr ctr;
     for (uint64_t v = blockIdx.x * kBS + threadIdx.x; v < total; v += stride) {
         const uint32_t i = (uint32_t) (v / lanes), sub_k = (uint32_t) (v - (uint64_t) i * lanes);
-        const uint2 c = seg_record(a.cand, a.seg_cap, seg_pre, i);
+        const uint2 c = a.flat[i];
         if (sub_k > leaf_last(c.y)) continue;
         const uint32_t p = c.x / (uint32_t) a.n_links, l = c.x - p * (uint32_t) a.n_links;
         uint32_t* flag = a.flags + p;
         if (load_flag(flag)) continue;
         const LinkInfo& L = a.robot->link[l];
         const float4 r0 = a.Ti[(size_t) c.x * 3], r1 = a.Ti[(size_t) c.x * 3 + 1], r2 = a.Ti[(size_t) c.x * 3 + 2];
-        const float tinf = fmaxf(fmaxf(fabsf(r0.w), fabsf(r1.w)), fabsf(r2.w));
         int hit = 0;
         {
             const uint32_t tri = leaf_start(c.y) + sub_k;
             const Tri t = a.sc.tri_prims[tri];
-            // vertex A moved as R^T (A - t); edges rotated from world differences, so that their
-            // error is relative to their own length: |A' - A'_exact| <= eps |t| + 5 eps |A - t|_1
-            const float3 wA = f3(t.a), wB = f3(t.b), wC = f3(t.c);
-            const float3 rel = make_float3(wA.x - r0.w, wA.y - r1.w, wA.z - r2.w);
-            const float3 A = rot(r0, r1, r2, rel);
-            const float3 e1 = rot(r0, r1, r2, sub(wB, wA)), e2 = rot(r0, r1, r2, sub(wC, wA));
-            const float en = fmaxf(l1(e1.x, e1.y, e1.z), l1(e2.x, e2.y, e2.z));
-            // B' = A' + e1', C' = A' + e2' add their own rounding: 8 eps |e|
-            const float delta = 1.05f * kEps32 * (tinf + 5.f * l1(rel.x, rel.y, rel.z) + 8.f * en) + 1e-30f;
+            const LinkTri m = link_tri(t, r0, r1, r2);
+            const float3 A = m.A, e1 = m.e1, e2 = m.e2;
+            const float delta = m.delta;
             if (COUNT) ctr.tris++;
             if (DIR == 1) {
                 const uint32_t owned = __float_as_uint(t.b.w);
                 if (owned & 1u) hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 0u, A, e1, delta, st, flag, ctr);
                 if (!hit && (owned & 2u))
-                    hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 1u, add(A, e1), rot(r0, r1, r2, sub(wC, wB)), delta, st,
-                                                 flag, ctr);
+                    hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 1u, add(A, e1),
+                                                 rot(r0, r1, r2, sub(f3(t.c), f3(t.b))), delta, st, flag, ctr);
                 if (!hit && (owned & 4u))
                     hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 2u, add(A, e2), neg(e2), delta, st, flag, ctr);
             } else {
@@ -543,6 +594,10 @@ static void narrow_dir(const DiscreteArgs& a, int stack, bool count, hipStream_t
 }
 
 void launch_narrow(const DiscreteArgs& a, int stack, bool count, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kBS), 0, s, a.seg_count, a.seg_cap, a.seg_pre);
+    RTCD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_seg_gather, dim3(kSeg), dim3(kBS), 0, s, a.cand, a.seg_count, a.seg_cap, a.seg_pre, a.flat);
+    RTCD_LAUNCH_CHECK();
     // obstacle->robot first: it is cheaper and flags most colliding poses, which the
     // robot->obstacle pass then skips
     if (a.dir & 1) narrow_dir<1>(a, stack, count, s);

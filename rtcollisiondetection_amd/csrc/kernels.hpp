@@ -47,6 +47,8 @@ struct DiscreteArgs {
     uint32_t seg_cap;
     uint32_t* seg_count;   // [kCandSegments * kSegStride]: records appended to segment b at b * kSegStride
                            // (may exceed seg_cap: overflow bit 1)
+    uint32_t* seg_pre;     // [kCandSegments + 1] exclusive prefix of the clamped segment counts
+    uint2* flat;           // the segments gathered in order: [seg_pre[kCandSegments]] records
     unsigned long long* tri_candidates;  // triangles in the emitted leaves
     uint32_t* overflow;
     uint32_t* flags;       // [n] output
@@ -74,8 +76,9 @@ void launch_fk(const RobotDev* robot, int dof, int n_links, const float* poses, 
                float4* wbox, hipStream_t s);
 // Broad phase: one pass appending leaf candidates (wave-aggregated atomics).
 void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
-// Narrow phase over *total candidates (grid-stride, sized for the device): one pass per direction
-// in a.dir, then the FP64 re-decision of the queued tests.  `stack` covers the robot BVH depth.
+// Narrow phase: gathers the candidate segments into one array, then one grid-stride pass per
+// direction in a.dir, then the FP64 re-decision of the queued tests.  `stack` covers the robot
+// BVH depth.
 void launch_narrow(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
 
 // ---- swept volumes ----
