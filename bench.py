@@ -8,8 +8,9 @@ Headline (one JSON line on stdout): dense two-way discrete check, poses/s on one
 
 Also reported in the same line: per-direction rates, the plain per-query launch (one pose per call),
 linear (4,096 random walks of 8-64 waypoints) and quadratic (4,096 B-splines, 8 control points)
-trajectories/s on the shelf scene, the roofline of the dominant (narrow-phase) kernel and the CPU
-oracle baseline timed on a bounded sample of the same workload.
+trajectories/s on the shelf and dense scenes, the serving stream, the roofline of the dominant kernel
+(the obstacle->robot narrow pass) and the CPU oracle baseline timed on a bounded sample of the same
+workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
 """
@@ -25,10 +26,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes of the narrow kernels ("Measurement"): node record 64 B, triangle
-# record 48 B, edge record 32 B, per leaf candidate 8 B (record) + 48 B (link transform rows), per
-# candidate triangle 48 B (scene triangle), per direction pass.
-S_NODE, S_TRI, S_EDGE, S_LEAF, S_CTRI = 64, 48, 32, 8 + 48, 48
+# Algorithmic bytes of a narrow kernel (k_narrow<1> obstacle->robot, k_narrow<2> robot->obstacle), per
+# unit of counted work: a leaf record read by each of its lanes 8 B; a candidate triangle evaluated
+# (its pose not yet flagged) 96 B = scene triangle 48 B + the link's inverse-transform rows 48 B; a
+# robot BVH node record 64 B; a robot triangle tested 48 B (obstacle->robot) or a robot edge tested
+# 32 B (robot->obstacle).
+S_REC, S_EVAL, S_NODE, S_TRI, S_EDGE = 8, 96, 64, 48, 32
 
 
 def log(msg):
@@ -120,31 +123,47 @@ def main():
         run(3)
     torch.cuda.synchronize()
     log(f'[bench] timing {args.steps} steps')
-    narrow_ms, total_ms = [], []
+    stages = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         run(3)
-        st = ctx.stats()
-        narrow_ms.append(st['stage_ms'][4])
-        total_ms.append(st['last_ms'])
+        stages.append(ctx.stats()['stage_ms'])
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     value = n * args.steps / dt
     hits = int(dout.sum().item())
-    log(f'[bench] two-way: {value:.4g} poses/s, {dt / args.steps * 1e3:.2f} ms/step, {hits} collisions')
+    stage = np.mean(np.array(stages), axis=0)
+    log(f'[bench] two-way: {value:.4g} poses/s, {dt / args.steps * 1e3:.2f} ms/step, {hits} collisions, '
+        f'stages (FK, broad, gather, O2R, R2O+FP64) ms {np.round(stage, 3).tolist()}')
 
-    # counting run (untimed): node fetches / primitive tests for the algorithmic bytes
+    # Roofline of the dominant kernel.  Each narrow pass is one launch, bracketed by events on the
+    # library's stream (stage 3: k_narrow<1>, stage 4: k_narrow<2> + the short FP64 kernel).  Their
+    # work comes from untimed counting runs: the obstacle->robot direction alone is exactly the first
+    # pass of the two-way check (same candidates, flags start at zero), and the robot->obstacle pass
+    # is the two-way count minus it.
     ctx.set_counting(True)
+    run(1)
+    c1 = ctx.stats()
     run(3)
-    cst = ctx.stats()
+    c3 = ctx.stats()
     ctx.set_counting(False)
-    prim_tri = cst['prim_tests'] - cst['edge_tests']
-    bytes_launch = (S_NODE * cst['node_fetches'] + S_TRI * prim_tri + S_EDGE * cst['edge_tests'] +
-                    2 * (S_LEAF * cst['leaf_candidates'] + S_CTRI * cst['candidates']))
-    nar = float(np.mean(narrow_ms))
-    achieved = bytes_launch / (nar * 1e-3) / 1e9
+    lanes = 4  # threads per leaf record = the scene BVH leaf size (the context default)
+    rec_lanes = lanes * c1['leaf_candidates']
+    k1 = {'kernel': 'k_narrow<1> (obstacle->robot narrow pass)', 'ms': float(stage[3]),
+          'bytes': (S_REC * rec_lanes + S_EVAL * c1['narrow_candidates'] + S_NODE * c1['node_fetches'] +
+                    S_TRI * c1['prim_tests']),
+          'node_fetches': c1['node_fetches'], 'primitive_tests': c1['prim_tests'],
+          'candidates_evaluated': c1['narrow_candidates'], 'traffic_file': 'traffic_narrow_o2r_dense.json'}
+    k2 = {'kernel': 'k_narrow<2> (robot->obstacle narrow pass)', 'ms': float(stage[4]),
+          'bytes': (S_REC * rec_lanes + S_EVAL * (c3['narrow_candidates'] - c1['narrow_candidates']) +
+                    S_NODE * (c3['node_fetches'] - c1['node_fetches']) + S_EDGE * c3['edge_tests']),
+          'node_fetches': c3['node_fetches'] - c1['node_fetches'], 'primitive_tests': c3['edge_tests'],
+          'candidates_evaluated': c3['narrow_candidates'] - c1['narrow_candidates'],
+          'traffic_file': 'traffic_narrow_r2o_dense.json'}
+    dom, other = (k1, k2) if k1['ms'] >= k2['ms'] else (k2, k1)
+    achieved = dom['bytes'] / (dom['ms'] * 1e-3) / 1e9
     traffic = None
-    tp = os.path.join(ROOT, 'profiles', 'traffic_narrow_dense_two_way.json')
+    tp = os.path.join(ROOT, 'profiles', dom['traffic_file'])
     if os.path.exists(tp):
         try:
             traffic = json.load(open(tp)).get('hbm_bytes_per_launch')
@@ -152,12 +171,19 @@ def main():
             traffic = None
     roofline = {'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                 'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
-                'kernel': 'k_narrow (narrow phase)', 'kernel_ms': round(nar, 4),
-                'algorithmic_bytes': int(bytes_launch), 'node_fetches': int(cst['node_fetches']),
-                'prim_tests': int(cst['prim_tests']), 'edge_tests': int(cst['edge_tests']),
-                'candidate_tris': int(cst['candidates']), 'leaf_candidates': int(cst['leaf_candidates']),
-                'fp64_redecisions': int(cst['fp64_redecisions']),
-                'stage_ms': [round(x, 4) for x in st['stage_ms']]}
+                'kernel': dom['kernel'], 'kernel_ms': round(dom['ms'], 4), 'algorithmic_bytes': int(dom['bytes']),
+                'node_fetches': int(dom['node_fetches']), 'primitive_tests': int(dom['primitive_tests']),
+                'leaf_records': int(c1['leaf_candidates']), 'candidates_evaluated': int(dom['candidates_evaluated']),
+                'other_narrow': {'kernel': other['kernel'], 'kernel_ms': round(other['ms'], 4),
+                                 'achieved_gbs': round(other['bytes'] / (other['ms'] * 1e-3) / 1e9, 2)},
+                'stage_ms': {'fk': round(float(stage[0]), 4), 'broad': round(float(stage[1]), 4),
+                             'gather': round(float(stage[2]), 4), 'narrow_o2r': round(float(stage[3]), 4),
+                             'narrow_r2o_fp64': round(float(stage[4]), 4)},
+                'two_way_counts': {'broad_node_fetches': int(c3['broad_node_fetches']),
+                                   'candidate_tris': int(c3['candidates']),
+                                   'narrow_node_fetches': int(c3['node_fetches']),
+                                   'prim_tests': int(c3['prim_tests']), 'edge_tests': int(c3['edge_tests']),
+                                   'fp64_redecisions': int(c3['fp64_redecisions'])}}
     log(f'[bench] roofline {roofline}')
 
     extra = {}

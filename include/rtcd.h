@@ -71,11 +71,14 @@ typedef struct {
     uint32_t max_stack;      /* traversal stack size used by the last call */
     uint32_t max_depth;      /* deepest BVH involved in the last call */
     uint64_t edge_tests;     /* of prim_tests: tests that read a 32 B edge record (the rest read 48 B) */
-    double stage_ms[5];      /* discrete: FK, broad, (unused), (unused), narrow + FP64 re-decision (device ms) */
+    double stage_ms[5];      /* discrete (device ms): FK, broad phase, candidate gather, obstacle->robot narrow,
+                                robot->obstacle narrow + FP64 re-decision */
     uint64_t leaf_candidates; /* counting launches: discrete (pose, link, scene leaf) records of the broad phase;
                                  sweeps: subtrees handed off between passes */
     uint64_t cert_calls;     /* sweeps: FP32 capsule certificates evaluated (quadratic: one per refined sub-piece) */
     uint64_t broad_node_fetches; /* discrete: scene BVH nodes visited by the broad phase */
+    uint64_t narrow_candidates;  /* discrete: candidate triangles the narrow passes evaluated (not skipped
+                                    because their pose was already flagged), summed over directions */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */

@@ -48,7 +48,8 @@ class Stats(C.Structure):
     _fields_ = [('node_fetches', C.c_uint64), ('prim_tests', C.c_uint64), ('fp64_redecisions', C.c_uint64),
                 ('candidates', C.c_uint64), ('last_ms', C.c_double), ('max_stack', C.c_uint32),
                 ('max_depth', C.c_uint32), ('edge_tests', C.c_uint64), ('stage_ms', C.c_double * 5),
-                ('leaf_candidates', C.c_uint64), ('cert_calls', C.c_uint64), ('broad_node_fetches', C.c_uint64)]
+                ('leaf_candidates', C.c_uint64), ('cert_calls', C.c_uint64), ('broad_node_fetches', C.c_uint64),
+                ('narrow_candidates', C.c_uint64)]
 
 
 class Request(C.Structure):

@@ -165,9 +165,14 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     RTCD_HIP(hipEventRecord(ws.ev[1], s));
     if (a.F > 0) launch_broad(a, broad_stack, ws.counting, s);
     RTCD_HIP(hipEventRecord(ws.ev[2], s));
+    launch_gather(a, s);
     RTCD_HIP(hipEventRecord(ws.ev[3], s));
+    // obstacle->robot first: it is cheaper and flags most colliding poses, which the robot->obstacle
+    // pass then skips
+    if (dir & 1) launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
     RTCD_HIP(hipEventRecord(ws.ev[4], s));
-    launch_narrow(a, c->robot_depth + 1, ws.counting, s);
+    if (dir & 2) launch_narrow(a, 2, c->robot_depth + 1, ws.counting, s);
+    launch_redecide(a, s);
     RTCD_HIP(hipEventRecord(ws.ev[5], s));
     ws.timing_pending = true;
 }
@@ -196,7 +201,8 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                 if (ws.timing_pending) {
                     float ms = 0.f;
                     if (hipEventElapsedTime(&ms, ws.ev[0], ws.ev[5]) == hipSuccess) st->last_ms = ms;
-                    // stage k spans ev[k] (k = 0: start) .. ev[k+1]; ev[1] directly follows FK
+                    // stage k spans ev[k] .. ev[k+1]: FK, broad, gather, obstacle->robot narrow,
+                    // robot->obstacle narrow + FP64 re-decision
                     hipEvent_t b[6] = {ws.ev[0], ws.ev[1], ws.ev[2], ws.ev[3], ws.ev[4], ws.ev[5]};
                     for (int k = 0; k < 5; ++k) {
                         float sm = 0.f;
@@ -214,6 +220,7 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                     st->fp64_redecisions = k.fp64;
                     st->edge_tests = k.edge_tests;
                     st->broad_node_fetches = k.broad_nodes;
+                    st->narrow_candidates = k.narrow_tris;
                 }
                 st->cert_calls = 0;
             }

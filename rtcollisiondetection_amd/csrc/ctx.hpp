@@ -107,7 +107,7 @@ struct Workspace {
     DevBuf<float4> P32;       // [n_traj][S][n_ctrl]
     DevBuf<Stats> stats;
     std::vector<double> W_host;  // last quadratic weights uploaded to W64
-    // timing: ev[0] start, ev[1..4] after FK, broad count, scan, broad emit, ev[5] end
+    // timing: ev[0] start, ev[1..4] after FK, broad phase, gather, obstacle->robot narrow; ev[5] end
     hipEvent_t ev[6] = {};
     bool timing_pending = false;
     bool counting = false;

@@ -500,6 +500,7 @@ __global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
         atomicAdd(&a.stats->prim_tests, (unsigned long long) ctr.prims);
         atomicAdd(&a.stats->fp64, (unsigned long long) ctr.fp64);
         atomicAdd(&a.stats->edge_tests, (unsigned long long) ctr.edges);
+        atomicAdd(&a.stats->narrow_tris, (unsigned long long) ctr.tris);
     }
 }
 
@@ -593,15 +594,19 @@ static void narrow_dir(const DiscreteArgs& a, int stack, bool count, hipStream_t
     else throw std::runtime_error("BVH deeper than 64 levels");
 }
 
-void launch_narrow(const DiscreteArgs& a, int stack, bool count, hipStream_t s) {
+void launch_gather(const DiscreteArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kBS), 0, s, a.seg_count, a.seg_cap, a.seg_pre);
     RTCD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_seg_gather, dim3(kSeg), dim3(kBS), 0, s, a.cand, a.seg_count, a.seg_cap, a.seg_pre, a.flat);
     RTCD_LAUNCH_CHECK();
-    // obstacle->robot first: it is cheaper and flags most colliding poses, which the
-    // robot->obstacle pass then skips
-    if (a.dir & 1) narrow_dir<1>(a, stack, count, s);
-    if (a.dir & 2) narrow_dir<2>(a, stack, count, s);
+}
+
+void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStream_t s) {
+    if (dir == 1) narrow_dir<1>(a, stack, count, s);
+    else narrow_dir<2>(a, stack, count, s);
+}
+
+void launch_redecide(const DiscreteArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_redecide, dim3((unsigned) device_cus() * 2u), dim3(kBS), 0, s, a);
     RTCD_LAUNCH_CHECK();
 }

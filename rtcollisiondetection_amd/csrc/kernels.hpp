@@ -31,7 +31,7 @@ struct SceneDev {
 };
 
 struct Stats {
-    unsigned long long node_fetches, prim_tests, fp64, candidates, edge_tests, certs, broad_nodes;
+    unsigned long long node_fetches, prim_tests, fp64, candidates, edge_tests, certs, broad_nodes, narrow_tris;
 };
 
 struct DiscreteArgs {
@@ -76,10 +76,13 @@ void launch_fk(const RobotDev* robot, int dof, int n_links, const float* poses, 
                float4* wbox, hipStream_t s);
 // Broad phase: one pass appending leaf candidates (wave-aggregated atomics).
 void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
-// Narrow phase: gathers the candidate segments into one array, then one grid-stride pass per
-// direction in a.dir, then the FP64 re-decision of the queued tests.  `stack` covers the robot
-// BVH depth.
-void launch_narrow(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
+// Gathers the candidate segments, in order, into one flat array (a.flat, a.seg_pre).
+void launch_gather(const DiscreteArgs& a, hipStream_t s);
+// Narrow phase, one grid-stride pass in direction `dir` (1: obstacle->robot, 2: robot->obstacle)
+// over the gathered candidates.  `stack` covers the robot BVH depth.
+void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStream_t s);
+// FP64 re-decision of the queued tests.
+void launch_redecide(const DiscreteArgs& a, hipStream_t s);
 
 // ---- swept volumes ----
 struct SweepArgs {
