@@ -79,6 +79,8 @@ typedef struct {
     uint64_t broad_node_fetches; /* discrete: scene BVH nodes visited by the broad phase */
     uint64_t narrow_candidates;  /* discrete: candidate triangles the narrow passes evaluated (not skipped
                                     because their pose was already flagged), summed over directions */
+    uint32_t graph_replayed;     /* discrete: 1 if the last check replayed a captured graph (stage_ms
+                                    are then -1: the graph records no inner events) */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */
@@ -104,8 +106,10 @@ rtcd_status rtcd_scene_info(rtcd_ctx* ctx, uint32_t scene_id, uint32_t* n_tris, 
  * Replaces CollisionDetector::setNCheck + check + downloadResult (CollisionDetector.h:354-471). */
 rtcd_status rtcd_check_discrete(rtcd_ctx* ctx, uint32_t scene_id, int32_t direction, const float* poses,
                                 uint32_t n, uint32_t* out);
-/* Same with device pointers, enqueued on `stream` (hipStream_t, may be NULL); returns before the
- * work completes.  rtcd_sync(ctx, stream) waits and reports deferred errors. */
+/* Same with device pointers, enqueued on `stream` (hipStream_t); returns before the work completes.
+ * NULL selects the context's own stream, which is ordered with the device's null stream (work the
+ * caller put on the default stream runs first).  rtcd_sync(ctx, stream) waits and reports deferred
+ * errors. */
 rtcd_status rtcd_check_discrete_device(rtcd_ctx* ctx, uint32_t scene_id, int32_t direction, const float* d_poses,
                                        uint32_t n, uint32_t* d_out, void* stream);
 rtcd_status rtcd_sync(rtcd_ctx* ctx, void* stream);
@@ -176,7 +180,9 @@ typedef enum {
     RTCD_PARAM_BROAD_BUDGET_SMALL = 5, /* the same for batches below RTCD_PARAM_FRONTIER_MAX_TASKS tasks */
     RTCD_PARAM_BROAD_BUDGET_NEXT = 6,  /* the same in every broad pass after the first */
     RTCD_PARAM_BROAD_PASSES = 7,       /* broad-phase passes, 1..16; the last never hands off */
-    RTCD_PARAM_SWEEP_PASSES = 8        /* sweep passes, 1..16; the last never hands off */
+    RTCD_PARAM_SWEEP_PASSES = 8,       /* sweep passes, 1..16; the last never hands off */
+    RTCD_PARAM_GRAPHS = 9              /* 1 (default): a discrete check repeated with the same shape and
+                                          buffers replays a captured hipGraph; 0: always launch eagerly */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value);

@@ -22,7 +22,7 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
            'rtcd_quadratic_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
            'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param']
 PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad_budget': 4,
-          'broad_budget_small': 5, 'broad_budget_next': 6, 'broad_passes': 7, 'sweep_passes': 8}
+          'broad_budget_small': 5, 'broad_budget_next': 6, 'broad_passes': 7, 'sweep_passes': 8, 'graphs': 9}
 REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC = 1, 2, 3
 
 
@@ -49,7 +49,7 @@ class Stats(C.Structure):
                 ('candidates', C.c_uint64), ('last_ms', C.c_double), ('max_stack', C.c_uint32),
                 ('max_depth', C.c_uint32), ('edge_tests', C.c_uint64), ('stage_ms', C.c_double * 5),
                 ('leaf_candidates', C.c_uint64), ('cert_calls', C.c_uint64), ('broad_node_fetches', C.c_uint64),
-                ('narrow_candidates', C.c_uint64)]
+                ('narrow_candidates', C.c_uint64), ('graph_replayed', C.c_uint32)]
 
 
 class Request(C.Structure):

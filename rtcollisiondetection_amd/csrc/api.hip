@@ -75,7 +75,7 @@ size_t discrete_bytes(const rtcd_ctx* c, uint32_t n) {
 
 // Enqueue FK -> broad(count) -> scan -> broad(emit) -> narrow on ws.stream.  No host synchronisation.
 void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
-                      uint32_t* d_out, size_t budget) {
+                      uint32_t* d_out, size_t budget, bool events) {
     const hipStream_t s = ws.stream;
     const int NL = c->robot.n_links;
     const int l0 = c->cfg.skip_base_link ? 1 : 0;
@@ -124,7 +124,10 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     fill_u32(ws.scalars.p, kScalars, 0u, s);
     fill_u32(ws.seg_count.p, (size_t) kCandSegments * kSegStride, 0u, s);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, s);
-    RTCD_HIP(hipEventRecord(ws.ev[0], s));
+    auto mark = [&](int k) {
+        if (events) RTCD_HIP(hipEventRecord(ws.ev[k], s));
+    };
+    mark(0);
     launch_fk(c->d_robot.p, c->robot.dof, NL, d_poses, (int) n, ws.T64.p, ws.Ti.p, ws.wbox.p, s);
     DiscreteArgs a{};
     a.robot = c->d_robot.p;
@@ -162,19 +165,89 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.budget_next = c->broad_budget_next;
     a.n_passes = (int) c->broad_passes;
     const int broad_stack = stack_for(sc.tri_bvh.depth);
-    RTCD_HIP(hipEventRecord(ws.ev[1], s));
+    mark(1);
     if (a.F > 0) launch_broad(a, broad_stack, ws.counting, s);
-    RTCD_HIP(hipEventRecord(ws.ev[2], s));
+    mark(2);
     launch_gather(a, s);
-    RTCD_HIP(hipEventRecord(ws.ev[3], s));
+    mark(3);
     // obstacle->robot first: it is cheaper and flags most colliding poses, which the robot->obstacle
     // pass then skips
     if (dir & 1) launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
-    RTCD_HIP(hipEventRecord(ws.ev[4], s));
+    mark(4);
     if (dir & 2) launch_narrow(a, 2, c->robot_depth + 1, ws.counting, s);
     launch_redecide(a, s);
-    RTCD_HIP(hipEventRecord(ws.ev[5], s));
-    ws.timing_pending = true;
+    mark(5);
+    ws.timing_pending = events;
+    ws.stages_recorded = events;
+}
+
+void drop_graph(rtcd_ctx* c) {
+    if (c->dgraph.exec) (void) hipGraphExecDestroy(c->dgraph.exec);
+    c->dgraph.exec = nullptr;
+    c->dgraph.exec_key.clear();
+    c->dgraph.key.clear();
+    c->dgraph.failed_key.clear();
+}
+
+// Everything a captured discrete check bakes in: shape, the caller's buffers, every workspace,
+// scene and robot buffer with its capacity, and the launch parameters.
+static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, const SceneRes& sc, int dir,
+                                       const float* d_poses, uint32_t n, const uint32_t* d_out) {
+    auto P = [](const void* p) { return (uint64_t) (uintptr_t) p; };
+    return {P(&sc), P(sc.tri_bvh.nodes.p), P(sc.tris.p), P(sc.edge_bvh.nodes.p), P(sc.edges.p), P(sc.front.p),
+            P(c->d_robot.p), P(c->r_tri_nodes.p), P(c->r_tris.p), P(c->r_edge_nodes.p), P(c->r_edges.p),
+            P(ws.stream), P(d_poses), P(d_out), n, (uint64_t) dir,
+            P(ws.T64.p), P(ws.Ti.p), P(ws.wbox.p), P(ws.cand.p), ws.cand.n, P(ws.flat.p), ws.flat.n,
+            P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.spill.p), ws.spill.n, P(ws.unsure.p), ws.unsure.n,
+            P(ws.scalars.p), P(ws.stats.p), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
+            c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth};
+}
+
+void enqueue_discrete_main(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, uint32_t n, uint32_t* d_out) {
+    Workspace& ws = c->main;
+    auto& g = c->dgraph;
+    g.last_replayed = false;
+    if (!c->use_graphs || ws.counting) {
+        enqueue_discrete(c, ws, sc, dir, d_poses, n, d_out, 0);
+        return;
+    }
+    const hipStream_t s = ws.stream;
+    std::vector<uint64_t> k = graph_key(c, ws, sc, dir, d_poses, n, d_out);
+    if (!(g.exec && g.exec_key == k) && g.key == k && g.failed_key != k) {
+        // second call with this key: capture it (every buffer is already allocated at this size).
+        // A capture that fails for any reason leaves the key to eager launches.
+        drop_graph(c);
+        g.failed_key = k;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            bool ok = true;
+            try {
+                enqueue_discrete(c, ws, sc, dir, d_poses, n, d_out, 0, false);
+            } catch (...) {
+                ok = false;
+            }
+            hipGraph_t graph = nullptr;
+            if (hipStreamEndCapture(s, &graph) != hipSuccess) ok = false;
+            hipGraphExec_t exec = nullptr;
+            if (ok && graph && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                g.exec = exec;
+                g.exec_key = k;
+                g.failed_key.clear();
+            }
+            if (graph) (void) hipGraphDestroy(graph);
+        }
+        (void) hipGetLastError();
+    }
+    if (g.exec && g.exec_key == k) {
+        RTCD_HIP(hipEventRecord(ws.ev[0], s));
+        RTCD_HIP(hipGraphLaunch(g.exec, s));
+        RTCD_HIP(hipEventRecord(ws.ev[5], s));
+        ws.timing_pending = true;
+        ws.stages_recorded = false;
+        g.last_replayed = true;
+        return;
+    }
+    enqueue_discrete(c, ws, sc, dir, d_poses, n, d_out, 0);
+    g.key = graph_key(c, ws, sc, dir, d_poses, n, d_out);
 }
 
 void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
@@ -206,7 +279,9 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                     hipEvent_t b[6] = {ws.ev[0], ws.ev[1], ws.ev[2], ws.ev[3], ws.ev[4], ws.ev[5]};
                     for (int k = 0; k < 5; ++k) {
                         float sm = 0.f;
-                        st->stage_ms[k] = hipEventElapsedTime(&sm, b[k], b[k + 1]) == hipSuccess ? sm : -1.0;
+                        st->stage_ms[k] = ws.stages_recorded && hipEventElapsedTime(&sm, b[k], b[k + 1]) == hipSuccess
+                                              ? sm
+                                              : -1.0;
                     }
                 }
                 st->candidates = (uint64_t) sv[4] | ((uint64_t) sv[5] << 32);  // triangles in emitted leaves
@@ -223,6 +298,7 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                     st->narrow_candidates = k.narrow_tris;
                 }
                 st->cert_calls = 0;
+                st->graph_replayed = &ws == &c->main && c->dgraph.last_replayed ? 1u : 0u;
             }
             ws.timing_pending = false;
             return;
@@ -268,8 +344,10 @@ rtcd_status rtcd_create(const rtcd_config* cfg, rtcd_ctx** out) {
         RTCD_HIP(hipSetDevice(c.device));
         auto ctx = std::make_unique<rtcd_ctx>();
         ctx->cfg = c;
+        // the context's own stream (used when a *_device call passes NULL) is ordered with the
+        // device's null stream, so NULL means "after the caller's default-stream work"
         hipStream_t st = nullptr;
-        RTCD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        RTCD_HIP(hipStreamCreate(&st));
         ctx->main.init(st, true);
         *out = ctx.release();
         return RTCD_OK;
@@ -281,6 +359,7 @@ void rtcd_destroy(rtcd_ctx* ctx) {
     (void) hipSetDevice(ctx->cfg.device);
     (void) hipDeviceSynchronize();
     serve_shutdown(ctx);
+    drop_graph(ctx);
     ctx->main.destroy();
     delete ctx;
 }
@@ -424,6 +503,7 @@ rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
         ctx->d_robot.upload(&ctx->robot, 1, s);
         RTCD_HIP(hipStreamSynchronize(s));
         ctx->robot_depth = depth;
+        drop_graph(ctx);
         ctx->robot_loaded = true;
         return RTCD_OK;
     });
@@ -501,6 +581,7 @@ rtcd_status rtcd_scene_destroy(rtcd_ctx* ctx, uint32_t scene_id) {
     auto it = ctx->scenes.find(scene_id);
     if (it == ctx->scenes.end()) return fail(RTCD_ERR_UNKNOWN_ID, "unknown scene id");
     (void) hipStreamSynchronize(ctx->main.stream);
+    drop_graph(ctx);
     ctx->scenes.erase(it);  // served requests still in flight hold their own reference
     return RTCD_OK;
 }
@@ -533,7 +614,7 @@ rtcd_status rtcd_check_discrete_device(rtcd_ctx* ctx, uint32_t scene_id, int32_t
         if (stream) ws.stream = (hipStream_t) stream;
         ws.counting = ctx->counting;
         try {
-            enqueue_discrete(ctx, ws, *sc, direction, d_poses, n, d_out, 0);
+            enqueue_discrete_main(ctx, *sc, direction, d_poses, n, d_out);
         } catch (...) {
             ws.stream = own;
             throw;
@@ -604,7 +685,7 @@ rtcd_status rtcd_check_discrete(rtcd_ctx* ctx, uint32_t scene_id, int32_t direct
         ws.h_poses.reserve((size_t) n * ctx->robot.dof);
         ws.h_out.reserve(n);
         RTCD_HIP(hipMemcpyAsync(ws.h_poses.p, poses, sizeof(float) * n * ctx->robot.dof, hipMemcpyHostToDevice, s));
-        enqueue_discrete(ctx, ws, *sc, direction, ws.h_poses.p, n, ws.h_out.p, 0);
+        enqueue_discrete_main(ctx, *sc, direction, ws.h_poses.p, n, ws.h_out.p);
         RTCD_HIP(hipStreamSynchronize(s));
         finish_discrete(ctx, ws, *sc, direction, ws.h_poses.p, n, ws.h_out.p, 0, &ctx->last);
         RTCD_HIP(hipMemcpyAsync(out, ws.h_out.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
@@ -635,6 +716,10 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
         (param == RTCD_PARAM_BROAD_BUDGET ? ctx->broad_budget
          : param == RTCD_PARAM_BROAD_BUDGET_SMALL ? ctx->broad_budget_small : ctx->broad_budget_next) = (uint32_t) value;
         return RTCD_OK;
+    case RTCD_PARAM_GRAPHS:
+        ctx->use_graphs = value != 0;
+        if (!ctx->use_graphs) drop_graph(ctx);
+        return RTCD_OK;
     case RTCD_PARAM_BROAD_PASSES:
     case RTCD_PARAM_SWEEP_PASSES:
         if (value < 1 || value > (uint64_t) kMaxPasses) return fail(RTCD_ERR_INVALID_ARG, "passes must be 1..16");
@@ -657,6 +742,7 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_BROAD_BUDGET_NEXT: *value = ctx->broad_budget_next; return RTCD_OK;
     case RTCD_PARAM_BROAD_PASSES: *value = ctx->broad_passes; return RTCD_OK;
     case RTCD_PARAM_SWEEP_PASSES: *value = ctx->sweep_passes; return RTCD_OK;
+    case RTCD_PARAM_GRAPHS: *value = ctx->use_graphs ? 1 : 0; return RTCD_OK;
     default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
 }

@@ -110,6 +110,7 @@ struct Workspace {
     // timing: ev[0] start, ev[1..4] after FK, broad phase, gather, obstacle->robot narrow; ev[5] end
     hipEvent_t ev[6] = {};
     bool timing_pending = false;
+    bool stages_recorded = false;  // ev[1..4] belong to the last launch (not a replayed graph)
     bool counting = false;
 
     size_t bytes() const {
@@ -176,6 +177,16 @@ struct rtcd_ctx {
         uint32_t* out;
         hipStream_t stream;
     } pending;
+    // captured discrete check on the main workspace: replayed while the next call's key (shape,
+    // every buffer the kernels touch, launch parameters) equals the captured one
+    struct Graph {
+        std::vector<uint64_t> key;      // key of the last eager call (candidate for capture)
+        std::vector<uint64_t> exec_key; // key of exec
+        std::vector<uint64_t> failed_key; // a key whose capture failed: launched eagerly
+        hipGraphExec_t exec = nullptr;
+        bool last_replayed = false;
+    } dgraph;
+    bool use_graphs = true;
     // serving loop
     ServeState* serve = nullptr;
     size_t serve_bytes = 0;  // device bytes held by serving slots
@@ -198,8 +209,14 @@ inline std::shared_ptr<SceneRes> find_scene(rtcd_ctx* c, uint32_t id) {
 
 // ---- orchestration (api.hip: discrete, swept.hip: sweeps); all enqueue on ws.stream ----
 // Budget checks use `budget` bytes (0 = unlimited) for this workspace alone.
+// With events = false no timing events are recorded (used while capturing a graph).
 void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
-                      uint32_t* d_out, size_t budget);
+                      uint32_t* d_out, size_t budget, bool events = true);
+// enqueue_discrete on the main workspace, replaying a captured graph when the previous call had the
+// same key; the first repeat of a key is captured.  Counting launches are never captured.
+void enqueue_discrete_main(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, uint32_t n, uint32_t* d_out);
+// Destroys the captured graph (scene or robot changes, context teardown).
+void drop_graph(rtcd_ctx* c);
 // After ws.stream finished: re-runs on candidate overflow; fills *st (timing, counters) if non-null.
 void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
                      uint32_t* d_out, size_t budget, rtcd_stats* st);
