@@ -181,11 +181,33 @@ typedef enum {
     RTCD_PARAM_BROAD_BUDGET_NEXT = 6,  /* the same in every broad pass after the first */
     RTCD_PARAM_BROAD_PASSES = 7,       /* broad-phase passes, 1..16; the last never hands off */
     RTCD_PARAM_SWEEP_PASSES = 8,       /* sweep passes, 1..16; the last never hands off */
-    RTCD_PARAM_GRAPHS = 9              /* 1 (default): a discrete check repeated with the same shape and
+    RTCD_PARAM_GRAPHS = 9,             /* 1 (default): a discrete check repeated with the same shape and
                                           buffers replays a captured hipGraph; 0: always launch eagerly */
+    RTCD_PARAM_NARROW_BLOCK = 10       /* work-group size of the discrete narrow passes: 64, 128, 256 */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
+/* Rebuilds the robot's and every scene's BVHs with another leaf size (1..16); scene ids are kept.
+ * Replaces rtcd_config.leaf_size for the rest of the context's life.  No counterpart in RTCD, whose
+ * OptiX builds choose their own leaves (RTCD/Meshes/mesh.h:745-821). */
+rtcd_status rtcd_set_leaf_size(rtcd_ctx* ctx, uint32_t leaf_size);
 rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value);
+
+/* Autotuner.  Times `sample` (a request as the serving loop takes it; `out` is not used) on the device
+ * under the context's launch-shape choices by coordinate descent and keeps the fastest: for discrete
+ * samples the BVH leaf size (with RTCD_TUNE_LEAF_SIZE: 2, 4, 8, rebuilding the robot and every scene),
+ * the narrow work-group size and the broad-phase hand-off passes and budgets for the sample's batch
+ * size; for swept samples the sweep hand-off passes and budget.  Every configuration must reproduce
+ * the first one's flags, or the call fails with RTCD_ERR_INTERNAL and the context keeps its
+ * settings.  The serving loop must be stopped.  No counterpart in RTCD (compile-time shapes,
+ * CollisionDetector.h:396-433). */
+#define RTCD_TUNE_LEAF_SIZE 1u
+typedef struct {
+    double ms_before, ms_after;  /* device ms of the sample with the entry / chosen settings */
+    uint32_t configs_timed;
+    uint32_t leaf_size, narrow_block, broad_passes, broad_budget, broad_budget_small, broad_budget_next;
+    uint32_t sweep_budget, sweep_passes;
+} rtcd_tune_report;
+rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* sample, uint32_t flags, rtcd_tune_report* report);
 
 /* Counting mode: when enabled, check calls also count node fetches / primitive tests (slower). */
 rtcd_status rtcd_set_counting(rtcd_ctx* ctx, int32_t enable);

@@ -164,6 +164,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.budget = tasks < c->frontier_max_tasks ? c->broad_budget_small : c->broad_budget;
     a.budget_next = c->broad_budget_next;
     a.n_passes = (int) c->broad_passes;
+    a.narrow_bs = (int) c->narrow_bs;
     const int broad_stack = stack_for(sc.tri_bvh.depth);
     mark(1);
     if (a.F > 0) launch_broad(a, broad_stack, ws.counting, s);
@@ -200,7 +201,7 @@ static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, c
             P(ws.T64.p), P(ws.Ti.p), P(ws.wbox.p), P(ws.cand.p), ws.cand.n, P(ws.flat.p), ws.flat.n,
             P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.spill.p), ws.spill.n, P(ws.unsure.p), ws.unsure.n,
             P(ws.scalars.p), P(ws.stats.p), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
-            c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth};
+            c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth, c->narrow_bs};
 }
 
 void enqueue_discrete_main(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, uint32_t n, uint32_t* d_out) {
@@ -364,6 +365,143 @@ void rtcd_destroy(rtcd_ctx* ctx) {
     delete ctx;
 }
 
+// Builds and uploads the robot (validated description); keeps a host copy for rebuilds.
+static rtcd_status load_robot_impl(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
+    RTCD_HIP(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->main.stream;
+    const int NL = (int) d->dof + 1;
+    RobotDev R{};
+    R.dof = (int) d->dof;
+    R.n_links = NL;
+    R.n_spheres = (int) d->n_spheres;
+    memcpy(R.base, d->base, sizeof(float) * 12);
+    memcpy(R.cmask, d->cos_mask, sizeof(float) * 12 * d->dof);
+    memcpy(R.smask, d->sin_mask, sizeof(float) * 12 * d->dof);
+    memcpy(R.omask, d->one_mask, sizeof(float) * 12 * d->dof);
+    if (!finite_all(R.base, 12) || !finite_all(R.cmask, 12 * d->dof) || !finite_all(R.smask, 12 * d->dof) ||
+        !finite_all(R.omask, 12 * d->dof))
+        return fail(RTCD_ERR_INVALID_ARG, "non-finite FK mask");
+    std::vector<BNode> all_tn, all_en;
+    std::vector<Tri> all_t;
+    std::vector<Edge> all_e;
+    int depth = 1;
+    for (int l = 0; l < NL; ++l) {
+        const uint32_t v0 = d->link_vert_offsets[l], v1 = d->link_vert_offsets[l + 1];
+        const uint32_t t0 = d->link_tri_offsets[l], t1 = d->link_tri_offsets[l + 1];
+        if (v1 < v0 || t1 < t0) return fail(RTCD_ERR_INVALID_ARG, "offsets must be non-decreasing");
+        const float* V = d->verts + 3 * (size_t) v0;
+        const int32_t* F = d->tris + 3 * (size_t) t0;
+        const int nv = (int) (v1 - v0), nt = (int) (t1 - t0);
+        if (!finite_all(V, 3 * (size_t) nv)) return fail(RTCD_ERR_INVALID_ARG, "non-finite link vertex");
+        for (int i = 0; i < 3 * nt; ++i)
+            if (F[i] < 0 || F[i] >= nv) return fail(RTCD_ERR_INVALID_ARG, "link triangle index out of range");
+        LinkInfo& L = R.link[l];
+        double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        if (nv > 0)
+            for (int k = 0; k < 3; ++k) { lo[k] = V[k]; hi[k] = V[k]; }
+        for (int i = 0; i < nv; ++i)
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = std::min(lo[k], (double) V[3 * i + k]);
+                hi[k] = std::max(hi[k], (double) V[3 * i + k]);
+            }
+        double c[3], sc = 0, rv = 0, e1 = 0, e2 = 0, dm = 0;
+        for (int k = 0; k < 3; ++k) {
+            L.lo[k] = (float) lo[k];
+            L.hi[k] = (float) hi[k];
+            L.c[k] = (float) (0.5 * (lo[k] + hi[k]));
+            c[k] = L.c[k];
+            sc = std::max(sc, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
+        }
+        for (int i = 0; i < nv; ++i)
+            rv = std::max(rv, std::fabs(V[3 * i] - c[0]) + std::fabs(V[3 * i + 1] - c[1]) + std::fabs(V[3 * i + 2] - c[2]));
+        EdgeSet es = unique_edges(F, nt);
+        // triangles (exact vertices) and their boxes
+        std::vector<Tri> tri(nt);
+        std::vector<float4> tb(2 * (size_t) nt);
+        for (int f = 0; f < nt; ++f) {
+            const float* a = V + 3 * F[3 * f];
+            const float* b = V + 3 * F[3 * f + 1];
+            const float* cc = V + 3 * F[3 * f + 2];
+            tri[f].a = make_float4(a[0], a[1], a[2], bits_f((uint32_t) f));
+            tri[f].b = make_float4(b[0], b[1], b[2], bits_f(es.owned[f]));
+            tri[f].c = make_float4(cc[0], cc[1], cc[2], 0.f);
+            e1 = std::max(e1, std::fabs((double) b[0] - a[0]) + std::fabs((double) b[1] - a[1]) + std::fabs((double) b[2] - a[2]));
+            e2 = std::max(e2, std::fabs((double) cc[0] - a[0]) + std::fabs((double) cc[1] - a[1]) + std::fabs((double) cc[2] - a[2]));
+            tb[2 * f] = make_float4(std::min({a[0], b[0], cc[0]}), std::min({a[1], b[1], cc[1]}), std::min({a[2], b[2], cc[2]}), 0.f);
+            tb[2 * f + 1] = make_float4(std::max({a[0], b[0], cc[0]}), std::max({a[1], b[1], cc[1]}), std::max({a[2], b[2], cc[2]}), 0.f);
+        }
+        const int ne = (int) es.edges.size() / 2;
+        std::vector<Edge> edg(ne);
+        std::vector<float4> eb(2 * (size_t) ne);
+        for (int e = 0; e < ne; ++e) {
+            const float* p = V + 3 * es.edges[2 * e];
+            const float* q = V + 3 * es.edges[2 * e + 1];
+            edg[e].p = make_float4(p[0], p[1], p[2], bits_f((uint32_t) e));
+            edg[e].q = make_float4(q[0], q[1], q[2], 0.f);
+            dm = std::max(dm, std::fabs((double) q[0] - p[0]) + std::fabs((double) q[1] - p[1]) + std::fabs((double) q[2] - p[2]));
+            eb[2 * e] = make_float4(std::min(p[0], q[0]), std::min(p[1], q[1]), std::min(p[2], q[2]), 0.f);
+            eb[2 * e + 1] = make_float4(std::max(p[0], q[0]), std::max(p[1], q[1]), std::max(p[2], q[2]), 0.f);
+        }
+        L.scale = round_up(sc);
+        L.rv = round_up(rv * (1 + 1e-6));
+        L.e1max = round_up(e1 * (1 + 1e-6));
+        L.e2max = round_up(e2 * (1 + 1e-6));
+        L.dmax = round_up(dm * (1 + 1e-6));
+        BvhOut bt, be;
+        std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->cfg.leaf_size, s, bt);
+        std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->cfg.leaf_size, s, be);
+        L.tri_node0 = (int) all_tn.size();
+        L.tri_prim0 = (int) all_t.size();
+        L.n_tri = nt;
+        L.tri_has_root = bt.n_nodes > 0;
+        L.edge_node0 = (int) all_en.size();
+        L.edge_prim0 = (int) all_e.size();
+        L.n_edge = ne;
+        L.edge_has_root = be.n_nodes > 0;
+        std::vector<BNode> hn = bt.nodes.download(bt.n_nodes, s);
+        all_tn.insert(all_tn.end(), hn.begin(), hn.end());
+        for (int i = 0; i < nt; ++i) all_t.push_back(tri[pt[i]]);
+        hn = be.nodes.download(be.n_nodes, s);
+        all_en.insert(all_en.end(), hn.begin(), hn.end());
+        for (int i = 0; i < ne; ++i) all_e.push_back(edg[pe[i]]);
+        depth = std::max(depth, std::max(bt.depth, be.depth));
+    }
+    if (!d->n_spheres) R.n_spheres = 0;
+    std::vector<float4> sph(d->n_spheres);
+    for (uint32_t i = 0; i < d->n_spheres; ++i) {
+        if (d->sphere_link[i] < 0 || d->sphere_link[i] >= NL) return fail(RTCD_ERR_INVALID_ARG, "sphere link out of range");
+        if (!finite_all(d->spheres + 4 * i, 4) || d->spheres[4 * i + 3] < 0.f)
+            return fail(RTCD_ERR_INVALID_ARG, "invalid sphere");
+        sph[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
+    }
+    ctx->r_tri_nodes.alloc(all_tn.size());
+    ctx->r_tri_nodes.upload(all_tn, s);
+    ctx->r_tris.alloc(all_t.size());
+    ctx->r_tris.upload(all_t, s);
+    ctx->r_edge_nodes.alloc(all_en.size());
+    ctx->r_edge_nodes.upload(all_en, s);
+    ctx->r_edges.alloc(all_e.size());
+    ctx->r_edges.upload(all_e, s);
+    ctx->r_spheres.alloc(std::max<size_t>(sph.size(), 1));
+    ctx->r_spheres.upload(sph, s);
+    std::vector<int32_t> sl(d->sphere_link, d->sphere_link + d->n_spheres);
+    ctx->r_sphere_link.alloc(std::max<size_t>(sl.size(), 1));
+    ctx->r_sphere_link.upload(sl, s);
+    ctx->n_spheres = (int) d->n_spheres;
+    std::vector<float> rad(std::max<size_t>(d->n_spheres, 1), 0.f);
+    for (uint32_t i = 0; i < d->n_spheres; ++i) rad[i] = d->spheres[4 * i + 3];
+    ctx->sphere_radius.alloc(rad.size());
+    ctx->sphere_radius.upload(rad, s);
+    ctx->robot = R;
+    ctx->d_robot.alloc(1);
+    ctx->d_robot.upload(&ctx->robot, 1, s);
+    RTCD_HIP(hipStreamSynchronize(s));
+    ctx->robot_depth = depth;
+    drop_graph(ctx);
+    ctx->robot_loaded = true;
+    return RTCD_OK;
+}
+
 rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
     if (!ctx || !d) return fail(RTCD_ERR_INVALID_ARG, "null argument");
     if (d->dof < 1 || d->dof > (uint32_t) kMaxDof) return fail(RTCD_ERR_INVALID_ARG, "dof must be 1..15");
@@ -373,140 +511,68 @@ rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
     if (d->n_spheres && (!d->spheres || !d->sphere_link)) return fail(RTCD_ERR_INVALID_ARG, "null sphere arrays");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
-        RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = ctx->main.stream;
-        const int NL = (int) d->dof + 1;
-        RobotDev R{};
-        R.dof = (int) d->dof;
-        R.n_links = NL;
-        R.n_spheres = (int) d->n_spheres;
-        memcpy(R.base, d->base, sizeof(float) * 12);
-        memcpy(R.cmask, d->cos_mask, sizeof(float) * 12 * d->dof);
-        memcpy(R.smask, d->sin_mask, sizeof(float) * 12 * d->dof);
-        memcpy(R.omask, d->one_mask, sizeof(float) * 12 * d->dof);
-        if (!finite_all(R.base, 12) || !finite_all(R.cmask, 12 * d->dof) || !finite_all(R.smask, 12 * d->dof) ||
-            !finite_all(R.omask, 12 * d->dof))
-            return fail(RTCD_ERR_INVALID_ARG, "non-finite FK mask");
-        std::vector<BNode> all_tn, all_en;
-        std::vector<Tri> all_t;
-        std::vector<Edge> all_e;
-        int depth = 1;
-        for (int l = 0; l < NL; ++l) {
-            const uint32_t v0 = d->link_vert_offsets[l], v1 = d->link_vert_offsets[l + 1];
-            const uint32_t t0 = d->link_tri_offsets[l], t1 = d->link_tri_offsets[l + 1];
-            if (v1 < v0 || t1 < t0) return fail(RTCD_ERR_INVALID_ARG, "offsets must be non-decreasing");
-            const float* V = d->verts + 3 * (size_t) v0;
-            const int32_t* F = d->tris + 3 * (size_t) t0;
-            const int nv = (int) (v1 - v0), nt = (int) (t1 - t0);
-            if (!finite_all(V, 3 * (size_t) nv)) return fail(RTCD_ERR_INVALID_ARG, "non-finite link vertex");
-            for (int i = 0; i < 3 * nt; ++i)
-                if (F[i] < 0 || F[i] >= nv) return fail(RTCD_ERR_INVALID_ARG, "link triangle index out of range");
-            LinkInfo& L = R.link[l];
-            double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-            if (nv > 0)
-                for (int k = 0; k < 3; ++k) { lo[k] = V[k]; hi[k] = V[k]; }
-            for (int i = 0; i < nv; ++i)
-                for (int k = 0; k < 3; ++k) {
-                    lo[k] = std::min(lo[k], (double) V[3 * i + k]);
-                    hi[k] = std::max(hi[k], (double) V[3 * i + k]);
-                }
-            double c[3], sc = 0, rv = 0, e1 = 0, e2 = 0, dm = 0;
-            for (int k = 0; k < 3; ++k) {
-                L.lo[k] = (float) lo[k];
-                L.hi[k] = (float) hi[k];
-                L.c[k] = (float) (0.5 * (lo[k] + hi[k]));
-                c[k] = L.c[k];
-                sc = std::max(sc, std::max(std::fabs(lo[k]), std::fabs(hi[k])));
-            }
-            for (int i = 0; i < nv; ++i)
-                rv = std::max(rv, std::fabs(V[3 * i] - c[0]) + std::fabs(V[3 * i + 1] - c[1]) + std::fabs(V[3 * i + 2] - c[2]));
-            EdgeSet es = unique_edges(F, nt);
-            // triangles (exact vertices) and their boxes
-            std::vector<Tri> tri(nt);
-            std::vector<float4> tb(2 * (size_t) nt);
-            for (int f = 0; f < nt; ++f) {
-                const float* a = V + 3 * F[3 * f];
-                const float* b = V + 3 * F[3 * f + 1];
-                const float* cc = V + 3 * F[3 * f + 2];
-                tri[f].a = make_float4(a[0], a[1], a[2], bits_f((uint32_t) f));
-                tri[f].b = make_float4(b[0], b[1], b[2], bits_f(es.owned[f]));
-                tri[f].c = make_float4(cc[0], cc[1], cc[2], 0.f);
-                e1 = std::max(e1, std::fabs((double) b[0] - a[0]) + std::fabs((double) b[1] - a[1]) + std::fabs((double) b[2] - a[2]));
-                e2 = std::max(e2, std::fabs((double) cc[0] - a[0]) + std::fabs((double) cc[1] - a[1]) + std::fabs((double) cc[2] - a[2]));
-                tb[2 * f] = make_float4(std::min({a[0], b[0], cc[0]}), std::min({a[1], b[1], cc[1]}), std::min({a[2], b[2], cc[2]}), 0.f);
-                tb[2 * f + 1] = make_float4(std::max({a[0], b[0], cc[0]}), std::max({a[1], b[1], cc[1]}), std::max({a[2], b[2], cc[2]}), 0.f);
-            }
-            const int ne = (int) es.edges.size() / 2;
-            std::vector<Edge> edg(ne);
-            std::vector<float4> eb(2 * (size_t) ne);
-            for (int e = 0; e < ne; ++e) {
-                const float* p = V + 3 * es.edges[2 * e];
-                const float* q = V + 3 * es.edges[2 * e + 1];
-                edg[e].p = make_float4(p[0], p[1], p[2], bits_f((uint32_t) e));
-                edg[e].q = make_float4(q[0], q[1], q[2], 0.f);
-                dm = std::max(dm, std::fabs((double) q[0] - p[0]) + std::fabs((double) q[1] - p[1]) + std::fabs((double) q[2] - p[2]));
-                eb[2 * e] = make_float4(std::min(p[0], q[0]), std::min(p[1], q[1]), std::min(p[2], q[2]), 0.f);
-                eb[2 * e + 1] = make_float4(std::max(p[0], q[0]), std::max(p[1], q[1]), std::max(p[2], q[2]), 0.f);
-            }
-            L.scale = round_up(sc);
-            L.rv = round_up(rv * (1 + 1e-6));
-            L.e1max = round_up(e1 * (1 + 1e-6));
-            L.e2max = round_up(e2 * (1 + 1e-6));
-            L.dmax = round_up(dm * (1 + 1e-6));
-            BvhOut bt, be;
-            std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->cfg.leaf_size, s, bt);
-            std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->cfg.leaf_size, s, be);
-            L.tri_node0 = (int) all_tn.size();
-            L.tri_prim0 = (int) all_t.size();
-            L.n_tri = nt;
-            L.tri_has_root = bt.n_nodes > 0;
-            L.edge_node0 = (int) all_en.size();
-            L.edge_prim0 = (int) all_e.size();
-            L.n_edge = ne;
-            L.edge_has_root = be.n_nodes > 0;
-            std::vector<BNode> hn = bt.nodes.download(bt.n_nodes, s);
-            all_tn.insert(all_tn.end(), hn.begin(), hn.end());
-            for (int i = 0; i < nt; ++i) all_t.push_back(tri[pt[i]]);
-            hn = be.nodes.download(be.n_nodes, s);
-            all_en.insert(all_en.end(), hn.begin(), hn.end());
-            for (int i = 0; i < ne; ++i) all_e.push_back(edg[pe[i]]);
-            depth = std::max(depth, std::max(bt.depth, be.depth));
-        }
-        if (!d->n_spheres) R.n_spheres = 0;
-        std::vector<float4> sph(d->n_spheres);
-        for (uint32_t i = 0; i < d->n_spheres; ++i) {
-            if (d->sphere_link[i] < 0 || d->sphere_link[i] >= NL) return fail(RTCD_ERR_INVALID_ARG, "sphere link out of range");
-            if (!finite_all(d->spheres + 4 * i, 4) || d->spheres[4 * i + 3] < 0.f)
-                return fail(RTCD_ERR_INVALID_ARG, "invalid sphere");
-            sph[i] = make_float4(d->spheres[4 * i], d->spheres[4 * i + 1], d->spheres[4 * i + 2], d->spheres[4 * i + 3]);
-        }
-        ctx->r_tri_nodes.alloc(all_tn.size());
-        ctx->r_tri_nodes.upload(all_tn, s);
-        ctx->r_tris.alloc(all_t.size());
-        ctx->r_tris.upload(all_t, s);
-        ctx->r_edge_nodes.alloc(all_en.size());
-        ctx->r_edge_nodes.upload(all_en, s);
-        ctx->r_edges.alloc(all_e.size());
-        ctx->r_edges.upload(all_e, s);
-        ctx->r_spheres.alloc(std::max<size_t>(sph.size(), 1));
-        ctx->r_spheres.upload(sph, s);
-        std::vector<int32_t> sl(d->sphere_link, d->sphere_link + d->n_spheres);
-        ctx->r_sphere_link.alloc(std::max<size_t>(sl.size(), 1));
-        ctx->r_sphere_link.upload(sl, s);
-        ctx->n_spheres = (int) d->n_spheres;
-        std::vector<float> rad(std::max<size_t>(d->n_spheres, 1), 0.f);
-        for (uint32_t i = 0; i < d->n_spheres; ++i) rad[i] = d->spheres[4 * i + 3];
-        ctx->sphere_radius.alloc(rad.size());
-        ctx->sphere_radius.upload(rad, s);
-        ctx->robot = R;
-        ctx->d_robot.alloc(1);
-        ctx->d_robot.upload(&ctx->robot, 1, s);
-        RTCD_HIP(hipStreamSynchronize(s));
-        ctx->robot_depth = depth;
-        drop_graph(ctx);
-        ctx->robot_loaded = true;
-        return RTCD_OK;
+        rtcd_status st = load_robot_impl(ctx, d);
+        if (st == RTCD_OK) ctx->robot_copy = std::make_unique<RobotCopy>(*d);
+        return st;
     });
+}
+
+// Builds a scene's BVHs on the device (validated input); keeps host copies for rebuilds.
+static std::shared_ptr<SceneRes> build_scene(rtcd_ctx* ctx, const float* verts, uint32_t n_verts,
+                                             const int32_t* tris, uint32_t n_tris) {
+    RTCD_HIP(hipSetDevice(ctx->cfg.device));
+    hipStream_t s = ctx->main.stream;
+    auto sc = std::make_shared<SceneRes>();
+    EdgeSet es = unique_edges(tris, n_tris);
+    const int nt = (int) n_tris, ne = (int) es.edges.size() / 2;
+    std::vector<Tri> tri(nt);
+    std::vector<float4> tb(2 * (size_t) nt);
+    double scale = 0;
+    for (uint32_t i = 0; i < 3 * n_verts; ++i) scale = std::max(scale, (double) std::fabs(verts[i]));
+    for (int f = 0; f < nt; ++f) {
+        const float* a = verts + 3 * tris[3 * f];
+        const float* b = verts + 3 * tris[3 * f + 1];
+        const float* c = verts + 3 * tris[3 * f + 2];
+        tri[f].a = make_float4(a[0], a[1], a[2], bits_f((uint32_t) f));
+        tri[f].b = make_float4(b[0], b[1], b[2], bits_f(es.owned[f]));
+        tri[f].c = make_float4(c[0], c[1], c[2], 0.f);
+        tb[2 * f] = make_float4(std::min({a[0], b[0], c[0]}), std::min({a[1], b[1], c[1]}), std::min({a[2], b[2], c[2]}), 0.f);
+        tb[2 * f + 1] = make_float4(std::max({a[0], b[0], c[0]}), std::max({a[1], b[1], c[1]}), std::max({a[2], b[2], c[2]}), 0.f);
+    }
+    std::vector<Edge> edg(ne);
+    std::vector<float4> eb(2 * (size_t) ne);
+    for (int e = 0; e < ne; ++e) {
+        const float* p = verts + 3 * es.edges[2 * e];
+        const float* q = verts + 3 * es.edges[2 * e + 1];
+        edg[e].p = make_float4(p[0], p[1], p[2], bits_f((uint32_t) e));
+        edg[e].q = make_float4(q[0], q[1], q[2], 0.f);
+        eb[2 * e] = make_float4(std::min(p[0], q[0]), std::min(p[1], q[1]), std::min(p[2], q[2]), 0.f);
+        eb[2 * e + 1] = make_float4(std::max(p[0], q[0]), std::max(p[1], q[1]), std::max(p[2], q[2]), 0.f);
+    }
+    std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->cfg.leaf_size, s, sc->tri_bvh);
+    std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->cfg.leaf_size, s, sc->edge_bvh);
+    {
+        std::vector<BNode> hn = sc->tri_bvh.nodes.download(sc->tri_bvh.n_nodes, s);
+        std::vector<FEntry> fr;
+        build_frontiers(hn, fr, sc->front_levels);
+        sc->front.alloc(std::max<size_t>(fr.size(), 1));
+        if (!fr.empty()) sc->front.upload(fr, s);
+    }
+    std::vector<Tri> tri_s(nt);
+    for (int i = 0; i < nt; ++i) tri_s[i] = tri[pt[i]];
+    std::vector<Edge> edg_s(ne);
+    for (int i = 0; i < ne; ++i) edg_s[i] = edg[pe[i]];
+    sc->tris.alloc(nt);
+    sc->tris.upload(tri_s, s);
+    sc->edges.alloc(ne);
+    sc->edges.upload(edg_s, s);
+    sc->n_tri = nt;
+    sc->n_edge = ne;
+    sc->scale = (float) scale;
+    sc->h_verts.assign(verts, verts + 3 * (size_t) n_verts);
+    sc->h_tris.assign(tris, tris + 3 * (size_t) n_tris);
+    RTCD_HIP(hipStreamSynchronize(s));
+    return sc;
 }
 
 rtcd_status rtcd_scene_create(rtcd_ctx* ctx, const float* verts, uint32_t n_verts, const int32_t* tris,
@@ -518,59 +584,44 @@ rtcd_status rtcd_scene_create(rtcd_ctx* ctx, const float* verts, uint32_t n_vert
         if (tris[i] < 0 || (uint32_t) tris[i] >= n_verts) return fail(RTCD_ERR_INVALID_ARG, "scene triangle index out of range");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
-        RTCD_HIP(hipSetDevice(ctx->cfg.device));
-        hipStream_t s = ctx->main.stream;
-        auto sc = std::make_shared<SceneRes>();
-        EdgeSet es = unique_edges(tris, n_tris);
-        const int nt = (int) n_tris, ne = (int) es.edges.size() / 2;
-        std::vector<Tri> tri(nt);
-        std::vector<float4> tb(2 * (size_t) nt);
-        double scale = 0;
-        for (uint32_t i = 0; i < 3 * n_verts; ++i) scale = std::max(scale, (double) std::fabs(verts[i]));
-        for (int f = 0; f < nt; ++f) {
-            const float* a = verts + 3 * tris[3 * f];
-            const float* b = verts + 3 * tris[3 * f + 1];
-            const float* c = verts + 3 * tris[3 * f + 2];
-            tri[f].a = make_float4(a[0], a[1], a[2], bits_f((uint32_t) f));
-            tri[f].b = make_float4(b[0], b[1], b[2], bits_f(es.owned[f]));
-            tri[f].c = make_float4(c[0], c[1], c[2], 0.f);
-            tb[2 * f] = make_float4(std::min({a[0], b[0], c[0]}), std::min({a[1], b[1], c[1]}), std::min({a[2], b[2], c[2]}), 0.f);
-            tb[2 * f + 1] = make_float4(std::max({a[0], b[0], c[0]}), std::max({a[1], b[1], c[1]}), std::max({a[2], b[2], c[2]}), 0.f);
-        }
-        std::vector<Edge> edg(ne);
-        std::vector<float4> eb(2 * (size_t) ne);
-        for (int e = 0; e < ne; ++e) {
-            const float* p = verts + 3 * es.edges[2 * e];
-            const float* q = verts + 3 * es.edges[2 * e + 1];
-            edg[e].p = make_float4(p[0], p[1], p[2], bits_f((uint32_t) e));
-            edg[e].q = make_float4(q[0], q[1], q[2], 0.f);
-            eb[2 * e] = make_float4(std::min(p[0], q[0]), std::min(p[1], q[1]), std::min(p[2], q[2]), 0.f);
-            eb[2 * e + 1] = make_float4(std::max(p[0], q[0]), std::max(p[1], q[1]), std::max(p[2], q[2]), 0.f);
-        }
-        std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->cfg.leaf_size, s, sc->tri_bvh);
-        std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->cfg.leaf_size, s, sc->edge_bvh);
-        {
-            std::vector<BNode> hn = sc->tri_bvh.nodes.download(sc->tri_bvh.n_nodes, s);
-            std::vector<FEntry> fr;
-            build_frontiers(hn, fr, sc->front_levels);
-            sc->front.alloc(std::max<size_t>(fr.size(), 1));
-            if (!fr.empty()) sc->front.upload(fr, s);
-        }
-        std::vector<Tri> tri_s(nt);
-        for (int i = 0; i < nt; ++i) tri_s[i] = tri[pt[i]];
-        std::vector<Edge> edg_s(ne);
-        for (int i = 0; i < ne; ++i) edg_s[i] = edg[pe[i]];
-        sc->tris.alloc(nt);
-        sc->tris.upload(tri_s, s);
-        sc->edges.alloc(ne);
-        sc->edges.upload(edg_s, s);
-        sc->n_tri = nt;
-        sc->n_edge = ne;
-        sc->scale = (float) scale;
-        RTCD_HIP(hipStreamSynchronize(s));
+        auto sc = build_scene(ctx, verts, n_verts, tris, n_tris);
         uint32_t id = ctx->next_scene++;
         ctx->scenes[id] = std::move(sc);
         *scene_id = id;
+        return RTCD_OK;
+    });
+}
+
+}  // extern "C"
+
+namespace rtcd {
+// Rebuilds the robot and every scene with ctx->cfg.leaf_size (scene ids are kept).
+void rebuild_bvhs(rtcd_ctx* ctx) {
+    drop_graph(ctx);
+    if (ctx->robot_copy) {
+        const rtcd_robot_desc d = ctx->robot_copy->desc();
+        if (load_robot_impl(ctx, &d) != RTCD_OK) throw std::runtime_error("robot rebuild failed: " + g_err);
+    }
+    for (auto& kv : ctx->scenes) {
+        std::shared_ptr<SceneRes> old = kv.second;
+        kv.second = build_scene(ctx, old->h_verts.data(), (uint32_t) (old->h_verts.size() / 3), old->h_tris.data(),
+                                (uint32_t) (old->h_tris.size() / 3));
+    }
+}
+}  // namespace rtcd
+
+extern "C" {
+
+rtcd_status rtcd_set_leaf_size(rtcd_ctx* ctx, uint32_t leaf_size) {
+    if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
+    if (leaf_size < 1 || leaf_size > 16) return fail(RTCD_ERR_INVALID_ARG, "leaf_size must be 1..16");
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    return guarded([&]() -> rtcd_status {
+        RTCD_HIP(hipSetDevice(ctx->cfg.device));
+        if (ctx->serve) return fail(RTCD_ERR_NOT_READY, "stop the serving loop before rebuilding");
+        RTCD_HIP(hipStreamSynchronize(ctx->main.stream));
+        ctx->cfg.leaf_size = leaf_size;
+        rebuild_bvhs(ctx);
         return RTCD_OK;
     });
 }
@@ -716,6 +767,10 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
         (param == RTCD_PARAM_BROAD_BUDGET ? ctx->broad_budget
          : param == RTCD_PARAM_BROAD_BUDGET_SMALL ? ctx->broad_budget_small : ctx->broad_budget_next) = (uint32_t) value;
         return RTCD_OK;
+    case RTCD_PARAM_NARROW_BLOCK:
+        if (value != 64 && value != 128 && value != 256) return fail(RTCD_ERR_INVALID_ARG, "narrow block must be 64, 128 or 256");
+        ctx->narrow_bs = (uint32_t) value;
+        return RTCD_OK;
     case RTCD_PARAM_GRAPHS:
         ctx->use_graphs = value != 0;
         if (!ctx->use_graphs) drop_graph(ctx);
@@ -743,6 +798,7 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_BROAD_PASSES: *value = ctx->broad_passes; return RTCD_OK;
     case RTCD_PARAM_SWEEP_PASSES: *value = ctx->sweep_passes; return RTCD_OK;
     case RTCD_PARAM_GRAPHS: *value = ctx->use_graphs ? 1 : 0; return RTCD_OK;
+    case RTCD_PARAM_NARROW_BLOCK: *value = ctx->narrow_bs; return RTCD_OK;
     default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
 }

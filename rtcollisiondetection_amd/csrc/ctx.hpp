@@ -52,6 +52,8 @@ inline std::vector<int32_t> build_bvh(const std::vector<float4>& boxes, int n, i
 }
 
 struct SceneRes {
+    std::vector<float> h_verts;                   // host copy of the input (rebuilds)
+    std::vector<int32_t> h_tris;
     BvhOut tri_bvh, edge_bvh;
     DevBuf<FEntry> front;                         // frontier levels of the triangle BVH, concatenated
     std::vector<std::pair<int, int>> front_levels; // (offset, size): sizes 2, 4, ... up to 1024
@@ -131,6 +133,45 @@ struct Workspace {
     }
 };
 
+// Host copy of a robot description, so that its BVHs can be rebuilt with another leaf size.
+struct RobotCopy {
+    uint32_t dof = 0, n_spheres = 0;
+    std::vector<float> base, cmask, smask, omask, verts, spheres;
+    std::vector<uint32_t> vofs, tofs;
+    std::vector<int32_t> tris, sphere_link;
+    explicit RobotCopy(const rtcd_robot_desc& d) : dof(d.dof), n_spheres(d.n_spheres) {
+        const size_t NL = d.dof + 1;
+        base.assign(d.base, d.base + 12);
+        cmask.assign(d.cos_mask, d.cos_mask + 12 * (size_t) d.dof);
+        smask.assign(d.sin_mask, d.sin_mask + 12 * (size_t) d.dof);
+        omask.assign(d.one_mask, d.one_mask + 12 * (size_t) d.dof);
+        vofs.assign(d.link_vert_offsets, d.link_vert_offsets + NL + 1);
+        tofs.assign(d.link_tri_offsets, d.link_tri_offsets + NL + 1);
+        verts.assign(d.verts, d.verts + 3 * (size_t) vofs[NL]);
+        tris.assign(d.tris, d.tris + 3 * (size_t) tofs[NL]);
+        if (n_spheres) {
+            spheres.assign(d.spheres, d.spheres + 4 * (size_t) n_spheres);
+            sphere_link.assign(d.sphere_link, d.sphere_link + n_spheres);
+        }
+    }
+    rtcd_robot_desc desc() const {
+        rtcd_robot_desc d{};
+        d.dof = dof;
+        d.base = base.data();
+        d.cos_mask = cmask.data();
+        d.sin_mask = smask.data();
+        d.one_mask = omask.data();
+        d.verts = verts.data();
+        d.link_vert_offsets = vofs.data();
+        d.tris = tris.data();
+        d.link_tri_offsets = tofs.data();
+        d.spheres = n_spheres ? spheres.data() : nullptr;
+        d.sphere_link = n_spheres ? sphere_link.data() : nullptr;
+        d.n_spheres = n_spheres;
+        return d;
+    }
+};
+
 }  // namespace rtcd
 
 using namespace rtcd;
@@ -141,6 +182,7 @@ struct rtcd_ctx {
     // robot
     bool robot_loaded = false;
     RobotDev robot{};
+    std::unique_ptr<RobotCopy> robot_copy;
     DevBuf<RobotDev> d_robot;
     DevBuf<BNode> r_tri_nodes, r_edge_nodes;
     DevBuf<Tri> r_tris;
@@ -157,6 +199,7 @@ struct rtcd_ctx {
     uint32_t broad_budget_next = 16;            // the same in the passes after the first
     uint32_t broad_passes = 12;                 // broad hand-off passes (the last never hands off)
     uint32_t sweep_passes = 8;                  // sweep hand-off passes
+    uint32_t narrow_bs = 256;                   // work-group size of the discrete narrow passes
     int n_spheres = 0;
     int robot_depth = 1;
     // scenes (shared so that in-flight served requests keep a scene alive)
@@ -217,6 +260,8 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
 void enqueue_discrete_main(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, uint32_t n, uint32_t* d_out);
 // Destroys the captured graph (scene or robot changes, context teardown).
 void drop_graph(rtcd_ctx* c);
+// Rebuilds the robot and every scene with ctx->cfg.leaf_size (scene ids are kept).
+void rebuild_bvhs(rtcd_ctx* c);
 // After ws.stream finished: re-runs on candidate overflow; fills *st (timing, counters) if non-null.
 void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
                      uint32_t* d_out, size_t budget, rtcd_stats* st);

@@ -313,7 +313,7 @@ __device__ __forceinline__ void defer(const DiscreteArgs& a, uint32_t key, uint3
 
 // Obstacle edge j of scene triangle `tri` (link-frame origin P with abs. error <= delta per
 // component, direction d = R^T (Q - P) with rel. error <= 6 eps) against the link-triangle LBVH.
-template <int STACK, bool COUNT>
+template <int STACK, int BS, bool COUNT>
 __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, uint32_t j, float3 P,
                         float3 d, float delta, int* st, const uint32_t* flag, Ctr& ctr) {
     if (!L.tri_has_root) return 0;
@@ -356,7 +356,7 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, 
         }
         if (h0 && h1) {
             if (sp >= STACK) { atomicOr(a.overflow, 2u); return 0; }
-            st[sp * kBS] = nd.c1;
+            st[sp * BS] = nd.c1;
             ++sp;
             ni = nd.c0;
         } else if (h0) {
@@ -366,14 +366,14 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, 
         } else {
             if (sp == 0) return 0;
             --sp;
-            ni = st[sp * kBS];
+            ni = st[sp * BS];
         }
     }
 }
 
 // Scene triangle (link frame A with abs. error <= delta, e1 = R^T (B - A), e2 = R^T (C - A) with
 // rel. error <= 6 eps) against the link-edge LBVH.
-template <int STACK, bool COUNT>
+template <int STACK, int BS, bool COUNT>
 __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, float3 A, float3 e1,
                        float3 e2, float delta, int* st, const uint32_t* flag, Ctr& ctr) {
     if (!L.edge_has_root) return 0;
@@ -439,7 +439,7 @@ __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, u
         }
         if (h0 && h1) {
             if (sp >= STACK) { atomicOr(a.overflow, 2u); return 0; }
-            st[sp * kBS] = nd.c1;
+            st[sp * BS] = nd.c1;
             ++sp;
             ni = nd.c0;
         } else if (h0) {
@@ -449,22 +449,22 @@ __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, u
         } else {
             if (sp == 0) return 0;
             --sp;
-            ni = st[sp * kBS];
+            ni = st[sp * BS];
         }
     }
 }
 
 // One direction over all candidates (grid-stride).  DIR 1: obstacle->robot, 2: robot->obstacle.
-template <int DIR, int STACK, bool COUNT>
-__global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
-    __shared__ int stk[STACK * kBS];
+template <int DIR, int STACK, int BS, bool COUNT>
+__global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a) {
+    __shared__ int stk[STACK * BS];
     int* st = stk + threadIdx.x;
     // each leaf record is spread over `lanes` adjacent threads, one triangle per thread
     const uint32_t lanes = (uint32_t) a.leaf_lanes;
     const uint64_t total = (uint64_t) a.seg_pre[kSeg] * lanes;
-    const uint32_t stride = gridDim.x * kBS;
+    const uint32_t stride = gridDim.x * BS;
     Ctr ctr;
-    for (uint64_t v = blockIdx.x * kBS + threadIdx.x; v < total; v += stride) {
+    for (uint64_t v = blockIdx.x * BS + threadIdx.x; v < total; v += stride) {
         const uint32_t i = (uint32_t) (v / lanes), sub_k = (uint32_t) (v - (uint64_t) i * lanes);
         const uint2 c = a.flat[i];
         if (sub_k > leaf_last(c.y)) continue;
@@ -483,14 +483,14 @@ __global__ __launch_bounds__(kBS) void k_narrow(DiscreteArgs a) {
             if (COUNT) ctr.tris++;
             if (DIR == 1) {
                 const uint32_t owned = __float_as_uint(t.b.w);
-                if (owned & 1u) hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 0u, A, e1, delta, st, flag, ctr);
+                if (owned & 1u) hit = o2r_edge<STACK, BS, COUNT>(a, L, c.x, tri, 0u, A, e1, delta, st, flag, ctr);
                 if (!hit && (owned & 2u))
-                    hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 1u, add(A, e1),
+                    hit = o2r_edge<STACK, BS, COUNT>(a, L, c.x, tri, 1u, add(A, e1),
                                                  rot(r0, r1, r2, sub(f3(t.c), f3(t.b))), delta, st, flag, ctr);
                 if (!hit && (owned & 4u))
-                    hit = o2r_edge<STACK, COUNT>(a, L, c.x, tri, 2u, add(A, e2), neg(e2), delta, st, flag, ctr);
+                    hit = o2r_edge<STACK, BS, COUNT>(a, L, c.x, tri, 2u, add(A, e2), neg(e2), delta, st, flag, ctr);
             } else {
-                hit = r2o_tri<STACK, COUNT>(a, L, c.x, tri, A, e1, e2, delta, st, flag, ctr);
+                hit = r2o_tri<STACK, BS, COUNT>(a, L, c.x, tri, A, e1, e2, delta, st, flag, ctr);
             }
         }
         if (hit) atomicOr(flag, 1u);
@@ -577,12 +577,20 @@ void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s) {
     else throw std::runtime_error("BVH deeper than 64 levels");
 }
 
+template <int DIR, int STACK, int BS>
+static void narrow_bs(const DiscreteArgs& a, bool count, hipStream_t s) {
+    // as many threads as 8 blocks of 256 per CU, whatever the block size
+    const unsigned nb = (unsigned) device_cus() * 8u * (256u / BS);
+    if (count) hipLaunchKernelGGL((k_narrow<DIR, STACK, BS, true>), dim3(nb), dim3(BS), 0, s, a);
+    else hipLaunchKernelGGL((k_narrow<DIR, STACK, BS, false>), dim3(nb), dim3(BS), 0, s, a);
+    RTCD_LAUNCH_CHECK();
+}
+
 template <int DIR, int STACK>
 static void narrow_t(const DiscreteArgs& a, bool count, hipStream_t s) {
-    const unsigned nb = (unsigned) device_cus() * 8u;
-    if (count) hipLaunchKernelGGL((k_narrow<DIR, STACK, true>), dim3(nb), dim3(kBS), 0, s, a);
-    else hipLaunchKernelGGL((k_narrow<DIR, STACK, false>), dim3(nb), dim3(kBS), 0, s, a);
-    RTCD_LAUNCH_CHECK();
+    if (a.narrow_bs == 64) narrow_bs<DIR, STACK, 64>(a, count, s);
+    else if (a.narrow_bs == 128) narrow_bs<DIR, STACK, 128>(a, count, s);
+    else narrow_bs<DIR, STACK, 256>(a, count, s);
 }
 
 template <int DIR>

@@ -66,6 +66,7 @@ struct DiscreteArgs {
     uint32_t budget;       // node visits before a pass-0 walk hands off its pending subtrees
     uint32_t budget_next;  // the same in later passes
     int n_passes;          // broad passes (<= kMaxPasses); the last never hands off
+    int narrow_bs;         // work-group size of the narrow passes: 64, 128 or 256
 };
 // Most hand-off passes of one broad phase or sweep: the first walks every task, each later one the
 // subtrees handed off by the previous pass.

@@ -123,6 +123,27 @@ class Context:
         """Set a launch-shape parameter (see rtcd_param in include/rtcd.h)."""
         check(lib().rtcd_set_param(self.h, self.PARAMS[name], int(value)))
 
+    def set_leaf_size(self, leaf_size):
+        """Rebuild the robot's and every scene's BVHs with another leaf size (scene ids are kept)."""
+        check(lib().rtcd_set_leaf_size(self.h, int(leaf_size)))
+
+    def autotune(self, kind, sid, data, offsets=None, direction=_lib.TWO_WAY, n_wp=16, n_ctrl=8, leaf_size=True):
+        """Tune the launch shapes on a sample (kind: _lib.REQ_DISCRETE / REQ_LINEAR / REQ_QUADRATIC) and
+        keep the fastest; returns the tuner's report as a dict."""
+        data = _f32(data)
+        offs = np.ascontiguousarray(offsets, dtype=np.uint32) if offsets is not None else None
+        if kind == _lib.REQ_LINEAR:
+            n = len(offs) - 1
+        else:
+            n = data.shape[0]
+            if kind == _lib.REQ_QUADRATIC:
+                n_wp = data.shape[1]
+        r = _lib.Request(kind, direction, sid, n, data.ctypes.data_as(C.c_void_p),
+                         offs.ctypes.data_as(C.c_void_p) if offs is not None else None, n_wp, n_ctrl, None)
+        rep = _lib.TuneReport()
+        check(lib().rtcd_autotune(self.h, C.byref(r), _lib.TUNE_LEAF_SIZE if leaf_size else 0, C.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in rep._fields_}
+
     def get_param(self, name):
         v = C.c_uint64()
         check(lib().rtcd_get_param(self.h, self.PARAMS[name], C.byref(v)))
