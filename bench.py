@@ -82,6 +82,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=2)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-extra', action='store_true')
+    ap.add_argument('--tune', action='store_true',
+                    help='run the library autotuner on the workload first (untimed) and time the tuned shapes')
     ap.add_argument('--poses', type=int, default=65536)
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run a plain 256 MiB device-to-device copy 5 times (reference for HBM counters)')
@@ -112,7 +114,15 @@ def main():
     q = samplers.uniform_poses(n, rb.lower, rb.upper, seed=1)
     dq = torch.from_numpy(q).cuda()
     dout = torch.zeros(n, dtype=torch.int32, device='cuda')
-    stream = torch.cuda.current_stream()
+    tuning = None
+    if args.tune:
+        # untimed: the library's autotuner picks the leaf size, narrow work-group size and broad-phase
+        # hand-off shape for this workload (every configuration it times must give the same flags)
+        from rtcollisiondetection_amd import _lib
+        t = time.perf_counter()
+        tuning = ctx.autotune(_lib.REQ_DISCRETE, sid, q, direction=3, leaf_size=True)
+        tuning['tune_s'] = round(time.perf_counter() - t, 2)
+        log(f'[bench] autotune {tuning}')
 
     def run(direction, dposes=dq, dres=dout):
         ctx.check_discrete_device(sid, dposes, dres, direction)
@@ -147,7 +157,7 @@ def main():
     run(3)
     c3 = ctx.stats()
     ctx.set_counting(False)
-    lanes = 4  # threads per leaf record = the scene BVH leaf size (the context default)
+    lanes = tuning['leaf_size'] if tuning else 4  # threads per leaf record = the scene BVH leaf size
     rec_lanes = lanes * c1['leaf_candidates']
     k1 = {'kernel': 'k_narrow<1> (obstacle->robot narrow pass)', 'ms': float(stage[3]),
           'bytes': (S_REC * rec_lanes + S_EVAL * c1['narrow_candidates'] + S_NODE * c1['node_fetches'] +
@@ -194,15 +204,28 @@ def main():
             for _ in range(3):
                 run(d)
             extra[f'{name}_poses_per_s'] = round(n * 3 / (time.perf_counter() - t), 1)
-        # plain per-query launch: one pose per blocking call (host buffers), 256 queries
+        # plain per-query launch: one pose per blocking call (host buffers), eager launches, 256 queries
         qh = q[:256]
+        ctx.set_param('graphs', 0)
         ctx.check_discrete(sid, qh[:1], 3)
         t = time.perf_counter()
         for i in range(256):
             ctx.check_discrete(sid, qh[i:i + 1], 3)
         plain = 256 / (time.perf_counter() - t)
+        ctx.set_param('graphs', 1)
         extra['plain_per_query_poses_per_s'] = round(plain, 1)
         extra['batched_over_plain'] = round(value / plain, 1)
+        # repeated small checks replay a captured hipGraph: device-API latency (submit -> flags ready)
+        for nn in (1, 1024):
+            dqs, dos = dq[:nn].contiguous(), torch.zeros(nn, dtype=torch.int32, device='cuda')
+            lat = []
+            for _ in range(22):
+                t = time.perf_counter()
+                ctx.check_discrete_device(sid, dqs, dos, 3)
+                ctx.sync()
+                lat.append(time.perf_counter() - t)
+            extra[f'graph_latency_ms_n{nn}'] = round(float(np.median(lat[2:])) * 1e3, 3)
+            extra[f'graph_replayed_n{nn}'] = ctx.stats()['graph_replayed']
         log(f'[bench] extra discrete {extra}')
         # swept volumes on the shelf scene
         shelf = scenes.scene_shelf()
@@ -221,6 +244,17 @@ def main():
             ctx.check_quadratic(sid2, wp, 8)
         extra['quadratic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
         extra['quadratic_ms_device'] = round(ctx.stats()['last_ms'], 3)
+        # the same trajectory sets on the dense scene
+        ctx.check_linear(sid, traj, offs)
+        t = time.perf_counter()
+        for _ in range(3):
+            ctx.check_linear(sid, traj, offs)
+        extra['dense_linear_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
+        ctx.check_quadratic(sid, wp, 8)
+        t = time.perf_counter()
+        for _ in range(3):
+            ctx.check_quadratic(sid, wp, 8)
+        extra['dense_quadratic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
         log(f'[bench] extra sweeps {extra}')
         extra['serving'] = serve_bench(ctx, rb, [ctx.create_scene(*_boxes8()), sid2, sid])
         log(f'[bench] serving {extra["serving"]}')
@@ -253,7 +287,7 @@ def main():
                                'shelf lattice + 50k-triangle icosphere blob (50,768 tris, 76,152 edges)',
                    'poses': n, 'scene_tris': 50768, 'scene_edges': 76152, 'direction': 'two-way',
                    'collisions': hits},
-        'roofline': roofline, 'cpu_baseline': cpu, 'extra': extra,
+        'roofline': roofline, 'cpu_baseline': cpu, 'tuning': tuning, 'extra': extra,
     }
     print(json.dumps(out), flush=True)
 

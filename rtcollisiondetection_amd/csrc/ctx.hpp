@@ -199,7 +199,7 @@ struct rtcd_ctx {
     uint32_t broad_budget_next = 16;            // the same in the passes after the first
     uint32_t broad_passes = 12;                 // broad hand-off passes (the last never hands off)
     uint32_t sweep_passes = 8;                  // sweep hand-off passes
-    uint32_t narrow_bs = 256;                   // work-group size of the discrete narrow passes
+    uint32_t narrow_bs = 64;                    // work-group size of the discrete narrow passes
     int n_spheres = 0;
     int robot_depth = 1;
     // scenes (shared so that in-flight served requests keep a scene alive)
