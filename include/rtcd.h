@@ -154,11 +154,14 @@ typedef struct {
     uint64_t completed, failed;
     double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> flags in the caller's buffer */
     double span_ms;                         /* first submit -> last completion */
-    uint32_t queued_max;
+    uint32_t queued_max;                    /* most work units queued at once */
 } rtcd_serve_stats;
-/* Starts the worker: `slots` concurrent requests (each with its own stream and buffers), admitted
- * while their estimated device bytes fit `mem_budget` (0 = 8 GiB).  input/offsets are copied at
- * launch; the caller keeps them and `out` alive until rtcd_wait returns. */
+/* Starts the worker with `slots` (1..16) concurrent work units, each slot with its own stream and
+ * buffers.  A request is split into units of at most 4,096 poses or 1,024 trajectories; a free slot
+ * takes the queued unit of the smallest request (FIFO among equal sizes) unless a unit has waited
+ * more than 20 ms, and only while the units' estimated device bytes fit `mem_budget` (0 = 8 GiB).
+ * input/offsets are read when a unit launches: the caller keeps them and `out` alive until
+ * rtcd_wait returns.  A request whose unit alone exceeds the budget is rejected with RTCD_ERR_OOM. */
 rtcd_status rtcd_serve_start(rtcd_ctx* ctx, uint32_t slots, uint64_t mem_budget);
 rtcd_status rtcd_serve_stop(rtcd_ctx* ctx);
 rtcd_status rtcd_submit(rtcd_ctx* ctx, const rtcd_request* req, uint64_t* id);

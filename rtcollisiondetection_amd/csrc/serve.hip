@@ -3,11 +3,17 @@
 // RTCD has no serving layer; its closest analogue is the NSTREAM ThreadPool in
 // CollisionDetector::check (RTCD/CollisionDetector/CollisionDetector.h:396-433, threadPool.h:10-92),
 // which launches fixed batches on round-robin streams and joins every NSTREAM tasks.  Here a worker
-// thread owns `slots` workspaces, each on its own stream.  Requests are admitted while a slot is
-// free and their estimated device bytes fit the memory budget; a request that does not fit may be
-// overtaken by smaller ones for at most kBypassMs, so small requests are not blocked behind large
-// ones.  Admitted requests run concurrently on the device; completion is detected with events, the
-// flags are copied to the caller's buffer, and submit->done latency is recorded.
+// thread owns `slots` workspaces, each on its own stream, and runs work units on them:
+//  * a request is split into units of at most kChunkDiscrete poses or kChunkSweep trajectories
+//    (queries are independent), so no unit holds a slot for long;
+//  * a free slot takes the queued unit of the smallest size class (log2 of the request size,
+//    FIFO within a class), unless some unit has waited longer than kAgingMs, which then goes
+//    first (except on slot 0, which always serves the smallest class), and batches with it the queued units of the same kind, scene and shape (FIFO) up to the
+//    chunk size: many small requests become one launch (continuous batching);
+//  * a batch is admitted only while its estimated device bytes fit the memory budget;
+//  * admitted batches run concurrently on the device; completion is detected with events, each
+//    unit's flags are copied into its range of the caller's buffer, and a request completes (its
+//    submit->done latency recorded) when its last unit does.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -19,7 +25,9 @@
 namespace rtcd {
 
 namespace {
-constexpr double kBypassMs = 5.0;
+constexpr double kAgingMs = 20.0;
+constexpr uint32_t kChunkDiscrete = 4096;
+constexpr uint32_t kChunkSweep = 1024;
 
 double now_ms() {
     using namespace std::chrono;
@@ -47,18 +55,36 @@ struct Req {
     uint64_t id = 0;
     rtcd_request r{};
     std::shared_ptr<SceneRes> scene;
-    size_t bytes = 0;
-    double t_submit = 0, t_start = 0, t_done = 0;
+    double t_submit = 0, t_done = 0;
     rtcd_status status = RTCD_OK;
     std::string err;
     bool done = false;
-    uint32_t n_wp_total = 0;
+    uint32_t units_left = 0;
+};
+
+// A contiguous range [i0, i0 + n) of a request's queries.
+struct Unit {
+    Req* q = nullptr;
+    uint32_t i0 = 0, n = 0;
+    uint32_t nw = 0;         // waypoints of the range (linear: sum of its lengths; quadratic: n * n_wp)
+    size_t bytes = 0;
+    int cls = 0;             // size class of the request: admission order
+};
+
+// Units launched together: same kind, scene and shape; unit k covers batch queries
+// [first[k], first[k] + units[k].n).
+struct Batch {
+    std::vector<Unit> units;
+    std::vector<uint32_t> first;
+    uint32_t n = 0, nw = 0;
+    size_t bytes = 0;
 };
 
 struct Slot {
     Workspace ws;
     hipEvent_t done_ev = nullptr;
-    Req* cur = nullptr;
+    Batch cur;
+    bool busy = false;
     PinnedBuf<float> in;
     PinnedBuf<uint32_t> offs;
     PinnedBuf<uint32_t> out;
@@ -71,7 +97,7 @@ struct ServeState {
     std::thread worker;
     std::mutex m;
     std::condition_variable cv_work, cv_done;
-    std::deque<Req*> queue;
+    std::deque<Unit> queue;
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
     uint64_t next_id = 1;
     bool stop = false;
@@ -84,75 +110,168 @@ struct ServeState {
 
 namespace {
 
-size_t request_bytes(const rtcd_ctx* c, const rtcd_request& r, uint32_t nw) {
-    const size_t staging = (size_t) nw * c->robot.dof * 4 + (size_t) r.n * 4 * (r.kind == RTCD_REQ_QUADRATIC ? 1 : 1);
-    if (r.kind == RTCD_REQ_DISCRETE) return discrete_bytes(c, r.n) + staging;
-    if (r.kind == RTCD_REQ_LINEAR) return sweep_bytes(c, r.n, nw, 0) + staging;
-    return sweep_bytes(c, r.n, nw, r.n_ctrl) + staging;
+size_t unit_bytes(const rtcd_ctx* c, const rtcd_request& r, uint32_t n, uint32_t nw) {
+    const size_t staging = (size_t) nw * c->robot.dof * 4 + (size_t) n * 4;
+    if (r.kind == RTCD_REQ_DISCRETE) return discrete_bytes(c, n) + staging;
+    if (r.kind == RTCD_REQ_LINEAR) return sweep_bytes(c, n, nw, 0) + staging;
+    return sweep_bytes(c, n, nw, r.n_ctrl) + staging;
 }
 
-// Launch request q on slot sl (worker thread, no lock held).
-void launch(ServeState* S, Slot& sl, Req* q) {
+// First input waypoint of a unit's range in its request's input.
+size_t first_waypoint(const Unit& u) {
+    const rtcd_request& r = u.q->r;
+    if (r.kind == RTCD_REQ_DISCRETE) return u.i0;
+    if (r.kind == RTCD_REQ_LINEAR) return r.offsets[u.i0];
+    return (size_t) u.i0 * r.n_wp;
+}
+
+// Launch batch b on slot sl (worker thread, no lock held): stage the units' inputs one after the
+// other, enqueue one check over the whole batch.
+void launch(ServeState* S, Slot& sl, const Batch& b) {
     rtcd_ctx* c = S->ctx;
-    const rtcd_request& r = q->r;
+    const rtcd_request& r = b.units[0].q->r;  // kind, scene and shape are common to the batch
     Workspace& ws = sl.ws;
     const hipStream_t s = ws.stream;
     const uint32_t dof = (uint32_t) c->robot.dof;
-    const size_t nin = (size_t) q->n_wp_total * dof;
+    const size_t nin = (size_t) b.nw * dof;
     sl.in.reserve(nin);
-    std::memcpy(sl.in.p, r.input, nin * sizeof(float));
+    size_t at = 0;
+    for (const Unit& u : b.units) {
+        std::memcpy(sl.in.p + at, u.q->r.input + first_waypoint(u) * dof, (size_t) u.nw * dof * sizeof(float));
+        at += (size_t) u.nw * dof;
+    }
     ws.h_poses.reserve(nin);
     RTCD_HIP(hipMemcpyAsync(ws.h_poses.p, sl.in.p, nin * sizeof(float), hipMemcpyHostToDevice, s));
-    ws.h_out.reserve(r.n);
+    ws.h_out.reserve(b.n);
+    SceneRes& sc = *b.units[0].q->scene;
     if (r.kind == RTCD_REQ_DISCRETE) {
-        enqueue_discrete(c, ws, *q->scene, r.direction, ws.h_poses.p, r.n, ws.h_out.p, 0);
+        enqueue_discrete(c, ws, sc, r.direction, ws.h_poses.p, b.n, ws.h_out.p, 0);
     } else if (r.kind == RTCD_REQ_LINEAR) {
-        sl.offs.reserve(r.n + 1);
-        std::memcpy(sl.offs.p, r.offsets, sizeof(uint32_t) * (r.n + 1));
-        ws.h_offsets.reserve(r.n + 1);
-        RTCD_HIP(hipMemcpyAsync(ws.h_offsets.p, sl.offs.p, sizeof(uint32_t) * (r.n + 1), hipMemcpyHostToDevice, s));
-        enqueue_linear(c, ws, *q->scene, ws.h_poses.p, ws.h_offsets.p, r.n, q->n_wp_total, ws.h_out.p, nullptr, 0);
+        sl.offs.reserve(b.n + 1);
+        uint32_t k = 0, base = 0;
+        for (const Unit& u : b.units) {
+            const uint32_t* o = u.q->r.offsets + u.i0;
+            for (uint32_t i = 0; i < u.n; ++i) sl.offs.p[k++] = base + (o[i] - o[0]);
+            base += u.nw;
+        }
+        sl.offs.p[k] = base;
+        ws.h_offsets.reserve(b.n + 1);
+        RTCD_HIP(hipMemcpyAsync(ws.h_offsets.p, sl.offs.p, sizeof(uint32_t) * (b.n + 1), hipMemcpyHostToDevice, s));
+        enqueue_linear(c, ws, sc, ws.h_poses.p, ws.h_offsets.p, b.n, b.nw, ws.h_out.p, nullptr, 0);
     } else {
-        enqueue_quadratic(c, ws, *q->scene, ws.h_poses.p, r.n, r.n_wp, r.n_ctrl, nullptr, ws.h_out.p, nullptr, 0);
+        enqueue_quadratic(c, ws, sc, ws.h_poses.p, b.n, r.n_wp, r.n_ctrl, nullptr, ws.h_out.p, nullptr, 0);
     }
-    sl.out.reserve(r.n);
-    RTCD_HIP(hipMemcpyAsync(sl.out.p, ws.h_out.p, sizeof(uint32_t) * r.n, hipMemcpyDeviceToHost, s));
+    sl.out.reserve(b.n);
+    RTCD_HIP(hipMemcpyAsync(sl.out.p, ws.h_out.p, sizeof(uint32_t) * b.n, hipMemcpyDeviceToHost, s));
     RTCD_HIP(hipEventRecord(sl.done_ev, s));
 }
 
-// Completion of the request on slot sl (its done event has fired).
+// Completion of the batch on slot sl (its done event has fired): re-run on overflow, scatter the
+// flags into each unit's range of its caller's buffer.
 void complete(ServeState* S, Slot& sl) {
-    Req* q = sl.cur;
+    const Batch& b = sl.cur;
     rtcd_ctx* c = S->ctx;
-    const rtcd_request& r = q->r;
+    const rtcd_request& r = b.units[0].q->r;
+    SceneRes& sc = *b.units[0].q->scene;
     if (r.kind == RTCD_REQ_DISCRETE) {
-        // re-runs synchronously on the slot stream if the candidate buffer overflowed
+        // re-runs synchronously on the slot stream if a candidate segment or the FP64 queue overflowed
         std::vector<uint32_t> sv = sl.ws.scalars.download(2, sl.ws.stream);
         if (sv[1]) {
-            finish_discrete(c, sl.ws, *q->scene, r.direction, sl.ws.h_poses.p, r.n, sl.ws.h_out.p, 0, nullptr);
-            RTCD_HIP(hipMemcpyAsync(sl.out.p, sl.ws.h_out.p, sizeof(uint32_t) * r.n, hipMemcpyDeviceToHost,
+            finish_discrete(c, sl.ws, sc, r.direction, sl.ws.h_poses.p, b.n, sl.ws.h_out.p, 0, nullptr);
+            RTCD_HIP(hipMemcpyAsync(sl.out.p, sl.ws.h_out.p, sizeof(uint32_t) * b.n, hipMemcpyDeviceToHost,
                                     sl.ws.stream));
             RTCD_HIP(hipStreamSynchronize(sl.ws.stream));
         }
     } else {
-        // re-runs synchronously if a candidate segment or the FP64 queue overflowed
         std::vector<uint32_t> sv = sl.ws.scalars.download(3, sl.ws.stream);
         if (sv[1] & 7u) {
             auto rerun = [&]() {
                 if (r.kind == RTCD_REQ_LINEAR)
-                    enqueue_linear(c, sl.ws, *q->scene, sl.ws.h_poses.p, sl.ws.h_offsets.p, r.n, q->n_wp_total,
-                                   sl.ws.h_out.p, nullptr, 0);
+                    enqueue_linear(c, sl.ws, sc, sl.ws.h_poses.p, sl.ws.h_offsets.p, b.n, b.nw, sl.ws.h_out.p, nullptr,
+                                   0);
                 else
-                    enqueue_quadratic(c, sl.ws, *q->scene, sl.ws.h_poses.p, r.n, r.n_wp, r.n_ctrl, nullptr, sl.ws.h_out.p,
+                    enqueue_quadratic(c, sl.ws, sc, sl.ws.h_poses.p, b.n, r.n_wp, r.n_ctrl, nullptr, sl.ws.h_out.p,
                                       nullptr, 0);
             };
             finish_sweep(c, sl.ws, nullptr, rerun);
-            RTCD_HIP(hipMemcpyAsync(sl.out.p, sl.ws.h_out.p, sizeof(uint32_t) * r.n, hipMemcpyDeviceToHost,
+            RTCD_HIP(hipMemcpyAsync(sl.out.p, sl.ws.h_out.p, sizeof(uint32_t) * b.n, hipMemcpyDeviceToHost,
                                     sl.ws.stream));
             RTCD_HIP(hipStreamSynchronize(sl.ws.stream));
         }
     }
-    std::memcpy(r.out, sl.out.p, sizeof(uint32_t) * r.n);
+    for (size_t k = 0; k < b.units.size(); ++k)
+        std::memcpy(b.units[k].q->r.out + b.units[k].i0, sl.out.p + b.first[k], sizeof(uint32_t) * b.units[k].n);
+}
+
+// Units that can share one launch: same kind, scene, direction and quadratic shape.
+bool same_shape(const Unit& a, const Unit& b) {
+    const rtcd_request &x = a.q->r, &y = b.q->r;
+    return x.kind == y.kind && a.q->scene == b.q->scene &&
+           (x.kind != RTCD_REQ_DISCRETE || x.direction == y.direction) &&
+           (x.kind != RTCD_REQ_QUADRATIC || (x.n_wp == y.n_wp && x.n_ctrl == y.n_ctrl));
+}
+
+// Next batch to admit (lock held).  Its lead unit is the smallest size class, FIFO within it,
+// unless `aging` and the oldest unit has waited longer than kAgingMs; then queued units of the same
+// shape join it in submit order while the batch stays within the chunk size and the memory budget.
+// The batch's units are removed from the queue.  Empty if nothing fits.
+Batch pick_batch(ServeState* S, double tnow, bool aging) {
+    Batch b;
+    auto lead = S->queue.end();
+    for (auto it = S->queue.begin(); it != S->queue.end(); ++it) {
+        if (S->in_use + it->bytes > S->budget) continue;
+        if (aging && tnow - it->q->t_submit > kAgingMs) {  // the queue is in submit order
+            lead = it;
+            break;
+        }
+        if (lead == S->queue.end() || it->cls < lead->cls) lead = it;
+    }
+    if (lead == S->queue.end()) return b;
+    const Unit head = *lead;
+    const rtcd_ctx* c = S->ctx;
+    const uint32_t chunk = head.q->r.kind == RTCD_REQ_DISCRETE ? kChunkDiscrete : kChunkSweep;
+    auto add = [&](const Unit& u) {
+        b.first.push_back(b.n);
+        b.units.push_back(u);
+        b.n += u.n;
+        b.nw += u.nw;
+    };
+    add(head);
+    b.bytes = head.bytes;
+    S->queue.erase(lead);
+    for (auto it = S->queue.begin(); it != S->queue.end() && b.n < chunk;) {
+        if (!same_shape(*it, head) || b.n + it->n > chunk) {
+            ++it;
+            continue;
+        }
+        const size_t nb = unit_bytes(c, head.q->r, b.n + it->n, b.nw + it->nw);
+        if (S->in_use + nb > S->budget) break;
+        add(*it);
+        b.bytes = nb;
+        it = S->queue.erase(it);
+    }
+    return b;
+}
+
+// Records the end of unit u (lock held): the request completes with its last unit.
+void unit_done(ServeState* S, const Unit& u, rtcd_status st, const std::string& err) {
+    Req* q = u.q;
+    if (st != RTCD_OK && q->status == RTCD_OK) {
+        q->status = st;
+        q->err = err;
+    }
+    if (--q->units_left > 0) return;
+    q->t_done = now_ms();
+    q->done = true;
+    q->scene.reset();
+    if (q->status == RTCD_OK) {
+        S->lat.push_back(q->t_done - q->t_submit);
+        S->completed++;
+    } else {
+        S->failed++;
+    }
+    S->t_last = q->t_done;
+    S->cv_done.notify_all();
 }
 
 void worker_main(ServeState* S) {
@@ -163,9 +282,8 @@ void worker_main(ServeState* S) {
         // 1. completions
         for (auto& up : S->slots) {
             Slot& sl = *up;
-            if (!sl.cur) continue;
+            if (!sl.busy) continue;
             if (hipEventQuery(sl.done_ev) == hipErrorNotReady) continue;
-            Req* q = sl.cur;
             lk.unlock();
             rtcd_status st = RTCD_OK;
             std::string err;
@@ -180,63 +298,43 @@ void worker_main(ServeState* S) {
                 err = e.what();
             }
             lk.lock();
-            q->t_done = now_ms();
-            q->status = st;
-            q->err = err;
-            q->done = true;
-            q->scene.reset();
-            S->in_use -= q->bytes;
-            sl.cur = nullptr;
-            if (st == RTCD_OK) {
-                S->lat.push_back(q->t_done - q->t_submit);
-                S->completed++;
-            } else {
-                S->failed++;
-            }
-            S->t_last = q->t_done;
+            for (const Unit& u : sl.cur.units) unit_done(S, u, st, err);
+            S->in_use -= sl.cur.bytes;
+            sl.busy = false;
             progressed = true;
-            S->cv_done.notify_all();
         }
-        // 2. admissions
-        for (auto& up : S->slots) {
-            Slot& sl = *up;
-            if (sl.cur || S->queue.empty()) continue;
-            Req* pick = nullptr;
-            const double tnow = now_ms();
-            for (auto it = S->queue.begin(); it != S->queue.end(); ++it) {
-                if (S->in_use + (*it)->bytes <= S->budget) {
-                    pick = *it;
-                    S->queue.erase(it);
-                    break;
-                }
-                // the head has waited too long: stop letting smaller requests overtake it
-                if (tnow - (*it)->t_submit > kBypassMs) break;
-            }
-            if (!pick) break;
-            S->in_use += pick->bytes;
-            pick->t_start = tnow;
-            sl.cur = pick;
+        // 2. admissions.  Slot 0 always takes the smallest queued request first (no aging), so a
+        // small request never waits behind aged bulk work on every slot.
+        for (size_t si = 0; si < S->slots.size(); ++si) {
+            Slot& sl = *S->slots[si];
+            if (sl.busy || S->queue.empty()) continue;
+            Batch b = pick_batch(S, now_ms(), si != 0 || S->slots.size() == 1);
+            if (b.units.empty()) break;
+            sl.cur = std::move(b);
+            S->in_use += sl.cur.bytes;
+            sl.busy = true;
             lk.unlock();
+            rtcd_status st = RTCD_OK;
+            std::string err;
             try {
-                launch(S, sl, pick);
+                launch(S, sl, sl.cur);
+            } catch (const HipError& e) {
+                st = e.code == hipErrorOutOfMemory ? RTCD_ERR_OOM : RTCD_ERR_HIP;
+                err = e.what();
             } catch (const std::exception& e) {
-                lk.lock();
-                pick->status = RTCD_ERR_HIP;
-                pick->err = e.what();
-                pick->done = true;
-                pick->t_done = now_ms();
-                pick->scene.reset();
-                S->in_use -= pick->bytes;
-                S->failed++;
-                sl.cur = nullptr;
-                S->cv_done.notify_all();
-                lk.unlock();
+                st = RTCD_ERR_INTERNAL;
+                err = e.what();
             }
             lk.lock();
+            if (st != RTCD_OK) {
+                for (const Unit& u : sl.cur.units) unit_done(S, u, st, err);
+                S->in_use -= sl.cur.bytes;
+                sl.busy = false;
+            }
             progressed = true;
         }
         bool busy = false;
-        for (auto& up : S->slots) busy |= up->cur != nullptr;
+        for (auto& up : S->slots) busy |= up->busy;
         if (S->stop && !busy && S->queue.empty()) break;
         if (!progressed) {
             if (busy) {
@@ -337,16 +435,32 @@ rtcd_status rtcd_submit(rtcd_ctx* ctx, const rtcd_request* r, uint64_t* id) {
     auto q = std::make_unique<Req>();
     q->r = *r;
     q->scene = sc;
-    q->n_wp_total = nw;
-    q->bytes = request_bytes(ctx, *r, nw);
-    if (q->bytes > S->budget) return fail(RTCD_ERR_OOM, "request larger than the serving memory budget");
+    // units of at most `chunk` queries; the size class orders admission (small requests first)
+    const uint32_t chunk = r->kind == RTCD_REQ_DISCRETE ? kChunkDiscrete : kChunkSweep;
+    int cls = 0;
+    while ((1u << cls) < r->n && cls < 31) ++cls;
+    std::vector<Unit> units;
+    for (uint32_t i0 = 0; i0 < r->n; i0 += chunk) {
+        Unit u;
+        u.q = q.get();
+        u.i0 = i0;
+        u.n = std::min(chunk, r->n - i0);
+        u.nw = r->kind == RTCD_REQ_DISCRETE ? u.n
+               : r->kind == RTCD_REQ_LINEAR ? r->offsets[i0 + u.n] - r->offsets[i0]
+                                            : u.n * r->n_wp;
+        u.bytes = unit_bytes(ctx, *r, u.n, u.nw);
+        u.cls = cls;
+        if (u.bytes > S->budget) return fail(RTCD_ERR_OOM, "request unit larger than the serving memory budget");
+        units.push_back(u);
+    }
+    q->units_left = (uint32_t) units.size();
     q->t_submit = now_ms();
     {
         std::lock_guard<std::mutex> g(S->m);
         q->id = S->next_id++;
         *id = q->id;
         if (S->completed + S->failed == 0 && S->reqs.empty()) S->t_first = q->t_submit;
-        S->queue.push_back(q.get());
+        for (const Unit& u : units) S->queue.push_back(u);
         S->queued_max = std::max(S->queued_max, S->queue.size());
         S->reqs[q->id] = std::move(q);
     }
