@@ -128,6 +128,9 @@ def main():
         ctx.check_discrete_device(sid, dposes, dres, direction)
         ctx.sync()
 
+    # the timed steps launch eagerly so that the library records its per-kernel events (a replayed
+    # graph records none); graph replay is measured separately below, for small batches
+    ctx.set_param('graphs', 0)
     log(f'[bench] warmup {args.warmup} x two-way on {n} poses')
     for _ in range(args.warmup):
         run(3)
@@ -151,6 +154,7 @@ def main():
     # work comes from untimed counting runs: the obstacle->robot direction alone is exactly the first
     # pass of the two-way check (same candidates, flags start at zero), and the robot->obstacle pass
     # is the two-way count minus it.
+    ctx.set_param('graphs', 1)
     ctx.set_counting(True)
     run(1)
     c1 = ctx.stats()
