@@ -27,7 +27,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of a narrow kernel (k_narrow<1> obstacle->robot, k_narrow<2> robot->obstacle), per
-# unit of counted work: a leaf record read by each of its lanes 8 B; a candidate triangle evaluated
+# unit of counted work: a leaf record read by each of its lanes 8 B (robot->obstacle reads only the
+# records of poses obstacle->robot left unflagged); a candidate triangle evaluated
 # (its pose not yet flagged) 96 B = scene triangle 48 B + the link's inverse-transform rows 48 B; a
 # robot BVH node record 64 B; a robot triangle tested 48 B (obstacle->robot) or a robot edge tested
 # 32 B (robot->obstacle).
@@ -169,7 +170,7 @@ def main():
           'node_fetches': c1['node_fetches'], 'primitive_tests': c1['prim_tests'],
           'candidates_evaluated': c1['narrow_candidates'], 'traffic_file': 'traffic_narrow_o2r_dense.json'}
     k2 = {'kernel': 'k_narrow<2> (robot->obstacle narrow pass)', 'ms': float(stage[4]),
-          'bytes': (S_REC * rec_lanes + S_EVAL * (c3['narrow_candidates'] - c1['narrow_candidates']) +
+          'bytes': (S_REC * lanes * c3['r2o_records'] + S_EVAL * (c3['narrow_candidates'] - c1['narrow_candidates']) +
                     S_NODE * (c3['node_fetches'] - c1['node_fetches']) + S_EDGE * c3['edge_tests']),
           'node_fetches': c3['node_fetches'] - c1['node_fetches'], 'primitive_tests': c3['edge_tests'],
           'candidates_evaluated': c3['narrow_candidates'] - c1['narrow_candidates'],

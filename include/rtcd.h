@@ -81,6 +81,8 @@ typedef struct {
                                     because their pose was already flagged), summed over directions */
     uint32_t graph_replayed;     /* discrete: 1 if the last check replayed a captured graph (stage_ms
                                     are then -1: the graph records no inner events) */
+    uint64_t r2o_records;        /* two-way: leaf records of the poses the obstacle->robot pass left
+                                    unflagged, the only ones the robot->obstacle pass reads */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */
@@ -186,7 +188,14 @@ typedef enum {
     RTCD_PARAM_SWEEP_PASSES = 8,       /* sweep passes, 1..16; the last never hands off */
     RTCD_PARAM_GRAPHS = 9,             /* 1 (default): a discrete check repeated with the same shape and
                                           buffers replays a captured hipGraph; 0: always launch eagerly */
-    RTCD_PARAM_NARROW_BLOCK = 10       /* work-group size of the discrete narrow passes: 64, 128, 256 */
+    RTCD_PARAM_NARROW_BLOCK = 10,      /* work-group size of the discrete narrow passes: 64, 128, 256 */
+    RTCD_PARAM_NARROW_BUDGET = 11,     /* robot->obstacle: link-BVH nodes a narrow walk visits before handing
+                                          its pending subtrees to the next pass (>= 1) */
+    RTCD_PARAM_NARROW_BUDGET_NEXT = 12, /* the same in every robot->obstacle pass after the first */
+    RTCD_PARAM_NARROW_PASSES = 13,     /* robot->obstacle passes, 1..16; the last never hands off */
+    RTCD_PARAM_O2R_BUDGET = 14,        /* obstacle->robot: the same three */
+    RTCD_PARAM_O2R_BUDGET_NEXT = 15,
+    RTCD_PARAM_O2R_PASSES = 16
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 /* Rebuilds the robot's and every scene's BVHs with another leaf size (1..16); scene ids are kept.

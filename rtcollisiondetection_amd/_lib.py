@@ -25,7 +25,8 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
 TUNE_LEAF_SIZE = 1
 PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad_budget': 4,
           'broad_budget_small': 5, 'broad_budget_next': 6, 'broad_passes': 7, 'sweep_passes': 8, 'graphs': 9,
-          'narrow_block': 10}
+          'narrow_block': 10, 'narrow_budget': 11, 'narrow_budget_next': 12, 'narrow_passes': 13,
+          'o2r_budget': 14, 'o2r_budget_next': 15, 'o2r_passes': 16}
 REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC = 1, 2, 3
 
 
@@ -52,7 +53,8 @@ class Stats(C.Structure):
                 ('candidates', C.c_uint64), ('last_ms', C.c_double), ('max_stack', C.c_uint32),
                 ('max_depth', C.c_uint32), ('edge_tests', C.c_uint64), ('stage_ms', C.c_double * 5),
                 ('leaf_candidates', C.c_uint64), ('cert_calls', C.c_uint64), ('broad_node_fetches', C.c_uint64),
-                ('narrow_candidates', C.c_uint64), ('graph_replayed', C.c_uint32)]
+                ('narrow_candidates', C.c_uint64), ('graph_replayed', C.c_uint32),
+                ('r2o_records', C.c_uint64)]
 
 
 class Request(C.Structure):

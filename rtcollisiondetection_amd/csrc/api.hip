@@ -69,7 +69,7 @@ size_t discrete_bytes(const rtcd_ctx* c, uint32_t n) {
     const size_t NL = (size_t) c->robot.n_links, tasks = (NL - (c->cfg.skip_base_link ? 1 : 0)) * n;
     const size_t cap = std::max<size_t>((size_t) 1 << 20, tasks * 64);
     const size_t unsure = std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
-    return n * NL * (12 * 8 + 3 * 16 + 2 * 16) + 2 * cap * 8 + ((size_t) 2 << 20) * 8 +
+    return n * NL * (12 * 8 + 3 * 16 + 2 * 16) + 2 * cap * 8 + ((size_t) 2 << 20) * 8 + ((size_t) 1 << 20) * 16 +
            unsure * 16 + (size_t) n * (4 * c->robot.dof + 4) + 64;
 }
 
@@ -117,6 +117,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     ws.scalars.reserve(kScalars);
     ws.stats.reserve(1);
     if (ws.spill.n < ((size_t) 2 << 20)) ws.spill.reserve((size_t) 2 << 20);
+    if (ws.nspill.n < ((size_t) 1 << 20)) ws.nspill.reserve((size_t) 1 << 20);
 
     ws.seg_count.reserve((size_t) kCandSegments * kSegStride);
     ws.seg_pre.reserve(kCandSegments + 1);
@@ -164,6 +165,13 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.budget = tasks < c->frontier_max_tasks ? c->broad_budget_small : c->broad_budget;
     a.budget_next = c->broad_budget_next;
     a.n_passes = (int) c->broad_passes;
+    a.nspill_cap = (uint32_t) std::min<size_t>(ws.nspill.n / 2, 0x7fffffffu);
+    a.nspill[0] = ws.nspill.p;
+    a.nspill[1] = ws.nspill.p + a.nspill_cap;
+    a.nspill_count = ws.scalars.p + 24;
+    a.narrow_budget = c->o2r_budget;
+    a.narrow_budget_next = c->o2r_budget_next;
+    a.narrow_passes = (int) c->o2r_passes;
     a.narrow_bs = (int) c->narrow_bs;
     const int broad_stack = stack_for(sc.tri_bvh.depth);
     mark(1);
@@ -175,6 +183,16 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     // pass then skips
     if (dir & 1) launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
     mark(4);
+    if (dir == 3) {
+        // the broad records are no longer needed: the compacted list reuses their buffer
+        a.r2o_list = ws.cand.p;
+        a.r2o_count = ws.scalars.p + 3;
+        launch_compact_unflagged(a, s);
+    }
+    a.nspill_count = ws.scalars.p + 40;  // fresh hand-off counters for robot->obstacle
+    a.narrow_budget = c->narrow_budget;
+    a.narrow_budget_next = c->narrow_budget_next;
+    a.narrow_passes = (int) c->narrow_passes;
     if (dir & 2) launch_narrow(a, 2, c->robot_depth + 1, ws.counting, s);
     launch_redecide(a, s);
     mark(5);
@@ -199,7 +217,9 @@ static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, c
             P(c->d_robot.p), P(c->r_tri_nodes.p), P(c->r_tris.p), P(c->r_edge_nodes.p), P(c->r_edges.p),
             P(ws.stream), P(d_poses), P(d_out), n, (uint64_t) dir,
             P(ws.T64.p), P(ws.Ti.p), P(ws.wbox.p), P(ws.cand.p), ws.cand.n, P(ws.flat.p), ws.flat.n,
-            P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.spill.p), ws.spill.n, P(ws.unsure.p), ws.unsure.n,
+            P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.spill.p), ws.spill.n, P(ws.nspill.p), ws.nspill.n,
+            P(ws.unsure.p), ws.unsure.n, c->narrow_budget, c->narrow_budget_next, c->narrow_passes, c->o2r_budget,
+            c->o2r_budget_next, c->o2r_passes,
             P(ws.scalars.p), P(ws.stats.p), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
             c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth, c->narrow_bs};
 }
@@ -287,6 +307,7 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                 }
                 st->candidates = (uint64_t) sv[4] | ((uint64_t) sv[5] << 32);  // triangles in emitted leaves
                 st->leaf_candidates = ws.counting ? sv[0] : 0;
+                st->r2o_records = dir == 3 ? sv[3] : 0;
                 st->max_depth = (uint32_t) std::max(c->robot_depth, sc.tri_bvh.depth);
                 st->max_stack = (uint32_t) stack_for(sc.tri_bvh.depth);
                 if (ws.counting) {
@@ -777,9 +798,28 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
         return RTCD_OK;
     case RTCD_PARAM_BROAD_PASSES:
     case RTCD_PARAM_SWEEP_PASSES:
+    case RTCD_PARAM_NARROW_PASSES:
+    case RTCD_PARAM_O2R_PASSES: {
         if (value < 1 || value > (uint64_t) kMaxPasses) return fail(RTCD_ERR_INVALID_ARG, "passes must be 1..16");
-        (param == RTCD_PARAM_BROAD_PASSES ? ctx->broad_passes : ctx->sweep_passes) = (uint32_t) value;
+        uint32_t* f = param == RTCD_PARAM_BROAD_PASSES   ? &ctx->broad_passes
+                      : param == RTCD_PARAM_SWEEP_PASSES ? &ctx->sweep_passes
+                      : param == RTCD_PARAM_NARROW_PASSES ? &ctx->narrow_passes
+                                                          : &ctx->o2r_passes;
+        *f = (uint32_t) value;
         return RTCD_OK;
+    }
+    case RTCD_PARAM_NARROW_BUDGET:
+    case RTCD_PARAM_NARROW_BUDGET_NEXT:
+    case RTCD_PARAM_O2R_BUDGET:
+    case RTCD_PARAM_O2R_BUDGET_NEXT: {
+        if (value < 1 || value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "budget must be 1..2^32-1");
+        uint32_t* f = param == RTCD_PARAM_NARROW_BUDGET        ? &ctx->narrow_budget
+                      : param == RTCD_PARAM_NARROW_BUDGET_NEXT ? &ctx->narrow_budget_next
+                      : param == RTCD_PARAM_O2R_BUDGET         ? &ctx->o2r_budget
+                                                               : &ctx->o2r_budget_next;
+        *f = (uint32_t) value;
+        return RTCD_OK;
+    }
     default:
         return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
@@ -799,6 +839,12 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_SWEEP_PASSES: *value = ctx->sweep_passes; return RTCD_OK;
     case RTCD_PARAM_GRAPHS: *value = ctx->use_graphs ? 1 : 0; return RTCD_OK;
     case RTCD_PARAM_NARROW_BLOCK: *value = ctx->narrow_bs; return RTCD_OK;
+    case RTCD_PARAM_NARROW_BUDGET: *value = ctx->narrow_budget; return RTCD_OK;
+    case RTCD_PARAM_NARROW_BUDGET_NEXT: *value = ctx->narrow_budget_next; return RTCD_OK;
+    case RTCD_PARAM_NARROW_PASSES: *value = ctx->narrow_passes; return RTCD_OK;
+    case RTCD_PARAM_O2R_BUDGET: *value = ctx->o2r_budget; return RTCD_OK;
+    case RTCD_PARAM_O2R_BUDGET_NEXT: *value = ctx->o2r_budget_next; return RTCD_OK;
+    case RTCD_PARAM_O2R_PASSES: *value = ctx->o2r_passes; return RTCD_OK;
     default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
 }

@@ -89,7 +89,9 @@ struct Workspace {
     DevBuf<double> T64;
     DevBuf<float4> Ti, wbox;
     // scalars (kScalars words): [0] leaf records (counting), [1] overflow bits, [2] queued FP64 tests,
-    // [4..5] discrete triangles in emitted leaves (u64), [8..] hand-offs per pass (broad or sweep)
+    // [3] records left for robot->obstacle, [4..5] discrete triangles in emitted leaves (u64),
+    // [8..23] hand-offs per pass (broad or sweep), [24..39] / [40..55] narrow hand-offs per pass of
+    // obstacle->robot / robot->obstacle
     DevBuf<uint32_t> counts, offsets, scan_tmp, scalars;
     DevBuf<uint2> cand;
     uint32_t cand_cap = 0;
@@ -98,6 +100,7 @@ struct Workspace {
     DevBuf<uint2> flat;          // candidate segments gathered in order (capacity of cand)
     DevBuf<uint4> unsure;  // FP64 re-decision queue
     DevBuf<uint2> spill;   // hand-off queues of the broad phase (two halves)
+    DevBuf<uint4> nspill;  // hand-off queues of the narrow phase (two halves)
     // host staging
     DevBuf<float> h_poses;
     DevBuf<uint32_t> h_offsets, h_out, h_seg;
@@ -117,7 +120,7 @@ struct Workspace {
 
     size_t bytes() const {
         return T64.bytes() + Ti.bytes() + wbox.bytes() + counts.bytes() + offsets.bytes() + scan_tmp.bytes() +
-               scalars.bytes() + cand.bytes() + flat.bytes() + unsure.bytes() + spill.bytes() + h_poses.bytes() + h_offsets.bytes() + h_out.bytes() + h_seg.bytes() +
+               scalars.bytes() + cand.bytes() + flat.bytes() + unsure.bytes() + spill.bytes() + nspill.bytes() + h_poses.bytes() + h_offsets.bytes() + h_out.bytes() + h_seg.bytes() +
                c64.bytes() + c32.bytes() + seg_traj.bytes() + seg_wp.bytes() + P64.bytes() + W64.bytes() + P32.bytes();
     }
     void init(hipStream_t s, bool own) {
@@ -200,6 +203,12 @@ struct rtcd_ctx {
     uint32_t broad_passes = 12;                 // broad hand-off passes (the last never hands off)
     uint32_t sweep_passes = 8;                  // sweep hand-off passes
     uint32_t narrow_bs = 64;                    // work-group size of the discrete narrow passes
+    uint32_t narrow_budget = 16;                // robot->obstacle: link-BVH nodes before a walk hands off
+    uint32_t narrow_budget_next = 8;            // the same in later passes
+    uint32_t narrow_passes = 6;                 // robot->obstacle passes (the last never hands off)
+    uint32_t o2r_budget = 128;                  // obstacle->robot: the same three (one pass: no hand-off)
+    uint32_t o2r_budget_next = 64;
+    uint32_t o2r_passes = 1;
     int n_spheres = 0;
     int robot_depth = 1;
     // scenes (shared so that in-flight served requests keep a scene alive)

@@ -297,10 +297,80 @@ __global__ __launch_bounds__(kBS) void k_seg_gather(const uint2* __restrict__ ca
     for (uint32_t j = threadIdx.x; j < n; j += kBS) flat[(size_t) base + j] = src[j];
 }
 
+// After the obstacle->robot pass: the records of poses it left unflagged, in order, into a.r2o_list
+// (block b compacts the b-th contiguous chunk of the gathered list: one sweep counts, one global
+// atomic per block reserves, a second sweep writes with block-wide ranks).
+__global__ __launch_bounds__(kBS) void k_compact_unflagged(DiscreteArgs a) {
+    __shared__ uint32_t wsum[kBS / 64];
+    __shared__ uint32_t base;
+    const uint32_t total = a.seg_pre[kSeg];
+    const uint32_t chunk = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t beg = min(total, blockIdx.x * chunk), end = min(total, beg + chunk);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    auto keep = [&](uint32_t i) {
+        return i < end && !load_flag(a.flags + a.flat[i].x / (uint32_t) a.n_links);
+    };
+    uint32_t mine = 0;
+    for (uint32_t i = beg + threadIdx.x; i < end; i += kBS) mine += keep(i) ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) mine += __shfl_down(mine, o, 64);
+    if (lane == 0) wsum[wave] = mine;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kBS / 64; ++w) t += wsum[w];
+        base = t ? atomicAdd(a.r2o_count, t) : 0u;
+    }
+    __syncthreads();
+    uint32_t run = base;
+    for (uint32_t it = beg; it < end; it += kBS) {
+        const uint32_t i = it + threadIdx.x;
+        const bool k = keep(i);
+        const uint64_t m = __ballot(k);
+        const uint32_t rank = (uint32_t) __popcll(m & ((1ull << lane) - 1ull));
+        __syncthreads();
+        if (lane == 0) wsum[wave] = (uint32_t) __popcll(m);
+        __syncthreads();
+        uint32_t before = 0, all = 0;
+        for (int w = 0; w < kBS / 64; ++w) {
+            before += w < (int) wave ? wsum[w] : 0u;
+            all += wsum[w];
+        }
+        if (k) a.r2o_list[run + before + rank] = a.flat[i];
+        run += all;
+    }
+}
+
 // ------------------------------------------------------------------------------------------ narrow
 struct Ctr {
     uint32_t nodes = 0, prims = 0, fp64 = 0, edges = 0, tris = 0;
 };
+
+// Work hand-off of a narrow traversal: once a walk has visited `budget` nodes in a pass that may
+// hand off, its pending subtrees (the next node and its stack) are appended to `out` as
+// (pose * NL + link, scene triangle, link-BVH node, edge j of the triangle or 3) and the walk ends;
+// the next pass resumes them.  A full queue only stops hand-offs.
+struct NarrowSpill {
+    uint4* out;
+    uint32_t* count;
+    uint32_t cap, budget;
+    bool may;
+};
+
+// Appends the pending subtrees (ni and the stack) of a walk; false if the queue is full (the slots
+// it reserved are marked empty and the walk goes on by itself).
+template <int BS>
+__device__ __forceinline__ bool spill_walk(const NarrowSpill& sp_, uint32_t key, uint32_t tri, uint32_t j, int ni,
+                                           const int* st, int sp) {
+    const uint32_t want = (uint32_t) sp + 1u;
+    const uint32_t o = atomicAdd(sp_.count, want);
+    if ((uint64_t) o + want <= sp_.cap) {
+        sp_.out[o] = make_uint4(key, tri, (uint32_t) ni, j);
+        for (int k = 0; k < sp; ++k) sp_.out[o + 1 + k] = make_uint4(key, tri, (uint32_t) st[k * BS], j);
+        return true;
+    }
+    for (uint64_t k = o; k < sp_.cap && k < (uint64_t) o + want; ++k) sp_.out[k] = make_uint4(0xffffffffu, 0u, 0u, 0u);
+    return false;
+}
 
 // A test whose FP32 signs fall inside their band is queued for the FP64 kernel: (pose*NL + link,
 // scene triangle, robot primitive, kind) with kind 0..2 = obstacle edge j of the triangle (O2R),
@@ -315,7 +385,8 @@ __device__ __forceinline__ void defer(const DiscreteArgs& a, uint32_t key, uint3
 // component, direction d = R^T (Q - P) with rel. error <= 6 eps) against the link-triangle LBVH.
 template <int STACK, int BS, bool COUNT>
 __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, uint32_t j, float3 P,
-                        float3 d, float delta, int* st, const uint32_t* flag, Ctr& ctr) {
+                        float3 d, float delta, int* st, const uint32_t* flag, Ctr& ctr, int start,
+                        const NarrowSpill& hs) {
     if (!L.tri_has_root) return 0;
     const float3 inv = safe_inv(d);
     const float D = l1(d.x, d.y, d.z) * (1.f + 8.f * kEps32);
@@ -327,9 +398,14 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, 
     const MTBands bd = make_bands(D, ad, S, as, L.e1max, kEps32 * L.e1max, L.e2max, kEps32 * L.e2max);
     const BNode* nodes = a.rb.tri_nodes + L.tri_node0;
     const Tri* prims = a.rb.tri_prims + L.tri_prim0;
-    int sp = 0, ni = 0, iter = 0;
+    int sp = 0, ni = start, iter = 0;
+    bool spill_ok = hs.may;
     for (;;) {
         if ((++iter & 15) == 0 && load_flag(flag)) return 0;
+        if (spill_ok && (uint32_t) iter > hs.budget) {
+            if (spill_walk<BS>(hs, key, tri, j, ni, st, sp)) return 0;
+            spill_ok = false;
+        }
         const BNode nd = nodes[ni];
         if (COUNT) ctr.nodes++;
         bool h0 = nd.c0 >= 0 && slab(P, inv, nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, pad);
@@ -375,7 +451,8 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, 
 // rel. error <= 6 eps) against the link-edge LBVH.
 template <int STACK, int BS, bool COUNT>
 __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, float3 A, float3 e1,
-                       float3 e2, float delta, int* st, const uint32_t* flag, Ctr& ctr) {
+                       float3 e2, float delta, int* st, const uint32_t* flag, Ctr& ctr, int start,
+                       const NarrowSpill& hs) {
     if (!L.edge_has_root) return 0;
     const float3 B = add(A, e1), C = add(A, e2);
     const float pad = 2.f * delta + 12.f * kEps32 * (l1(e1.x, e1.y, e1.z) + l1(e2.x, e2.y, e2.z)) + 1e-6f * L.scale;
@@ -406,9 +483,14 @@ __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, u
                         4.f * kEps32 * (fabsf(sd) + rr);
         return fabsf(sd) <= rr + m;
     };
-    int sp = 0, ni = 0, iter = 0;
+    int sp = 0, ni = start, iter = 0;
+    bool spill_ok = hs.may;
     for (;;) {
         if ((++iter & 15) == 0 && load_flag(flag)) return 0;
+        if (spill_ok && (uint32_t) iter > hs.budget) {
+            if (spill_walk<BS>(hs, key, tri, 3u, ni, st, sp)) return 0;
+            spill_ok = false;
+        }
         const BNode nd = nodes[ni];
         if (COUNT) ctr.nodes++;
         bool h0 = nd.c0 >= 0 && !(lx > nd.hi0x || hx < nd.lo0x || ly > nd.hi0y || hy < nd.lo0y || lz > nd.hi0z ||
@@ -455,45 +537,71 @@ __device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, u
 }
 
 // One direction over all candidates (grid-stride).  DIR 1: obstacle->robot, 2: robot->obstacle.
+// Candidate (key, scene triangle) in direction DIR: edge j of the triangle (obstacle->robot) or the
+// triangle (robot->obstacle) against the link BVH from node `start`.  All edges when j == 4.
 template <int DIR, int STACK, int BS, bool COUNT>
-__global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a) {
+__device__ __forceinline__ int narrow_candidate(const DiscreteArgs& a, uint32_t key, uint32_t tri, uint32_t j,
+                                                int start, int* st, uint32_t* flag, Ctr& ctr,
+                                                const NarrowSpill& hs) {
+    const uint32_t l = key % (uint32_t) a.n_links;
+    const LinkInfo& L = a.robot->link[l];
+    const float4 r0 = a.Ti[(size_t) key * 3], r1 = a.Ti[(size_t) key * 3 + 1], r2 = a.Ti[(size_t) key * 3 + 2];
+    const Tri t = a.sc.tri_prims[tri];
+    const LinkTri m = link_tri(t, r0, r1, r2);
+    if (DIR == 2) return r2o_tri<STACK, BS, COUNT>(a, L, key, tri, m.A, m.e1, m.e2, m.delta, st, flag, ctr, start, hs);
+    const uint32_t owned = __float_as_uint(t.b.w);
+    int hit = 0;
+    if ((j == 4 || j == 0) && (owned & 1u))
+        hit = o2r_edge<STACK, BS, COUNT>(a, L, key, tri, 0u, m.A, m.e1, m.delta, st, flag, ctr, start, hs);
+    if (!hit && (j == 4 || j == 1) && (owned & 2u))
+        hit = o2r_edge<STACK, BS, COUNT>(a, L, key, tri, 1u, add(m.A, m.e1), rot(r0, r1, r2, sub(f3(t.c), f3(t.b))),
+                                         m.delta, st, flag, ctr, start, hs);
+    if (!hit && (j == 4 || j == 2) && (owned & 4u))
+        hit = o2r_edge<STACK, BS, COUNT>(a, L, key, tri, 2u, add(m.A, m.e2), neg(m.e2), m.delta, st, flag, ctr, start,
+                                         hs);
+    return hit;
+}
+
+// One direction (DIR 1: obstacle->robot, 2: robot->obstacle), pass `pass` of a.narrow_passes.  Pass
+// 0 runs over the candidates (grid-stride, each leaf record spread over `lanes` adjacent threads,
+// one scene triangle per thread); later passes resume the walks the previous pass handed off.
+template <int DIR, int STACK, int BS, bool COUNT>
+__global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
     __shared__ int stk[STACK * BS];
     int* st = stk + threadIdx.x;
-    // each leaf record is spread over `lanes` adjacent threads, one triangle per thread
-    const uint32_t lanes = (uint32_t) a.leaf_lanes;
-    const uint64_t total = (uint64_t) a.seg_pre[kSeg] * lanes;
+    const NarrowSpill hs{a.nspill[pass & 1], a.nspill_count + pass, a.nspill_cap,
+                         pass == 0 ? a.narrow_budget : a.narrow_budget_next, pass < a.narrow_passes - 1};
     const uint32_t stride = gridDim.x * BS;
     Ctr ctr;
-    for (uint64_t v = blockIdx.x * BS + threadIdx.x; v < total; v += stride) {
-        const uint32_t i = (uint32_t) (v / lanes), sub_k = (uint32_t) (v - (uint64_t) i * lanes);
-        const uint2 c = a.flat[i];
-        if (sub_k > leaf_last(c.y)) continue;
-        const uint32_t p = c.x / (uint32_t) a.n_links, l = c.x - p * (uint32_t) a.n_links;
-        uint32_t* flag = a.flags + p;
-        if (load_flag(flag)) continue;
-        const LinkInfo& L = a.robot->link[l];
-        const float4 r0 = a.Ti[(size_t) c.x * 3], r1 = a.Ti[(size_t) c.x * 3 + 1], r2 = a.Ti[(size_t) c.x * 3 + 2];
-        int hit = 0;
-        {
-            const uint32_t tri = leaf_start(c.y) + sub_k;
-            const Tri t = a.sc.tri_prims[tri];
-            const LinkTri m = link_tri(t, r0, r1, r2);
-            const float3 A = m.A, e1 = m.e1, e2 = m.e2;
-            const float delta = m.delta;
+    if (pass == 0) {
+        // `lanes` adjacent threads share a leaf record, one scene triangle each: group g of a block
+        // takes records blockIdx.x * groups + g, stepping by gridDim.x * groups
+        const uint32_t lanes = (uint32_t) a.leaf_lanes;
+        const uint32_t groups = BS / lanes, g = threadIdx.x / lanes, sub_k = threadIdx.x - g * lanes;
+        if (g >= groups) return;
+        // robot->obstacle runs over the records of the poses obstacle->robot left unflagged
+        const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat;
+        const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg];
+        for (uint32_t i = blockIdx.x * groups + g; i < total; i += gridDim.x * groups) {
+            const uint2 c = list[i];
+            if (sub_k > leaf_last(c.y)) continue;
+            uint32_t* flag = a.flags + c.x / (uint32_t) a.n_links;
+            if (load_flag(flag)) continue;
             if (COUNT) ctr.tris++;
-            if (DIR == 1) {
-                const uint32_t owned = __float_as_uint(t.b.w);
-                if (owned & 1u) hit = o2r_edge<STACK, BS, COUNT>(a, L, c.x, tri, 0u, A, e1, delta, st, flag, ctr);
-                if (!hit && (owned & 2u))
-                    hit = o2r_edge<STACK, BS, COUNT>(a, L, c.x, tri, 1u, add(A, e1),
-                                                 rot(r0, r1, r2, sub(f3(t.c), f3(t.b))), delta, st, flag, ctr);
-                if (!hit && (owned & 4u))
-                    hit = o2r_edge<STACK, BS, COUNT>(a, L, c.x, tri, 2u, add(A, e2), neg(e2), delta, st, flag, ctr);
-            } else {
-                hit = r2o_tri<STACK, BS, COUNT>(a, L, c.x, tri, A, e1, e2, delta, st, flag, ctr);
-            }
+            if (narrow_candidate<DIR, STACK, BS, COUNT>(a, c.x, leaf_start(c.y) + sub_k, 4u, 0, st, flag, ctr, hs))
+                atomicOr(flag, 1u);
         }
-        if (hit) atomicOr(flag, 1u);
+    } else {
+        const uint4* in = a.nspill[(pass - 1) & 1];
+        const uint32_t total = min(a.nspill_count[pass - 1], a.nspill_cap);
+        for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < total; i += stride) {
+            const uint4 e = in[i];
+            if (e.x == 0xffffffffu) continue;  // slot of a hand-off that did not fit
+            uint32_t* flag = a.flags + e.x / (uint32_t) a.n_links;
+            if (load_flag(flag)) continue;
+            if (narrow_candidate<DIR, STACK, BS, COUNT>(a, e.x, e.y, e.w, (int) e.z, st, flag, ctr, hs))
+                atomicOr(flag, 1u);
+        }
     }
     if (COUNT) {
         atomicAdd(&a.stats->node_fetches, (unsigned long long) ctr.nodes);
@@ -581,9 +689,11 @@ template <int DIR, int STACK, int BS>
 static void narrow_bs(const DiscreteArgs& a, bool count, hipStream_t s) {
     // as many threads as 8 blocks of 256 per CU, whatever the block size
     const unsigned nb = (unsigned) device_cus() * 8u * (256u / BS);
-    if (count) hipLaunchKernelGGL((k_narrow<DIR, STACK, BS, true>), dim3(nb), dim3(BS), 0, s, a);
-    else hipLaunchKernelGGL((k_narrow<DIR, STACK, BS, false>), dim3(nb), dim3(BS), 0, s, a);
-    RTCD_LAUNCH_CHECK();
+    for (int pass = 0; pass < a.narrow_passes; ++pass) {
+        if (count) hipLaunchKernelGGL((k_narrow<DIR, STACK, BS, true>), dim3(nb), dim3(BS), 0, s, a, pass);
+        else hipLaunchKernelGGL((k_narrow<DIR, STACK, BS, false>), dim3(nb), dim3(BS), 0, s, a, pass);
+        RTCD_LAUNCH_CHECK();
+    }
 }
 
 template <int DIR, int STACK>
@@ -612,6 +722,11 @@ void launch_gather(const DiscreteArgs& a, hipStream_t s) {
 void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStream_t s) {
     if (dir == 1) narrow_dir<1>(a, stack, count, s);
     else narrow_dir<2>(a, stack, count, s);
+}
+
+void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_unflagged, dim3(kSeg), dim3(kBS), 0, s, a);
+    RTCD_LAUNCH_CHECK();
 }
 
 void launch_redecide(const DiscreteArgs& a, hipStream_t s) {

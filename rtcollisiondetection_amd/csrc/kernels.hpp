@@ -11,7 +11,7 @@ constexpr int kCandSegments = 2048;
 // same time then never contend for one line's atomics.
 constexpr int kSegStride = 32;
 // Words of the per-workspace scalar block (counters and flags of one launch).
-constexpr int kScalars = 32;
+constexpr int kScalars = 64;
 
 struct RobotBvh {
     const BNode* tri_nodes;
@@ -49,6 +49,8 @@ struct DiscreteArgs {
                            // (may exceed seg_cap: overflow bit 1)
     uint32_t* seg_pre;     // [kCandSegments + 1] exclusive prefix of the clamped segment counts
     uint2* flat;           // the segments gathered in order: [seg_pre[kCandSegments]] records
+    uint2* r2o_list;       // two-way checks: records of the poses obstacle->robot left unflagged, or null
+    uint32_t* r2o_count;   // their number
     unsigned long long* tri_candidates;  // triangles in the emitted leaves
     uint32_t* overflow;
     uint32_t* flags;       // [n] output
@@ -67,6 +69,11 @@ struct DiscreteArgs {
     uint32_t budget_next;  // the same in later passes
     int n_passes;          // broad passes (<= kMaxPasses); the last never hands off
     int narrow_bs;         // work-group size of the narrow passes: 64, 128 or 256
+    uint4* nspill[2];      // narrow hand-off queues (pose * NL + link, triangle, node, edge j or 3)
+    uint32_t nspill_cap;   // records per queue
+    uint32_t* nspill_count;  // [narrow_passes - 1] per direction: hand-offs of each pass
+    uint32_t narrow_budget, narrow_budget_next;  // link-BVH nodes a walk visits before handing off
+    int narrow_passes;     // passes per direction (<= kMaxPasses); the last never hands off
 };
 // Most hand-off passes of one broad phase or sweep: the first walks every task, each later one the
 // subtrees handed off by the previous pass.
@@ -82,6 +89,8 @@ void launch_gather(const DiscreteArgs& a, hipStream_t s);
 // Narrow phase, one grid-stride pass in direction `dir` (1: obstacle->robot, 2: robot->obstacle)
 // over the gathered candidates.  `stack` covers the robot BVH depth.
 void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStream_t s);
+// Compacts the gathered records of still-unflagged poses into a.r2o_list (after obstacle->robot).
+void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s);
 // FP64 re-decision of the queued tests.
 void launch_redecide(const DiscreteArgs& a, hipStream_t s);
 

@@ -641,6 +641,7 @@ void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::functio
             st->cert_calls = k.certs;
             st->broad_node_fetches = 0;
             st->narrow_candidates = 0;
+            st->r2o_records = 0;
             std::vector<uint32_t> sc = ws.scalars.download(kScalars, ws.stream);
             uint64_t tot = 0;
             for (int p = 0; p + 1 < (int) c->sweep_passes; ++p) tot += sc[8 + p];

@@ -103,7 +103,7 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
     struct Restore {
         rtcd_ctx* c;
         bool counting;
-        uint32_t leaf, nbs, passes, bud, bud_small, bud_next, sw_bud, sw_passes;
+        uint32_t leaf, nbs, passes, bud, bud_small, bud_next, sw_bud, sw_passes, n_passes, n_bud, o_passes;
         bool done = false;
         ~Restore() {
             c->counting = counting;
@@ -115,6 +115,9 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
             c->broad_budget_next = bud_next;
             c->sweep_budget = sw_bud;
             c->sweep_passes = sw_passes;
+            c->narrow_passes = n_passes;
+            c->narrow_budget = n_bud;
+            c->o2r_passes = o_passes;
             if (c->cfg.leaf_size != leaf) {
                 c->cfg.leaf_size = leaf;
                 try {
@@ -132,7 +135,10 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
               ctx->broad_budget_small,
               ctx->broad_budget_next,
               ctx->sweep_budget,
-              ctx->sweep_passes};
+              ctx->sweep_passes,
+              ctx->narrow_passes,
+              ctx->narrow_budget,
+              ctx->o2r_passes};
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
         ctx->counting = false;
@@ -181,6 +187,9 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
             }
             std::vector<Knob> knobs = {
                 {&ctx->narrow_bs, {64, 128, 256}},
+                {&ctx->narrow_passes, {1, 4, 6, 8}},
+                {&ctx->narrow_budget, {8, 16, 32, 64}},
+                {&ctx->o2r_passes, {1, 2, 4}},
                 {&ctx->broad_passes, {6, 8, 12, 16}},
                 {&ctx->broad_budget_next, {8, 16, 32}},
                 small ? Knob{&ctx->broad_budget_small, {8, 16, 32, 64}} : Knob{&ctx->broad_budget, {16, 32, 64, 128}},

@@ -43,8 +43,8 @@ def test_admission_rejects_oversized_and_bad_requests(env, panda):
     ctx, ids = env
     from rtcollisiondetection_amd.serving import Request
     from rtcollisiondetection_amd import samplers
-    # a 64-pose unit needs about 35 MB, a 4,096-pose unit (the 65,536-pose request's) about 53 MB
-    srv = Server(ctx, slots=2, mem_budget=40 << 20)
+    # a 64-pose unit needs about 52 MB, a 4,096-pose unit (the 65,536-pose request's) about 70 MB
+    srv = Server(ctx, slots=2, mem_budget=60 << 20)
     try:
         big = Request(_lib.REQ_DISCRETE, ids[0], 65536, samplers.uniform_poses(65536, panda.lower, panda.upper, 1))
         with pytest.raises(_lib.RtcdError) as e:
