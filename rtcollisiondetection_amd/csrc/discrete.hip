@@ -574,23 +574,49 @@ __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
     const uint32_t stride = gridDim.x * BS;
     Ctr ctr;
     if (pass == 0) {
-        // `lanes` adjacent threads share a leaf record, one scene triangle each: group g of a block
-        // takes records blockIdx.x * groups + g, stepping by gridDim.x * groups
+        // Each wave reads leaf records (`lanes` adjacent lanes per record, one scene triangle each),
+        // keeps the candidates whose pose is not flagged yet in an LDS queue, and evaluates them 64 at
+        // a time with every lane active: most candidates belong to poses flagged early, and a wave
+        // that evaluated them in place would run the whole test for a few live lanes.
+        __shared__ uint2 lq[BS / 64][128];
+        const int lane = (int) (threadIdx.x & 63);
+        uint2* q = lq[threadIdx.x >> 6];
+        int qn = 0;  // wave-uniform
         const uint32_t lanes = (uint32_t) a.leaf_lanes;
-        const uint32_t groups = BS / lanes, g = threadIdx.x / lanes, sub_k = threadIdx.x - g * lanes;
-        if (g >= groups) return;
+        const uint32_t gpw = 64u / lanes, g = (uint32_t) lane / lanes, sub_k = (uint32_t) lane - g * lanes;
+        const uint32_t wave = blockIdx.x * (BS / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (BS / 64);
         // robot->obstacle runs over the records of the poses obstacle->robot left unflagged
         const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat;
         const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg];
-        for (uint32_t i = blockIdx.x * groups + g; i < total; i += gridDim.x * groups) {
-            const uint2 c = list[i];
-            if (sub_k > leaf_last(c.y)) continue;
-            uint32_t* flag = a.flags + c.x / (uint32_t) a.n_links;
-            if (load_flag(flag)) continue;
+        auto eval = [&](uint2 e) {
+            uint32_t* flag = a.flags + e.x / (uint32_t) a.n_links;
+            if (load_flag(flag)) return;
             if (COUNT) ctr.tris++;
-            if (narrow_candidate<DIR, STACK, BS, COUNT>(a, c.x, leaf_start(c.y) + sub_k, 4u, 0, st, flag, ctr, hs))
-                atomicOr(flag, 1u);
+            if (narrow_candidate<DIR, STACK, BS, COUNT>(a, e.x, e.y, 4u, 0, st, flag, ctr, hs)) atomicOr(flag, 1u);
+        };
+        for (uint32_t base = wave * gpw; base < total; base += n_waves * gpw) {
+            const uint32_t i = base + g;
+            bool live = false;
+            uint2 cand = make_uint2(0u, 0u);
+            if (g < gpw && i < total) {
+                const uint2 c = list[i];
+                if (sub_k <= leaf_last(c.y) && !load_flag(a.flags + c.x / (uint32_t) a.n_links)) {
+                    live = true;
+                    cand = make_uint2(c.x, leaf_start(c.y) + sub_k);
+                }
+            }
+            const uint64_t m = __ballot(live);
+            if (live) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = cand;
+            qn += __popcll(m);
+            __builtin_amdgcn_wave_barrier();
+            while (qn >= 64) {
+                const uint2 e = q[qn - 64 + lane];
+                qn -= 64;
+                __builtin_amdgcn_wave_barrier();
+                eval(e);
+            }
         }
+        if (lane < qn) eval(q[lane]);
     } else {
         const uint4* in = a.nspill[(pass - 1) & 1];
         const uint32_t total = min(a.nspill_count[pass - 1], a.nspill_cap);
