@@ -140,8 +140,8 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
     // through an earlier link skip the later ones in the narrow phase); later passes stride.
     uint32_t* seg_count = a.seg_count + (size_t) blockIdx.x * kSegStride;
     uint2* seg_cand = a.cand + (size_t) blockIdx.x * a.seg_cap;
-    const uint2* in_q = pass > 0 ? a.spill[(pass - 1) & 1] : nullptr;
-    uint2* out_q = a.spill[pass & 1];
+    const uint2* in_q = pass > 0 ? ((pass - 1) & 1 ? a.spill[1] : a.spill[0]) : nullptr;
+    uint2* out_q = (pass & 1 ? a.spill[1] : a.spill[0]);
     uint32_t* out_cnt = a.spill_count + pass;
     const bool may_spill = pass < a.n_passes - 1;
     const uint32_t budget = pass == 0 ? a.budget : a.budget_next;
@@ -384,7 +384,7 @@ __device__ __forceinline__ void defer(const DiscreteArgs& a, uint32_t key, uint3
 // Obstacle edge j of scene triangle `tri` (link-frame origin P with abs. error <= delta per
 // component, direction d = R^T (Q - P) with rel. error <= 6 eps) against the link-triangle LBVH.
 template <int STACK, int BS, bool COUNT>
-__device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, uint32_t j, float3 P,
+__device__ __forceinline__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, uint32_t j, float3 P,
                         float3 d, float delta, int* st, const uint32_t* flag, Ctr& ctr, int start,
                         const NarrowSpill& hs) {
     if (!L.tri_has_root) return 0;
@@ -450,7 +450,7 @@ __device__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, 
 // Scene triangle (link frame A with abs. error <= delta, e1 = R^T (B - A), e2 = R^T (C - A) with
 // rel. error <= 6 eps) against the link-edge LBVH.
 template <int STACK, int BS, bool COUNT>
-__device__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, float3 A, float3 e1,
+__device__ __forceinline__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, float3 A, float3 e1,
                        float3 e2, float delta, int* st, const uint32_t* flag, Ctr& ctr, int start,
                        const NarrowSpill& hs) {
     if (!L.edge_has_root) return 0;
@@ -549,16 +549,16 @@ __device__ __forceinline__ int narrow_candidate(const DiscreteArgs& a, uint32_t 
     const Tri t = a.sc.tri_prims[tri];
     const LinkTri m = link_tri(t, r0, r1, r2);
     if (DIR == 2) return r2o_tri<STACK, BS, COUNT>(a, L, key, tri, m.A, m.e1, m.e2, m.delta, st, flag, ctr, start, hs);
-    const uint32_t owned = __float_as_uint(t.b.w);
+    const uint32_t owned = __float_as_uint(t.b.w) & (j == 4 ? 7u : 1u << j);
+    // the owned edges A->B, B->C, C->A, one traversal site (the loop is kept rolled)
     int hit = 0;
-    if ((j == 4 || j == 0) && (owned & 1u))
-        hit = o2r_edge<STACK, BS, COUNT>(a, L, key, tri, 0u, m.A, m.e1, m.delta, st, flag, ctr, start, hs);
-    if (!hit && (j == 4 || j == 1) && (owned & 2u))
-        hit = o2r_edge<STACK, BS, COUNT>(a, L, key, tri, 1u, add(m.A, m.e1), rot(r0, r1, r2, sub(f3(t.c), f3(t.b))),
-                                         m.delta, st, flag, ctr, start, hs);
-    if (!hit && (j == 4 || j == 2) && (owned & 4u))
-        hit = o2r_edge<STACK, BS, COUNT>(a, L, key, tri, 2u, add(m.A, m.e2), neg(m.e2), m.delta, st, flag, ctr, start,
-                                         hs);
+#pragma clang loop unroll(disable)
+    for (uint32_t e = 0; e < 3 && !hit; ++e) {
+        if (!(owned & (1u << e))) continue;
+        const float3 P = e == 0 ? m.A : e == 1 ? add(m.A, m.e1) : add(m.A, m.e2);
+        const float3 d = e == 0 ? m.e1 : e == 1 ? rot(r0, r1, r2, sub(f3(t.c), f3(t.b))) : neg(m.e2);
+        hit = o2r_edge<STACK, BS, COUNT>(a, L, key, tri, e, P, d, m.delta, st, flag, ctr, start, hs);
+    }
     return hit;
 }
 
@@ -569,7 +569,7 @@ template <int DIR, int STACK, int BS, bool COUNT>
 __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
     __shared__ int stk[STACK * BS];
     int* st = stk + threadIdx.x;
-    const NarrowSpill hs{a.nspill[pass & 1], a.nspill_count + pass, a.nspill_cap,
+    const NarrowSpill hs{(pass & 1 ? a.nspill[1] : a.nspill[0]), a.nspill_count + pass, a.nspill_cap,
                          pass == 0 ? a.narrow_budget : a.narrow_budget_next, pass < a.narrow_passes - 1};
     const uint32_t stride = gridDim.x * BS;
     Ctr ctr;
@@ -618,7 +618,7 @@ __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
         }
         if (lane < qn) eval(q[lane]);
     } else {
-        const uint4* in = a.nspill[(pass - 1) & 1];
+        const uint4* in = ((pass - 1) & 1 ? a.nspill[1] : a.nspill[0]);
         const uint32_t total = min(a.nspill_count[pass - 1], a.nspill_cap);
         for (uint32_t i = blockIdx.x * BS + threadIdx.x; i < total; i += stride) {
             const uint4 e = in[i];

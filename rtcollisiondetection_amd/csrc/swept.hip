@@ -291,8 +291,8 @@ __global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, int pass) {
     __shared__ int stk[STACK * kBS];
     int* st = stk + threadIdx.x;
     const int S = a.robot_n_spheres_host;
-    const uint2* in_q = pass > 0 ? a.spill[(pass - 1) & 1] : nullptr;
-    uint2* out_q = a.spill[pass & 1];
+    const uint2* in_q = pass > 0 ? ((pass - 1) & 1 ? a.spill[1] : a.spill[0]) : nullptr;
+    uint2* out_q = (pass & 1 ? a.spill[1] : a.spill[0]);
     uint32_t* out_cnt = a.spill_count + pass;
     const bool may_spill = pass < a.n_passes - 1;
     const uint32_t n_tasks = pass == 0 ? a.n_items : min(a.spill_count[pass - 1], a.spill_cap);
