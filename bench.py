@@ -162,7 +162,7 @@ def main():
     run(3)
     c3 = ctx.stats()
     ctx.set_counting(False)
-    lanes = tuning['leaf_size'] if tuning else 4  # threads per leaf record = the scene BVH leaf size
+    lanes = ctx.get_param('leaf_scene_tri')  # threads per leaf record = the scene-triangle leaf size
     rec_lanes = lanes * c1['leaf_candidates']
     k1 = {'kernel': 'k_narrow<1> (obstacle->robot narrow pass)', 'ms': float(stage[3]),
           'bytes': (S_REC * rec_lanes + S_EVAL * c1['narrow_candidates'] + S_NODE * c1['node_fetches'] +

@@ -40,7 +40,8 @@ typedef struct {
     int32_t device;          /* HIP device index */
     uint32_t skip_base_link; /* 1: link 0 (the fixed base) is not checked in discrete modes
                                 (RTCD/Robot/robot.h:17,113-115 zeroes its OBB); default 1 */
-    uint32_t leaf_size;      /* BVH leaf size (1..16), 0 = default (4) */
+    uint32_t leaf_size;      /* BVH leaf size (1..16) of every BVH; 0 = per-BVH defaults (robot links 4,
+                                scene triangles 8, scene edges 2) */
     uint64_t mem_budget;     /* bytes of device memory the context may hold for work buffers; 0 = no limit */
 } rtcd_config;
 
@@ -195,20 +196,26 @@ typedef enum {
     RTCD_PARAM_NARROW_PASSES = 13,     /* robot->obstacle passes, 1..16; the last never hands off */
     RTCD_PARAM_O2R_BUDGET = 14,        /* obstacle->robot: the same three */
     RTCD_PARAM_O2R_BUDGET_NEXT = 15,
-    RTCD_PARAM_O2R_PASSES = 16
+    RTCD_PARAM_O2R_PASSES = 16,
+    RTCD_PARAM_LEAF_ROBOT = 17,        /* read-only here (rtcd_set_leaf_sizes changes them): BVH leaf */
+    RTCD_PARAM_LEAF_SCENE_TRI = 18,    /* sizes of the robot links, scene triangles and scene edges */
+    RTCD_PARAM_LEAF_SCENE_EDGE = 19
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
-/* Rebuilds the robot's and every scene's BVHs with another leaf size (1..16); scene ids are kept.
- * Replaces rtcd_config.leaf_size for the rest of the context's life.  No counterpart in RTCD, whose
- * OptiX builds choose their own leaves (RTCD/Meshes/mesh.h:745-821). */
+/* Rebuilds the robot's and every scene's BVHs with other leaf sizes (1..16; 0 keeps one): the robot
+ * link BVHs, the scene triangle BVH (discrete checks) and the scene edge BVH (swept checks).  Scene
+ * ids are kept; rtcd_config.leaf_size only sets the initial three.  rtcd_set_leaf_size sets all
+ * three.  No counterpart in RTCD, whose OptiX builds choose their own leaves
+ * (RTCD/Meshes/mesh.h:745-821). */
+rtcd_status rtcd_set_leaf_sizes(rtcd_ctx* ctx, uint32_t robot, uint32_t scene_tri, uint32_t scene_edge);
 rtcd_status rtcd_set_leaf_size(rtcd_ctx* ctx, uint32_t leaf_size);
 rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value);
 
 /* Autotuner.  Times `sample` (a request as the serving loop takes it; `out` is not used) on the device
  * under the context's launch-shape choices by coordinate descent and keeps the fastest: for discrete
- * samples the BVH leaf size (with RTCD_TUNE_LEAF_SIZE: 2, 4, 8, rebuilding the robot and every scene),
+ * samples the scene-triangle and robot BVH leaf sizes (with RTCD_TUNE_LEAF_SIZE: 2, 4, 8, rebuilding),
  * the narrow work-group size and the broad-phase hand-off passes and budgets for the sample's batch
- * size; for swept samples the sweep hand-off passes and budget.  Every configuration must reproduce
+ * size; for swept samples the scene-edge leaf size and the sweep hand-off passes and budget.  Every configuration must reproduce
  * the first one's flags, or the call fails with RTCD_ERR_INTERNAL and the context keeps its
  * settings.  The serving loop must be stopped.  No counterpart in RTCD (compile-time shapes,
  * CollisionDetector.h:396-433). */
@@ -216,7 +223,8 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value);
 typedef struct {
     double ms_before, ms_after;  /* device ms of the sample with the entry / chosen settings */
     uint32_t configs_timed;
-    uint32_t leaf_size, narrow_block, broad_passes, broad_budget, broad_budget_small, broad_budget_next;
+    uint32_t leaf_robot, leaf_scene_tri, leaf_scene_edge;
+    uint32_t narrow_block, broad_passes, broad_budget, broad_budget_small, broad_budget_next;
     uint32_t sweep_budget, sweep_passes;
 } rtcd_tune_report;
 rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* sample, uint32_t flags, rtcd_tune_report* report);

@@ -21,12 +21,13 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
            'rtcd_check_linear', 'rtcd_check_linear_device', 'rtcd_check_quadratic', 'rtcd_check_quadratic_device',
            'rtcd_quadratic_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
            'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param',
-           'rtcd_set_leaf_size', 'rtcd_autotune']
+           'rtcd_set_leaf_size', 'rtcd_set_leaf_sizes', 'rtcd_autotune']
 TUNE_LEAF_SIZE = 1
 PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad_budget': 4,
           'broad_budget_small': 5, 'broad_budget_next': 6, 'broad_passes': 7, 'sweep_passes': 8, 'graphs': 9,
           'narrow_block': 10, 'narrow_budget': 11, 'narrow_budget_next': 12, 'narrow_passes': 13,
-          'o2r_budget': 14, 'o2r_budget_next': 15, 'o2r_passes': 16}
+          'o2r_budget': 14, 'o2r_budget_next': 15, 'o2r_passes': 16, 'leaf_robot': 17, 'leaf_scene_tri': 18,
+          'leaf_scene_edge': 19}
 REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC = 1, 2, 3
 
 
@@ -71,7 +72,8 @@ class ServeStats(C.Structure):
 
 class TuneReport(C.Structure):
     _fields_ = [('ms_before', C.c_double), ('ms_after', C.c_double), ('configs_timed', C.c_uint32),
-                ('leaf_size', C.c_uint32), ('narrow_block', C.c_uint32), ('broad_passes', C.c_uint32),
+                ('leaf_robot', C.c_uint32), ('leaf_scene_tri', C.c_uint32), ('leaf_scene_edge', C.c_uint32),
+                ('narrow_block', C.c_uint32), ('broad_passes', C.c_uint32),
                 ('broad_budget', C.c_uint32), ('broad_budget_small', C.c_uint32), ('broad_budget_next', C.c_uint32),
                 ('sweep_budget', C.c_uint32), ('sweep_passes', C.c_uint32)]
 
@@ -119,6 +121,7 @@ def lib():
         L.rtcd_set_param.argtypes = [vp, u32, C.c_uint64]
         L.rtcd_get_param.argtypes = [vp, u32, C.POINTER(C.c_uint64)]
         L.rtcd_set_leaf_size.argtypes = [vp, u32]
+        L.rtcd_set_leaf_sizes.argtypes = [vp, u32, u32, u32]
         L.rtcd_autotune.argtypes = [vp, C.POINTER(Request), u32, C.POINTER(TuneReport)]
         for name in EXPORTS:
             if name not in ('rtcd_destroy', 'rtcd_last_error'):

@@ -152,7 +152,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.flags = d_out;
     a.stats = ws.stats.p;
     a.broad_pad = 1e-5f * (1.f + sc.scale);
-    a.leaf_lanes = (int) c->cfg.leaf_size;
+    a.leaf_lanes = (int) c->leaf_scene_tri;
     a.front = sc.front.p + foff;
     a.F = sc.front_levels.empty() ? 0 : F;
     a.unsure = ws.unsure.p;
@@ -218,7 +218,7 @@ static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, c
             P(ws.stream), P(d_poses), P(d_out), n, (uint64_t) dir,
             P(ws.T64.p), P(ws.Ti.p), P(ws.wbox.p), P(ws.cand.p), ws.cand.n, P(ws.flat.p), ws.flat.n,
             P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.spill.p), ws.spill.n, P(ws.nspill.p), ws.nspill.n,
-            P(ws.unsure.p), ws.unsure.n, c->narrow_budget, c->narrow_budget_next, c->narrow_passes, c->o2r_budget,
+            P(ws.unsure.p), ws.unsure.n, c->leaf_scene_tri, c->narrow_budget, c->narrow_budget_next, c->narrow_passes, c->o2r_budget,
             c->o2r_budget_next, c->o2r_passes,
             P(ws.scalars.p), P(ws.stats.p), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
             c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth, c->narrow_bs};
@@ -354,18 +354,23 @@ rtcd_status rtcd_create(const rtcd_config* cfg, rtcd_ctx** out) {
     return guarded([&]() -> rtcd_status {
         rtcd_config c{};
         c.skip_base_link = 1;
-        c.leaf_size = 4;
-        if (cfg) {
-            c = *cfg;
-            if (c.leaf_size == 0) c.leaf_size = 4;
-        }
-        if (c.leaf_size > 16) return fail(RTCD_ERR_INVALID_ARG, "leaf_size must be 1..16");
+        if (cfg) c = *cfg;
+        if (c.leaf_size > 16) return fail(RTCD_ERR_INVALID_ARG, "leaf_size must be 0..16");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(RTCD_ERR_NO_DEVICE, "no HIP device");
         if (c.device < 0 || c.device >= ndev) return fail(RTCD_ERR_NO_DEVICE, "device index out of range");
         RTCD_HIP(hipSetDevice(c.device));
         auto ctx = std::make_unique<rtcd_ctx>();
         ctx->cfg = c;
+        if (c.leaf_size) {
+            ctx->leaf_robot = ctx->leaf_scene_tri = ctx->leaf_scene_edge = c.leaf_size;
+        } else {
+            // measured on the dense scene: broad phase and narrow passes favour triangle leaves of 8
+            // (robot 4), the sweeps edge leaves of 2
+            ctx->leaf_robot = 4;
+            ctx->leaf_scene_tri = 8;
+            ctx->leaf_scene_edge = 2;
+        }
         // the context's own stream (used when a *_device call passes NULL) is ordered with the
         // device's null stream, so NULL means "after the caller's default-stream work"
         hipStream_t st = nullptr;
@@ -469,8 +474,8 @@ static rtcd_status load_robot_impl(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
         L.e2max = round_up(e2 * (1 + 1e-6));
         L.dmax = round_up(dm * (1 + 1e-6));
         BvhOut bt, be;
-        std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->cfg.leaf_size, s, bt);
-        std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->cfg.leaf_size, s, be);
+        std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->leaf_robot, s, bt);
+        std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->leaf_robot, s, be);
         L.tri_node0 = (int) all_tn.size();
         L.tri_prim0 = (int) all_t.size();
         L.n_tri = nt;
@@ -570,8 +575,8 @@ static std::shared_ptr<SceneRes> build_scene(rtcd_ctx* ctx, const float* verts, 
         eb[2 * e] = make_float4(std::min(p[0], q[0]), std::min(p[1], q[1]), std::min(p[2], q[2]), 0.f);
         eb[2 * e + 1] = make_float4(std::max(p[0], q[0]), std::max(p[1], q[1]), std::max(p[2], q[2]), 0.f);
     }
-    std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->cfg.leaf_size, s, sc->tri_bvh);
-    std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->cfg.leaf_size, s, sc->edge_bvh);
+    std::vector<int32_t> pt = build_bvh(tb, nt, (int) ctx->leaf_scene_tri, s, sc->tri_bvh);
+    std::vector<int32_t> pe = build_bvh(eb, ne, (int) ctx->leaf_scene_edge, s, sc->edge_bvh);
     {
         std::vector<BNode> hn = sc->tri_bvh.nodes.download(sc->tri_bvh.n_nodes, s);
         std::vector<FEntry> fr;
@@ -633,18 +638,26 @@ void rebuild_bvhs(rtcd_ctx* ctx) {
 
 extern "C" {
 
-rtcd_status rtcd_set_leaf_size(rtcd_ctx* ctx, uint32_t leaf_size) {
+rtcd_status rtcd_set_leaf_sizes(rtcd_ctx* ctx, uint32_t robot, uint32_t scene_tri, uint32_t scene_edge) {
     if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
-    if (leaf_size < 1 || leaf_size > 16) return fail(RTCD_ERR_INVALID_ARG, "leaf_size must be 1..16");
+    for (uint32_t v : {robot, scene_tri, scene_edge})
+        if (v > 16) return fail(RTCD_ERR_INVALID_ARG, "leaf sizes must be 1..16 (0 keeps the current one)");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
         if (ctx->serve) return fail(RTCD_ERR_NOT_READY, "stop the serving loop before rebuilding");
         RTCD_HIP(hipStreamSynchronize(ctx->main.stream));
-        ctx->cfg.leaf_size = leaf_size;
+        if (robot) ctx->leaf_robot = robot;
+        if (scene_tri) ctx->leaf_scene_tri = scene_tri;
+        if (scene_edge) ctx->leaf_scene_edge = scene_edge;
         rebuild_bvhs(ctx);
         return RTCD_OK;
     });
+}
+
+rtcd_status rtcd_set_leaf_size(rtcd_ctx* ctx, uint32_t leaf_size) {
+    if (leaf_size < 1 || leaf_size > 16) return fail(RTCD_ERR_INVALID_ARG, "leaf_size must be 1..16");
+    return rtcd_set_leaf_sizes(ctx, leaf_size, leaf_size, leaf_size);
 }
 
 rtcd_status rtcd_scene_destroy(rtcd_ctx* ctx, uint32_t scene_id) {
@@ -788,6 +801,10 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
         (param == RTCD_PARAM_BROAD_BUDGET ? ctx->broad_budget
          : param == RTCD_PARAM_BROAD_BUDGET_SMALL ? ctx->broad_budget_small : ctx->broad_budget_next) = (uint32_t) value;
         return RTCD_OK;
+    case RTCD_PARAM_LEAF_ROBOT:
+    case RTCD_PARAM_LEAF_SCENE_TRI:
+    case RTCD_PARAM_LEAF_SCENE_EDGE:
+        return fail(RTCD_ERR_INVALID_ARG, "leaf sizes change through rtcd_set_leaf_sizes (it rebuilds the BVHs)");
     case RTCD_PARAM_NARROW_BLOCK:
         if (value != 64 && value != 128 && value != 256) return fail(RTCD_ERR_INVALID_ARG, "narrow block must be 64, 128 or 256");
         ctx->narrow_bs = (uint32_t) value;
@@ -838,6 +855,9 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_BROAD_PASSES: *value = ctx->broad_passes; return RTCD_OK;
     case RTCD_PARAM_SWEEP_PASSES: *value = ctx->sweep_passes; return RTCD_OK;
     case RTCD_PARAM_GRAPHS: *value = ctx->use_graphs ? 1 : 0; return RTCD_OK;
+    case RTCD_PARAM_LEAF_ROBOT: *value = ctx->leaf_robot; return RTCD_OK;
+    case RTCD_PARAM_LEAF_SCENE_TRI: *value = ctx->leaf_scene_tri; return RTCD_OK;
+    case RTCD_PARAM_LEAF_SCENE_EDGE: *value = ctx->leaf_scene_edge; return RTCD_OK;
     case RTCD_PARAM_NARROW_BLOCK: *value = ctx->narrow_bs; return RTCD_OK;
     case RTCD_PARAM_NARROW_BUDGET: *value = ctx->narrow_budget; return RTCD_OK;
     case RTCD_PARAM_NARROW_BUDGET_NEXT: *value = ctx->narrow_budget_next; return RTCD_OK;

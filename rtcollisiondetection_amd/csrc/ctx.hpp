@@ -203,6 +203,9 @@ struct rtcd_ctx {
     uint32_t broad_passes = 12;                 // broad hand-off passes (the last never hands off)
     uint32_t sweep_passes = 8;                  // sweep hand-off passes
     uint32_t narrow_bs = 64;                    // work-group size of the discrete narrow passes
+    // BVH leaf sizes (rtcd_config.leaf_size, rtcd_set_leaf_sizes): robot link BVHs, scene triangle
+    // BVH (also the threads per broad-phase leaf record), scene edge BVH (sweeps)
+    uint32_t leaf_robot = 4, leaf_scene_tri = 8, leaf_scene_edge = 2;
     uint32_t narrow_budget = 16;                // robot->obstacle: link-BVH nodes before a walk hands off
     uint32_t narrow_budget_next = 8;            // the same in later passes
     uint32_t narrow_passes = 6;                 // robot->obstacle passes (the last never hands off)

@@ -82,6 +82,27 @@ double descend(Runner& R, Knob& k, double best) {
     return best;
 }
 
+// The same for one of the BVH leaf sizes (2, 4, 8): every candidate rebuilds the BVHs.
+double descend_leaf(Runner& R, uint32_t* field, double best) {
+    const uint32_t keep = *field;
+    uint32_t arg = keep;
+    for (uint32_t v : {2u, 4u, 8u}) {
+        if (v == keep) continue;
+        *field = v;
+        rebuild_bvhs(R.c);
+        const double t = R.time();
+        if (t < best) {
+            best = t;
+            arg = v;
+        }
+    }
+    if (*field != arg) {
+        *field = arg;
+        rebuild_bvhs(R.c);
+    }
+    return best;
+}
+
 }  // namespace
 
 extern "C" {
@@ -103,7 +124,7 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
     struct Restore {
         rtcd_ctx* c;
         bool counting;
-        uint32_t leaf, nbs, passes, bud, bud_small, bud_next, sw_bud, sw_passes, n_passes, n_bud, o_passes;
+        uint32_t leaf_r, leaf_t, leaf_e, nbs, passes, bud, bud_small, bud_next, sw_bud, sw_passes, n_passes, n_bud, o_passes;
         bool done = false;
         ~Restore() {
             c->counting = counting;
@@ -118,8 +139,10 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
             c->narrow_passes = n_passes;
             c->narrow_budget = n_bud;
             c->o2r_passes = o_passes;
-            if (c->cfg.leaf_size != leaf) {
-                c->cfg.leaf_size = leaf;
+            if (c->leaf_robot != leaf_r || c->leaf_scene_tri != leaf_t || c->leaf_scene_edge != leaf_e) {
+                c->leaf_robot = leaf_r;
+                c->leaf_scene_tri = leaf_t;
+                c->leaf_scene_edge = leaf_e;
                 try {
                     rebuild_bvhs(c);
                 } catch (...) {
@@ -128,7 +151,9 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
         }
     } restore{ctx,
               ctx->counting,
-              ctx->cfg.leaf_size,
+              ctx->leaf_robot,
+              ctx->leaf_scene_tri,
+              ctx->leaf_scene_edge,
               ctx->narrow_bs,
               ctx->broad_passes,
               ctx->broad_budget,
@@ -168,22 +193,8 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
             const size_t tasks = (size_t) r->n * (ctx->robot.n_links - (ctx->cfg.skip_base_link ? 1 : 0));
             const bool small = tasks < ctx->frontier_max_tasks;
             if (flags & RTCD_TUNE_LEAF_SIZE) {
-                const uint32_t keep = ctx->cfg.leaf_size;
-                uint32_t arg = keep;
-                for (uint32_t v : {2u, 4u, 8u}) {
-                    if (v == keep) continue;
-                    ctx->cfg.leaf_size = v;
-                    rebuild_bvhs(ctx);
-                    const double t = R.time();
-                    if (t < best) {
-                        best = t;
-                        arg = v;
-                    }
-                }
-                if (ctx->cfg.leaf_size != arg) {
-                    ctx->cfg.leaf_size = arg;
-                    rebuild_bvhs(ctx);
-                }
+                best = descend_leaf(R, &ctx->leaf_scene_tri, best);
+                best = descend_leaf(R, &ctx->leaf_robot, best);
             }
             std::vector<Knob> knobs = {
                 {&ctx->narrow_bs, {64, 128, 256}},
@@ -196,13 +207,16 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
             };
             for (Knob& k : knobs) best = descend(R, k, best);
         } else {
+            if (flags & RTCD_TUNE_LEAF_SIZE) best = descend_leaf(R, &ctx->leaf_scene_edge, best);
             std::vector<Knob> knobs = {{&ctx->sweep_passes, {4, 6, 8, 12}}, {&ctx->sweep_budget, {8, 16, 32, 64}}};
             for (Knob& k : knobs) best = descend(R, k, best);
         }
         rep->ms_before = before;
         rep->ms_after = R.time();
         rep->configs_timed = R.timed;
-        rep->leaf_size = ctx->cfg.leaf_size;
+        rep->leaf_robot = ctx->leaf_robot;
+        rep->leaf_scene_tri = ctx->leaf_scene_tri;
+        rep->leaf_scene_edge = ctx->leaf_scene_edge;
         rep->narrow_block = ctx->narrow_bs;
         rep->broad_passes = ctx->broad_passes;
         rep->broad_budget = ctx->broad_budget;

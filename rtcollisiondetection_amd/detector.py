@@ -26,7 +26,7 @@ def _f32(a):
 
 
 class Context:
-    def __init__(self, device=0, skip_base_link=True, leaf_size=4, mem_budget=0):
+    def __init__(self, device=0, skip_base_link=True, leaf_size=0, mem_budget=0):
         cfg = _lib.Config(device, int(bool(skip_base_link)), leaf_size, mem_budget)
         h = C.c_void_p()
         check(lib().rtcd_create(C.byref(cfg), C.byref(h)))
@@ -126,6 +126,10 @@ class Context:
     def set_leaf_size(self, leaf_size):
         """Rebuild the robot's and every scene's BVHs with another leaf size (scene ids are kept)."""
         check(lib().rtcd_set_leaf_size(self.h, int(leaf_size)))
+
+    def set_leaf_sizes(self, robot=0, scene_tri=0, scene_edge=0):
+        """Rebuild with separate leaf sizes for the robot, scene-triangle and scene-edge BVHs (0 keeps one)."""
+        check(lib().rtcd_set_leaf_sizes(self.h, int(robot), int(scene_tri), int(scene_edge)))
 
     def autotune(self, kind, sid, data, offsets=None, direction=_lib.TWO_WAY, n_wp=16, n_ctrl=8, leaf_size=True):
         """Tune the launch shapes on a sample (kind: _lib.REQ_DISCRETE / REQ_LINEAR / REQ_QUADRATIC) and

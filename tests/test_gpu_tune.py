@@ -34,7 +34,8 @@ def test_autotune_discrete_then_parity(panda, orc_panda):
         rep = c.autotune(_lib.REQ_DISCRETE, sid, q, direction=3, leaf_size=True)
         print('tune report', rep)
         assert rep['configs_timed'] >= 10
-        assert rep['leaf_size'] in (2, 4, 8) and rep['narrow_block'] in (64, 128, 256)
+        assert rep['leaf_scene_tri'] in (2, 4, 8) and rep['leaf_robot'] in (2, 4, 8)
+        assert rep['narrow_block'] in (64, 128, 256)
         assert rep['ms_after'] > 0 and rep['ms_before'] > 0
         assert c.get_param('narrow_block') == rep['narrow_block']
         for seed in (41, 42):
@@ -66,14 +67,17 @@ def test_autotune_sweeps_then_parity(panda, orc_panda):
         c.close()
 
 
-@pytest.mark.parametrize('leaf', [1, 2, 8, 16])
+@pytest.mark.parametrize('leaf', [(1, 1, 1), (2, 2, 2), (8, 8, 8), (16, 16, 16), (4, 8, 2), (16, 2, 8)])
 def test_leaf_size_rebuild_parity(panda, orc_panda, leaf):
     c = _ctx(panda)
     try:
         sc = scenes.random_boxes(8, seed=1, center_lo=(0.2, -0.4, 0.1), center_hi=(0.6, 0.4, 0.8))
         sid = c.create_scene(sc.verts, sc.tris)
         m = O.scene_mesh(sc)
-        c.set_leaf_size(leaf)
+        if leaf[0] == leaf[1] == leaf[2]:
+            c.set_leaf_size(leaf[0])
+        else:
+            c.set_leaf_sizes(*leaf)
         q = samplers.uniform_poses(384, panda.lower, panda.upper, seed=45)
         for d in (1, 2, 3):
             assert _mismatch(c.check_discrete(sid, q, d), O.check_discrete(orc_panda, m, q, d)) == 0, d
