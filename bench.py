@@ -45,31 +45,40 @@ def _boxes8():
     return s.verts, s.tris
 
 
-def serve_bench(ctx, rb, scene_ids, rate=2000.0, duration=2.0):
+def serve_bench(ctx, rb, scene_ids, rate=2000.0, duration=4.0, slots=3, lane_cus=0):
     """Serving stream: Poisson arrivals at `rate` req/s, 50/30/20 discrete/linear/quadratic, sizes
-    log-uniform 1..8,192, three resident scenes; submit->result latency, and the same requests run
-    one by one through the blocking entry points (the plain per-request launch)."""
+    log-uniform 1..8,192, three resident scenes; submit->result latency (and its queue part), and the
+    same requests run one by one through the blocking entry points (the plain per-request launch).
+    Slot 0 is the small-request lane (requests of at most 1,024 queries) on `lane_cus` CUs."""
     from rtcollisiondetection_amd import _lib
     from rtcollisiondetection_amd.serving import Server, make_stream, run_alone, run_stream
     reqs = make_stream(rb, scene_ids, rate_hz=rate, duration_s=duration, seed=11)
     items = sum(r.n for r in reqs)
-    srv = Server(ctx, slots=4)
+    ctx.set_param('serve_lane_cus', lane_cus)
+    srv = Server(ctx, slots=slots)
     try:
         res, wall = run_stream(srv, reqs, realtime=True)
         st = srv.stats()
     finally:
         srv.stop()
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3) if len(a) else None
     lat = np.array([x[2] for x in res])
-    d1k = np.array([x[2] for x in res if x[0] == _lib.REQ_DISCRETE and x[1] <= 1024])
+    d1k = [x for x in res if x[0] == _lib.REQ_DISCRETE and x[1] <= 1024]
+    band = np.array([x[2] for x in d1k if x[1] >= 512])
     served_out = [r.out.copy() for r in reqs]
     t = time.perf_counter()
     mism = 0
     for r, o in zip(reqs, served_out):
         mism += int(np.sum(run_alone(ctx, r).astype(bool) != o.astype(bool)))
     plain_wall = time.perf_counter() - t
-    return {'requests': len(reqs), 'items': int(items), 'rate_hz': rate, 'duration_s': duration, 'slots': 4,
-            'p50_ms': round(float(np.percentile(lat, 50)), 3), 'p99_ms': round(float(np.percentile(lat, 99)), 3),
-            'p99_ms_discrete_le_1024': round(float(np.percentile(d1k, 99)), 3) if len(d1k) else None,
+    return {'requests': len(reqs), 'items': int(items), 'rate_hz': rate, 'duration_s': duration, 'slots': slots,
+            'lane': {'max_queries': int(ctx.get_param('serve_lane_max')), 'kinds': int(ctx.get_param('serve_lane_kinds')),
+                     'cus': int(st['lane_cus'])},
+            'p50_ms': pct(lat, 50), 'p99_ms': pct(lat, 99),
+            'p50_ms_discrete_le_1024': pct([x[2] for x in d1k], 50),
+            'p99_ms_discrete_le_1024': pct([x[2] for x in d1k], 99),
+            'p99_queue_ms_discrete_le_1024': pct([x[3] for x in d1k], 99),
+            'p99_ms_discrete_512_1024': pct(band, 99), 'n_discrete_512_1024': int(len(band)),
             'served_items_per_s': round(items / wall, 1), 'wall_s': round(wall, 3),
             'plain_per_request_items_per_s': round(items / plain_wall, 1),
             'served_over_plain': round(plain_wall / wall, 2), 'mismatches_alone_vs_mix': mism,
@@ -260,7 +269,7 @@ def main():
         for _ in range(3):
             ctx.check_quadratic(sid, wp, 8)
         extra['dense_quadratic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
-        log(f'[bench] extra sweeps {extra}')
+        log(f'[bench] extra sweeps { {k: v for k, v in extra.items() if "traj" in k or "ms_device" in k} }')
         extra['serving'] = serve_bench(ctx, rb, [ctx.create_scene(*_boxes8()), sid2, sid])
         log(f'[bench] serving {extra["serving"]}')
 

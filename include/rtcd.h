@@ -158,11 +158,15 @@ typedef struct {
     double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> flags in the caller's buffer */
     double span_ms;                         /* first submit -> last completion */
     uint32_t queued_max;                    /* most work units queued at once */
+    uint32_t lane_cus;                      /* CUs slot 0 runs on when CU masks are in effect, else 0 */
+    double queue_p50_ms, queue_p99_ms;      /* submit -> first unit launched */
 } rtcd_serve_stats;
 /* Starts the worker with `slots` (1..16) concurrent work units, each slot with its own stream and
- * buffers.  A request is split into units of at most 4,096 poses or 1,024 trajectories; a free slot
- * takes the queued unit of the smallest request (FIFO among equal sizes) unless a unit has waited
- * more than 20 ms, and only while the units' estimated device bytes fit `mem_budget` (0 = 8 GiB).
+ * buffers.  A request is split into units of at most RTCD_PARAM_SERVE_CHUNK_DISCRETE poses or
+ * RTCD_PARAM_SERVE_CHUNK_SWEEP trajectories; a free slot takes the queued unit of the smallest request
+ * (FIFO among equal sizes) unless a unit has waited more than 20 ms, and only while the units'
+ * estimated device bytes fit `mem_budget` (0 = 8 GiB).  With more than one slot, slot 0 is a lane for
+ * requests of at most RTCD_PARAM_SERVE_LANE_MAX queries (it never ages and never takes larger ones).
  * input/offsets are read when a unit launches: the caller keeps them and `out` alive until
  * rtcd_wait returns.  A request whose unit alone exceeds the budget is rejected with RTCD_ERR_OOM. */
 rtcd_status rtcd_serve_start(rtcd_ctx* ctx, uint32_t slots, uint64_t mem_budget);
@@ -171,6 +175,8 @@ rtcd_status rtcd_submit(rtcd_ctx* ctx, const rtcd_request* req, uint64_t* id);
 rtcd_status rtcd_poll(rtcd_ctx* ctx, uint64_t id, int32_t* done);
 /* Blocks until request `id` is done; returns its status and frees its record. */
 rtcd_status rtcd_wait(rtcd_ctx* ctx, uint64_t id, double* latency_ms);
+/* The same, also returning the time from submit until the request's first unit launched. */
+rtcd_status rtcd_wait_timed(rtcd_ctx* ctx, uint64_t id, double* latency_ms, double* queued_ms);
 rtcd_status rtcd_serve_get_stats(rtcd_ctx* ctx, rtcd_serve_stats* stats, int32_t reset);
 
 /* Launch-shape parameters of the kernels.  No counterpart in the reference, whose launch shapes are
@@ -199,7 +205,14 @@ typedef enum {
     RTCD_PARAM_O2R_PASSES = 16,
     RTCD_PARAM_LEAF_ROBOT = 17,        /* read-only here (rtcd_set_leaf_sizes changes them): BVH leaf */
     RTCD_PARAM_LEAF_SCENE_TRI = 18,    /* sizes of the robot links, scene triangles and scene edges */
-    RTCD_PARAM_LEAF_SCENE_EDGE = 19
+    RTCD_PARAM_LEAF_SCENE_EDGE = 19,
+    RTCD_PARAM_SERVE_LANE_MAX = 20,    /* serving (read at rtcd_serve_start): slot 0 takes only requests of at
+                                          most this many queries, 0 = any request (default 1,024) */
+    RTCD_PARAM_SERVE_LANE_CUS = 21,    /* serving: CUs reserved for slot 0 by stream CU masks, the other
+                                          slots get the rest; 0 (default) = no masks; multiples of 32 */
+    RTCD_PARAM_SERVE_CHUNK_DISCRETE = 22, /* serving: poses per work unit (default 4,096) */
+    RTCD_PARAM_SERVE_CHUNK_SWEEP = 23, /* serving: trajectories per work unit (default 1,024) */
+    RTCD_PARAM_SERVE_LANE_KINDS = 24   /* serving: request kinds slot 0 takes, bit (RTCD_REQ_* - 1) */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 /* Rebuilds the robot's and every scene's BVHs with other leaf sizes (1..16; 0 keeps one): the robot

@@ -20,14 +20,15 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
            'rtcd_scene_destroy', 'rtcd_scene_info', 'rtcd_check_discrete', 'rtcd_check_discrete_device', 'rtcd_sync',
            'rtcd_check_linear', 'rtcd_check_linear_device', 'rtcd_check_quadratic', 'rtcd_check_quadratic_device',
            'rtcd_quadratic_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
-           'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param',
+           'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_wait_timed', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param',
            'rtcd_set_leaf_size', 'rtcd_set_leaf_sizes', 'rtcd_autotune']
 TUNE_LEAF_SIZE = 1
 PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad_budget': 4,
           'broad_budget_small': 5, 'broad_budget_next': 6, 'broad_passes': 7, 'sweep_passes': 8, 'graphs': 9,
           'narrow_block': 10, 'narrow_budget': 11, 'narrow_budget_next': 12, 'narrow_passes': 13,
           'o2r_budget': 14, 'o2r_budget_next': 15, 'o2r_passes': 16, 'leaf_robot': 17, 'leaf_scene_tri': 18,
-          'leaf_scene_edge': 19}
+          'leaf_scene_edge': 19, 'serve_lane_max': 20, 'serve_lane_cus': 21, 'serve_chunk_discrete': 22,
+          'serve_chunk_sweep': 23, 'serve_lane_kinds': 24}
 REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC = 1, 2, 3
 
 
@@ -67,7 +68,8 @@ class Request(C.Structure):
 class ServeStats(C.Structure):
     _fields_ = [('completed', C.c_uint64), ('failed', C.c_uint64), ('p50_ms', C.c_double), ('p99_ms', C.c_double),
                 ('mean_ms', C.c_double), ('max_ms', C.c_double), ('span_ms', C.c_double),
-                ('queued_max', C.c_uint32)]
+                ('queued_max', C.c_uint32), ('lane_cus', C.c_uint32), ('queue_p50_ms', C.c_double),
+                ('queue_p99_ms', C.c_double)]
 
 
 class TuneReport(C.Structure):
@@ -117,6 +119,7 @@ def lib():
         L.rtcd_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
         L.rtcd_poll.argtypes = [vp, C.c_uint64, C.POINTER(i32)]
         L.rtcd_wait.argtypes = [vp, C.c_uint64, C.POINTER(C.c_double)]
+        L.rtcd_wait_timed.argtypes = [vp, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.rtcd_serve_get_stats.argtypes = [vp, C.POINTER(ServeStats), i32]
         L.rtcd_set_param.argtypes = [vp, u32, C.c_uint64]
         L.rtcd_get_param.argtypes = [vp, u32, C.POINTER(C.c_uint64)]

@@ -536,6 +536,7 @@ rtcd_status rtcd_robot_load(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
         return fail(RTCD_ERR_INVALID_ARG, "null array in robot description");
     if (d->n_spheres && (!d->spheres || !d->sphere_link)) return fail(RTCD_ERR_INVALID_ARG, "null sphere arrays");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    if (ctx->serve) return fail(RTCD_ERR_NOT_READY, "stop the serving loop before loading a robot");
     return guarded([&]() -> rtcd_status {
         rtcd_status st = load_robot_impl(ctx, d);
         if (st == RTCD_OK) ctx->robot_copy = std::make_unique<RobotCopy>(*d);
@@ -837,6 +838,27 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
         *f = (uint32_t) value;
         return RTCD_OK;
     }
+    case RTCD_PARAM_SERVE_LANE_MAX:
+        if (value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "lane size must fit 32 bits");
+        ctx->serve_lane_max = (uint32_t) value;
+        return RTCD_OK;
+    case RTCD_PARAM_SERVE_LANE_CUS:
+        if (value % 32 != 0 || value > 224)
+            return fail(RTCD_ERR_INVALID_ARG, "lane CUs must be 0 or a multiple of 32 up to 224");
+        ctx->serve_lane_cus = (uint32_t) value;
+        return RTCD_OK;
+    case RTCD_PARAM_SERVE_LANE_KINDS:
+        if (value < 1 || value > 7) return fail(RTCD_ERR_INVALID_ARG, "lane kinds must be a non-empty mask of bits 0..2");
+        ctx->serve_lane_kinds = (uint32_t) value;
+        return RTCD_OK;
+    case RTCD_PARAM_SERVE_CHUNK_DISCRETE:
+        if (value < 64 || value > 65536) return fail(RTCD_ERR_INVALID_ARG, "discrete chunk must be 64..65536");
+        ctx->serve_chunk_discrete = (uint32_t) value;
+        return RTCD_OK;
+    case RTCD_PARAM_SERVE_CHUNK_SWEEP:
+        if (value < 16 || value > 16384) return fail(RTCD_ERR_INVALID_ARG, "sweep chunk must be 16..16384");
+        ctx->serve_chunk_sweep = (uint32_t) value;
+        return RTCD_OK;
     default:
         return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
@@ -865,6 +887,11 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_O2R_BUDGET: *value = ctx->o2r_budget; return RTCD_OK;
     case RTCD_PARAM_O2R_BUDGET_NEXT: *value = ctx->o2r_budget_next; return RTCD_OK;
     case RTCD_PARAM_O2R_PASSES: *value = ctx->o2r_passes; return RTCD_OK;
+    case RTCD_PARAM_SERVE_LANE_MAX: *value = ctx->serve_lane_max; return RTCD_OK;
+    case RTCD_PARAM_SERVE_LANE_CUS: *value = ctx->serve_lane_cus; return RTCD_OK;
+    case RTCD_PARAM_SERVE_CHUNK_DISCRETE: *value = ctx->serve_chunk_discrete; return RTCD_OK;
+    case RTCD_PARAM_SERVE_CHUNK_SWEEP: *value = ctx->serve_chunk_sweep; return RTCD_OK;
+    case RTCD_PARAM_SERVE_LANE_KINDS: *value = ctx->serve_lane_kinds; return RTCD_OK;
     default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
 }

@@ -242,6 +242,12 @@ struct rtcd_ctx {
         bool last_replayed = false;
     } dgraph;
     bool use_graphs = true;
+    // serving policy, read by rtcd_serve_start: slot 0 serves only requests of at most serve_lane_max
+    // queries (0: any request); with serve_lane_cus > 0 it runs on that many CUs and the other slots on
+    // the rest; requests are split into units of at most serve_chunk_discrete poses or
+    // serve_chunk_sweep trajectories
+    uint32_t serve_lane_max = 1024, serve_lane_cus = 0, serve_chunk_discrete = 4096, serve_chunk_sweep = 1024;
+    uint32_t serve_lane_kinds = 7;  // request kinds the lane takes: bit (kind - 1)
     // serving loop
     ServeState* serve = nullptr;
     size_t serve_bytes = 0;  // device bytes held by serving slots

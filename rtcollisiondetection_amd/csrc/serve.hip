@@ -2,14 +2,19 @@
 //
 // RTCD has no serving layer; its closest analogue is the NSTREAM ThreadPool in
 // CollisionDetector::check (RTCD/CollisionDetector/CollisionDetector.h:396-433, threadPool.h:10-92),
-// which launches fixed batches on round-robin streams and joins every NSTREAM tasks.  Here a worker
-// thread owns `slots` workspaces, each on its own stream, and runs work units on them:
-//  * a request is split into units of at most kChunkDiscrete poses or kChunkSweep trajectories
+// which launches fixed batches on round-robin streams and joins every NSTREAM tasks.  Here each of
+// `slots` slots has its own host thread, workspace and stream, and runs work units:
+//  * a request is split into units of at most chunk_discrete poses or chunk_sweep trajectories
 //    (queries are independent), so no unit holds a slot for long;
 //  * a free slot takes the queued unit of the smallest size class (log2 of the request size,
 //    FIFO within a class), unless some unit has waited longer than kAgingMs, which then goes
-//    first (except on slot 0, which always serves the smallest class), and batches with it the queued units of the same kind, scene and shape (FIFO) up to the
+//    first, and batches with it the queued units of the same kind, scene and shape (FIFO) up to the
 //    chunk size: many small requests become one launch (continuous batching);
+//  * with more than one slot, slot 0 is a lane for small requests (at most lane_max queries, of
+//    the kinds in lane_kinds): it never ages, never takes a larger request and batches at most
+//    lane_max queries, so a small request finds a slot that no bulk unit holds; with lane CUs set,
+//    stream CU masks give slot 0 its own CUs and the other slots the rest, so bulk kernels cannot
+//    fill the lane's CUs either;
 //  * a batch is admitted only while its estimated device bytes fit the memory budget;
 //  * admitted batches run concurrently on the device; completion is detected with events, each
 //    unit's flags are copied into its range of the caller's buffer, and a request completes (its
@@ -20,14 +25,14 @@
 #include <thread>
 #include <unordered_map>
 
+#include <hip/hip_ext.h>
+
 #include "ctx.hpp"
 
 namespace rtcd {
 
 namespace {
 constexpr double kAgingMs = 20.0;
-constexpr uint32_t kChunkDiscrete = 4096;
-constexpr uint32_t kChunkSweep = 1024;
 
 double now_ms() {
     using namespace std::chrono;
@@ -55,7 +60,7 @@ struct Req {
     uint64_t id = 0;
     rtcd_request r{};
     std::shared_ptr<SceneRes> scene;
-    double t_submit = 0, t_done = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;  // t_start: the first unit's launch
     rtcd_status status = RTCD_OK;
     std::string err;
     bool done = false;
@@ -84,17 +89,17 @@ struct Slot {
     Workspace ws;
     hipEvent_t done_ev = nullptr;
     Batch cur;
-    bool busy = false;
     PinnedBuf<float> in;
     PinnedBuf<uint32_t> offs;
     PinnedBuf<uint32_t> out;
+    PinnedBuf<uint32_t> sv;  // the workspace's status scalars, copied back with the flags
+    std::thread th;
 };
 }  // namespace
 
 struct ServeState {
     rtcd_ctx* ctx = nullptr;
     std::vector<std::unique_ptr<Slot>> slots;
-    std::thread worker;
     std::mutex m;
     std::condition_variable cv_work, cv_done;
     std::deque<Unit> queue;
@@ -102,7 +107,11 @@ struct ServeState {
     uint64_t next_id = 1;
     bool stop = false;
     size_t budget = 0, in_use = 0;
-    std::vector<double> lat;
+    uint32_t chunk_discrete = 4096, chunk_sweep = 1024;
+    uint32_t lane_max = 0;   // slot 0 takes only requests of at most lane_max queries (0: any)
+    uint32_t lane_kinds = 7; // ... and of these kinds (bit kind - 1)
+    uint32_t lane_cus = 0;   // CUs of slot 0's stream mask (0: no masks)
+    std::vector<double> lat, qlat;
     uint64_t completed = 0, failed = 0;
     double t_first = 0, t_last = 0;
     size_t queued_max = 0;
@@ -125,7 +134,7 @@ size_t first_waypoint(const Unit& u) {
     return (size_t) u.i0 * r.n_wp;
 }
 
-// Launch batch b on slot sl (worker thread, no lock held): stage the units' inputs one after the
+// Launch batch b on slot sl (the slot's thread, no lock held): stage the units' inputs one after the
 // other, enqueue one check over the whole batch.
 void launch(ServeState* S, Slot& sl, const Batch& b) {
     rtcd_ctx* c = S->ctx;
@@ -162,6 +171,8 @@ void launch(ServeState* S, Slot& sl, const Batch& b) {
         enqueue_quadratic(c, ws, sc, ws.h_poses.p, b.n, r.n_wp, r.n_ctrl, nullptr, ws.h_out.p, nullptr, 0);
     }
     sl.out.reserve(b.n);
+    sl.sv.reserve(4);
+    RTCD_HIP(hipMemcpyAsync(sl.sv.p, ws.scalars.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     RTCD_HIP(hipMemcpyAsync(sl.out.p, ws.h_out.p, sizeof(uint32_t) * b.n, hipMemcpyDeviceToHost, s));
     RTCD_HIP(hipEventRecord(sl.done_ev, s));
 }
@@ -175,16 +186,14 @@ void complete(ServeState* S, Slot& sl) {
     SceneRes& sc = *b.units[0].q->scene;
     if (r.kind == RTCD_REQ_DISCRETE) {
         // re-runs synchronously on the slot stream if a candidate segment or the FP64 queue overflowed
-        std::vector<uint32_t> sv = sl.ws.scalars.download(2, sl.ws.stream);
-        if (sv[1]) {
+        if (sl.sv.p[1]) {
             finish_discrete(c, sl.ws, sc, r.direction, sl.ws.h_poses.p, b.n, sl.ws.h_out.p, 0, nullptr);
             RTCD_HIP(hipMemcpyAsync(sl.out.p, sl.ws.h_out.p, sizeof(uint32_t) * b.n, hipMemcpyDeviceToHost,
                                     sl.ws.stream));
             RTCD_HIP(hipStreamSynchronize(sl.ws.stream));
         }
     } else {
-        std::vector<uint32_t> sv = sl.ws.scalars.download(3, sl.ws.stream);
-        if (sv[1] & 7u) {
+        if (sl.sv.p[1] & 7u) {
             auto rerun = [&]() {
                 if (r.kind == RTCD_REQ_LINEAR)
                     enqueue_linear(c, sl.ws, sc, sl.ws.h_poses.p, sl.ws.h_offsets.p, b.n, b.nw, sl.ws.h_out.p, nullptr,
@@ -211,15 +220,22 @@ bool same_shape(const Unit& a, const Unit& b) {
            (x.kind != RTCD_REQ_QUADRATIC || (x.n_wp == y.n_wp && x.n_ctrl == y.n_ctrl));
 }
 
+// Whether the small-request lane (slot 0) may take unit u.
+bool lane_ok(const ServeState* S, const Unit& u, uint32_t lane) {
+    return u.q->r.n <= lane && ((S->lane_kinds >> (u.q->r.kind - 1)) & 1u);
+}
+
 // Next batch to admit (lock held).  Its lead unit is the smallest size class, FIFO within it,
 // unless `aging` and the oldest unit has waited longer than kAgingMs; then queued units of the same
 // shape join it in submit order while the batch stays within the chunk size and the memory budget.
-// The batch's units are removed from the queue.  Empty if nothing fits.
-Batch pick_batch(ServeState* S, double tnow, bool aging) {
+// With `lane` > 0 only units of requests of at most `lane` queries and of the lane's kinds are taken,
+// and the batch holds at most `lane` queries.  The batch's units are removed from the queue.  Empty if nothing fits.
+Batch pick_batch(ServeState* S, double tnow, bool aging, uint32_t lane) {
     Batch b;
     auto lead = S->queue.end();
     for (auto it = S->queue.begin(); it != S->queue.end(); ++it) {
         if (S->in_use + it->bytes > S->budget) continue;
+        if (lane && !lane_ok(S, *it, lane)) continue;
         if (aging && tnow - it->q->t_submit > kAgingMs) {  // the queue is in submit order
             lead = it;
             break;
@@ -229,7 +245,8 @@ Batch pick_batch(ServeState* S, double tnow, bool aging) {
     if (lead == S->queue.end()) return b;
     const Unit head = *lead;
     const rtcd_ctx* c = S->ctx;
-    const uint32_t chunk = head.q->r.kind == RTCD_REQ_DISCRETE ? kChunkDiscrete : kChunkSweep;
+    uint32_t chunk = head.q->r.kind == RTCD_REQ_DISCRETE ? S->chunk_discrete : S->chunk_sweep;
+    if (lane) chunk = std::min(chunk, lane);
     auto add = [&](const Unit& u) {
         b.first.push_back(b.n);
         b.units.push_back(u);
@@ -240,7 +257,7 @@ Batch pick_batch(ServeState* S, double tnow, bool aging) {
     b.bytes = head.bytes;
     S->queue.erase(lead);
     for (auto it = S->queue.begin(); it != S->queue.end() && b.n < chunk;) {
-        if (!same_shape(*it, head) || b.n + it->n > chunk) {
+        if (!same_shape(*it, head) || b.n + it->n > chunk || (lane && !lane_ok(S, *it, lane))) {
             ++it;
             continue;
         }
@@ -266,6 +283,7 @@ void unit_done(ServeState* S, const Unit& u, rtcd_status st, const std::string& 
     q->scene.reset();
     if (q->status == RTCD_OK) {
         S->lat.push_back(q->t_done - q->t_submit);
+        S->qlat.push_back(q->t_start - q->t_submit);
         S->completed++;
     } else {
         S->failed++;
@@ -274,77 +292,49 @@ void unit_done(ServeState* S, const Unit& u, rtcd_status st, const std::string& 
     S->cv_done.notify_all();
 }
 
-void worker_main(ServeState* S) {
+// One host thread per slot: take a batch (lock held), launch it, wait for its event, scatter the
+// flags, record the units' completion.  Slots never wait on each other's host work, so a small
+// request on the lane is not held up by a bulk launch being enqueued on another slot.
+void slot_main(ServeState* S, size_t si) {
     (void) hipSetDevice(S->ctx->cfg.device);
+    Slot& sl = *S->slots[si];
+    // slot 0, with other slots beside it, is the small-request lane: smallest request first (no
+    // aging), only requests of at most lane_max queries of the lane's kinds
+    const bool lane_slot = si == 0 && S->slots.size() > 1;
     std::unique_lock<std::mutex> lk(S->m);
     for (;;) {
-        bool progressed = false;
-        // 1. completions
-        for (auto& up : S->slots) {
-            Slot& sl = *up;
-            if (!sl.busy) continue;
-            if (hipEventQuery(sl.done_ev) == hipErrorNotReady) continue;
-            lk.unlock();
-            rtcd_status st = RTCD_OK;
-            std::string err;
-            try {
-                RTCD_HIP(hipEventSynchronize(sl.done_ev));
-                complete(S, sl);
-            } catch (const HipError& e) {
-                st = e.code == hipErrorOutOfMemory ? RTCD_ERR_OOM : RTCD_ERR_HIP;
-                err = e.what();
-            } catch (const std::exception& e) {
-                st = RTCD_ERR_INTERNAL;
-                err = e.what();
-            }
-            lk.lock();
-            for (const Unit& u : sl.cur.units) unit_done(S, u, st, err);
-            S->in_use -= sl.cur.bytes;
-            sl.busy = false;
-            progressed = true;
+        Batch b;
+        const double tnow = now_ms();
+        if (!S->queue.empty()) b = pick_batch(S, tnow, !lane_slot, lane_slot ? S->lane_max : 0);
+        if (b.units.empty()) {
+            if (S->stop && S->queue.empty()) break;
+            // woken by submits, completions (they free budget) and stop
+            S->cv_work.wait_for(lk, std::chrono::milliseconds(5));
+            continue;
         }
-        // 2. admissions.  Slot 0 always takes the smallest queued request first (no aging), so a
-        // small request never waits behind aged bulk work on every slot.
-        for (size_t si = 0; si < S->slots.size(); ++si) {
-            Slot& sl = *S->slots[si];
-            if (sl.busy || S->queue.empty()) continue;
-            Batch b = pick_batch(S, now_ms(), si != 0 || S->slots.size() == 1);
-            if (b.units.empty()) break;
-            sl.cur = std::move(b);
-            S->in_use += sl.cur.bytes;
-            sl.busy = true;
-            lk.unlock();
-            rtcd_status st = RTCD_OK;
-            std::string err;
-            try {
-                launch(S, sl, sl.cur);
-            } catch (const HipError& e) {
-                st = e.code == hipErrorOutOfMemory ? RTCD_ERR_OOM : RTCD_ERR_HIP;
-                err = e.what();
-            } catch (const std::exception& e) {
-                st = RTCD_ERR_INTERNAL;
-                err = e.what();
-            }
-            lk.lock();
-            if (st != RTCD_OK) {
-                for (const Unit& u : sl.cur.units) unit_done(S, u, st, err);
-                S->in_use -= sl.cur.bytes;
-                sl.busy = false;
-            }
-            progressed = true;
+        for (const Unit& u : b.units)
+            if (u.q->t_start == 0) u.q->t_start = tnow;
+        S->in_use += b.bytes;
+        sl.cur = std::move(b);
+        lk.unlock();
+        rtcd_status st = RTCD_OK;
+        std::string err;
+        try {
+            launch(S, sl, sl.cur);
+            RTCD_HIP(hipEventSynchronize(sl.done_ev));
+            complete(S, sl);
+        } catch (const HipError& e) {
+            st = e.code == hipErrorOutOfMemory ? RTCD_ERR_OOM : RTCD_ERR_HIP;
+            err = e.what();
+        } catch (const std::exception& e) {
+            st = RTCD_ERR_INTERNAL;
+            err = e.what();
         }
-        bool busy = false;
-        for (auto& up : S->slots) busy |= up->busy;
-        if (S->stop && !busy && S->queue.empty()) break;
-        if (!progressed) {
-            if (busy) {
-                lk.unlock();
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
-                lk.lock();
-            } else {
-                S->cv_work.wait_for(lk, std::chrono::milliseconds(5));
-            }
-        }
+        lk.lock();
+        for (const Unit& u : sl.cur.units) unit_done(S, u, st, err);
+        S->in_use -= sl.cur.bytes;
+        sl.cur = Batch{};
+        S->cv_work.notify_all();
     }
 }
 
@@ -358,7 +348,8 @@ void serve_shutdown(rtcd_ctx* ctx) {
         S->stop = true;
     }
     S->cv_work.notify_all();
-    if (S->worker.joinable()) S->worker.join();
+    for (auto& sl : S->slots)
+        if (sl->th.joinable()) sl->th.join();
     for (auto& sl : S->slots) {
         if (sl->done_ev) (void) hipEventDestroy(sl->done_ev);
         sl->ws.destroy();
@@ -383,16 +374,56 @@ rtcd_status rtcd_serve_start(rtcd_ctx* ctx, uint32_t slots, uint64_t mem_budget)
         auto S = std::make_unique<ServeState>();
         S->ctx = ctx;
         S->budget = mem_budget ? mem_budget : ((size_t) 8 << 30);
-        for (uint32_t i = 0; i < slots; ++i) {
-            auto sl = std::make_unique<Slot>();
-            hipStream_t st = nullptr;
-            RTCD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-            sl->ws.init(st, true);
-            RTCD_HIP(hipEventCreateWithFlags(&sl->done_ev, hipEventDisableTiming));
-            S->slots.push_back(std::move(sl));
+        S->chunk_discrete = ctx->serve_chunk_discrete;
+        S->chunk_sweep = ctx->serve_chunk_sweep;
+        S->lane_max = ctx->serve_lane_max;
+        S->lane_kinds = ctx->serve_lane_kinds;
+        // CU masks: on the 256-CU part (8 XCDs of 32 CUs), CU i goes to slot 0 iff
+        // ((i mod 8) - floor(i / 32)) mod 8 < lane_cus / 32.  Whether the driver maps mask bit i to XCD
+        // i mod 8 (interleaved) or to XCD i / 32 (blocked), both masks then hold CUs on every XCD.
+        std::vector<uint32_t> lane_mask, bulk_mask;
+        int n_cu = 0;
+        RTCD_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->cfg.device));
+        if (slots > 1 && ctx->serve_lane_cus && n_cu == 256) {
+            lane_mask.assign(8, 0u);
+            bulk_mask.assign(8, 0u);
+            const int m = (int) ctx->serve_lane_cus / 32;
+            for (int i = 0; i < 256; ++i) {
+                const bool lane = (((i % 8) - i / 32) % 8 + 8) % 8 < m;
+                (lane ? lane_mask : bulk_mask)[i / 32] |= 1u << (i % 32);
+            }
         }
+        try {
+            for (uint32_t i = 0; i < slots; ++i) {
+                S->slots.push_back(std::make_unique<Slot>());
+                Slot& sl = *S->slots.back();
+                hipStream_t st = nullptr;
+                if (!lane_mask.empty()) {
+                    std::vector<uint32_t>& mk = i == 0 ? lane_mask : bulk_mask;
+                    if (hipExtStreamCreateWithCUMask(&st, (uint32_t) mk.size(), mk.data()) != hipSuccess) {
+                        (void) hipGetLastError();
+                        st = nullptr;
+                        if (i > 0) throw std::runtime_error("a CU-masked serving stream could not be created");
+                        lane_mask.clear();  // no masks on any slot
+                    }
+                }
+                if (!st) RTCD_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+                sl.ws.stream = st;  // owned from here on
+                sl.ws.own_stream = true;
+                sl.ws.init(st, true);
+                RTCD_HIP(hipEventCreateWithFlags(&sl.done_ev, hipEventDisableTiming));
+            }
+        } catch (...) {
+            for (auto& sl : S->slots) {
+                if (sl->done_ev) (void) hipEventDestroy(sl->done_ev);
+                sl->ws.destroy();
+            }
+            throw;
+        }
+        S->lane_cus = lane_mask.empty() ? 0 : ctx->serve_lane_cus;
         ctx->serve = S.release();
-        ctx->serve->worker = std::thread(worker_main, ctx->serve);
+        for (size_t i = 0; i < ctx->serve->slots.size(); ++i)
+            ctx->serve->slots[i]->th = std::thread(slot_main, ctx->serve, i);
         return RTCD_OK;
     });
 }
@@ -436,7 +467,7 @@ rtcd_status rtcd_submit(rtcd_ctx* ctx, const rtcd_request* r, uint64_t* id) {
     q->r = *r;
     q->scene = sc;
     // units of at most `chunk` queries; the size class orders admission (small requests first)
-    const uint32_t chunk = r->kind == RTCD_REQ_DISCRETE ? kChunkDiscrete : kChunkSweep;
+    const uint32_t chunk = r->kind == RTCD_REQ_DISCRETE ? S->chunk_discrete : S->chunk_sweep;
     int cls = 0;
     while ((1u << cls) < r->n && cls < 31) ++cls;
     std::vector<Unit> units;
@@ -464,7 +495,7 @@ rtcd_status rtcd_submit(rtcd_ctx* ctx, const rtcd_request* r, uint64_t* id) {
         S->queued_max = std::max(S->queued_max, S->queue.size());
         S->reqs[q->id] = std::move(q);
     }
-    S->cv_work.notify_one();
+    S->cv_work.notify_all();  // the lane slot may not take this request: wake every slot
     return RTCD_OK;
 }
 
@@ -479,7 +510,9 @@ rtcd_status rtcd_poll(rtcd_ctx* ctx, uint64_t id, int32_t* done) {
     return RTCD_OK;
 }
 
-rtcd_status rtcd_wait(rtcd_ctx* ctx, uint64_t id, double* latency_ms) {
+rtcd_status rtcd_wait(rtcd_ctx* ctx, uint64_t id, double* latency_ms) { return rtcd_wait_timed(ctx, id, latency_ms, nullptr); }
+
+rtcd_status rtcd_wait_timed(rtcd_ctx* ctx, uint64_t id, double* latency_ms, double* queued_ms) {
     if (!ctx) return fail(RTCD_ERR_INVALID_ARG, "null ctx");
     ServeState* S = ctx->serve;
     if (!S) return fail(RTCD_ERR_NOT_READY, "serving loop not running");
@@ -489,6 +522,7 @@ rtcd_status rtcd_wait(rtcd_ctx* ctx, uint64_t id, double* latency_ms) {
     Req* q = it->second.get();
     S->cv_done.wait(g, [&] { return q->done; });
     if (latency_ms) *latency_ms = q->t_done - q->t_submit;
+    if (queued_ms) *queued_ms = q->t_start > 0 ? q->t_start - q->t_submit : 0.0;
     rtcd_status st = q->status;
     std::string err = q->err;
     S->reqs.erase(it);
@@ -501,9 +535,16 @@ rtcd_status rtcd_serve_get_stats(rtcd_ctx* ctx, rtcd_serve_stats* out, int32_t r
     ServeState* S = ctx->serve;
     if (!S) return fail(RTCD_ERR_NOT_READY, "serving loop not running");
     std::lock_guard<std::mutex> g(S->m);
-    std::vector<double> l = S->lat;
+    std::vector<double> l = S->lat, ql = S->qlat;
     std::sort(l.begin(), l.end());
-    auto pct = [&](double p) { return l.empty() ? 0.0 : l[std::min(l.size() - 1, (size_t) (p * (l.size() - 1) + 0.5))]; };
+    std::sort(ql.begin(), ql.end());
+    auto pct_of = [](const std::vector<double>& v, double p) {
+        return v.empty() ? 0.0 : v[std::min(v.size() - 1, (size_t) (p * (v.size() - 1) + 0.5))];
+    };
+    auto pct = [&](double p) { return pct_of(l, p); };
+    out->queue_p50_ms = pct_of(ql, 0.50);
+    out->queue_p99_ms = pct_of(ql, 0.99);
+    out->lane_cus = S->lane_cus;
     out->completed = S->completed;
     out->failed = S->failed;
     out->p50_ms = pct(0.50);
@@ -516,6 +557,7 @@ rtcd_status rtcd_serve_get_stats(rtcd_ctx* ctx, rtcd_serve_stats* out, int32_t r
     out->queued_max = (uint32_t) S->queued_max;
     if (reset) {
         S->lat.clear();
+        S->qlat.clear();
         S->completed = S->failed = 0;
         S->queued_max = 0;
         S->t_first = now_ms();

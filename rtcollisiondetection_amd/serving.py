@@ -53,13 +53,19 @@ class Server:
         return rid.value
 
     def wait(self, rid):
-        lat = C.c_double()
+        """Flags and submit->done latency (ms) of request rid."""
+        out, lat, _ = self.wait_timed(rid)
+        return out, lat
+
+    def wait_timed(self, rid):
+        """Flags, submit->done latency and submit->first launch (queue) time, in ms."""
+        lat, queued = C.c_double(), C.c_double()
         try:
-            check(lib().rtcd_wait(self.ctx.h, rid, C.byref(lat)))
+            check(lib().rtcd_wait_timed(self.ctx.h, rid, C.byref(lat), C.byref(queued)))
         finally:
             with self._lock:
                 req, _ = self._keep.pop(rid)
-        return req.out, lat.value
+        return req.out, lat.value, queued.value
 
     def stats(self, reset=False):
         st = _lib.ServeStats()
@@ -101,7 +107,7 @@ def make_stream(robot, scene_ids, rate_hz=2000.0, duration_s=5.0, seed=11, max_s
 
 def run_stream(server, reqs, realtime=True):
     """Submit reqs at their arrival times (real time) and wait for all.  Returns per-request
-    (kind, n, latency_ms) and the wall time."""
+    (kind, n, latency_ms, queued_ms) and the wall time."""
     ids = []
     t0 = time.perf_counter()
     for r in reqs:
@@ -112,8 +118,8 @@ def run_stream(server, reqs, realtime=True):
         ids.append(server.submit(r))
     res = []
     for r, rid in zip(reqs, ids):
-        out, lat = server.wait(rid)
-        res.append((r.kind, r.n, lat))
+        out, lat, queued = server.wait_timed(rid)
+        res.append((r.kind, r.n, lat, queued))
     return res, time.perf_counter() - t0
 
 

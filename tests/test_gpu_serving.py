@@ -60,3 +60,38 @@ def test_admission_rejects_oversized_and_bad_requests(env, panda):
         assert np.array_equal(out, ctx.check_discrete(ids[0], ok.data, 3)) and lat > 0
     finally:
         srv.stop()
+
+
+def test_lane_policy_chunks_and_cu_masks(panda):
+    """Small-request lane, multi-unit requests split across lane and bulk slots, CU-masked streams."""
+    from rtcollisiondetection_amd.detector import Context
+    ctx = Context()
+    try:
+        ctx.load_robot(panda)
+        sh = scenes.scene_shelf()
+        ids = [ctx.create_scene(sh.verts, sh.tris)]
+        with pytest.raises(_lib.RtcdError):
+            ctx.set_param('serve_lane_cus', 40)  # not a multiple of 32
+        with pytest.raises(_lib.RtcdError):
+            ctx.set_param('serve_lane_kinds', 0)
+        for k, v in (('serve_lane_max', 64), ('serve_lane_kinds', 1), ('serve_lane_cus', 64),
+                     ('serve_chunk_discrete', 256), ('serve_chunk_sweep', 64)):
+            ctx.set_param(k, v)
+            assert ctx.get_param(k) == v
+        reqs = make_stream(panda, ids, rate_hz=400.0, duration_s=0.25, seed=6, max_size=1500)
+        assert any(r.n > 256 for r in reqs)
+        srv = Server(ctx, slots=3)
+        try:
+            ids_ = [srv.submit(r) for r in reqs]
+            timed = [srv.wait_timed(i) for i in ids_]
+            st = srv.stats()
+        finally:
+            srv.stop()
+        print('lane CUs in effect:', st['lane_cus'], 'queue p99', st['queue_p99_ms'])
+        assert st['lane_cus'] in (0, 64)
+        assert st['completed'] == len(reqs) and st['failed'] == 0
+        for (out, lat, queued), r in zip(timed, reqs):
+            assert 0.0 <= queued <= lat
+            assert np.array_equal(out, run_alone(ctx, r)), (r.kind, r.n)
+    finally:
+        ctx.close()
