@@ -562,6 +562,170 @@ __device__ __forceinline__ int narrow_candidate(const DiscreteArgs& a, uint32_t 
     return hit;
 }
 
+// Obstacle->robot pass without hand-off, with persistent lanes: every lane holds its own candidate
+// (scene triangle, pose-link) and, once that candidate is decided, takes the next one from the
+// wave's LDS queue, which the wave tops up from its share of the leaf records.  A wave that ran its
+// 64 candidates to completion together would keep every lane waiting for the longest walk: the
+// walks are short on average (a few link-BVH nodes per edge) but vary widely.  The walks, the tests
+// and their FP64 hand-over are those of o2r_edge.
+template <int STACK, int BS, bool COUNT>
+__device__ __forceinline__ void o2r_lanes(const DiscreteArgs& a, uint2* q, int* st, Ctr& ctr) {
+    constexpr int kRefill = 16;  // idle lanes that make the wave refill
+    constexpr int kSteps = 8;    // node visits between refill checks
+    const int lane = (int) (threadIdx.x & 63);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t lanes = (uint32_t) a.leaf_lanes;
+    const uint32_t gpw = 64u / lanes, g = (uint32_t) lane / lanes, sub_k = (uint32_t) lane - g * lanes;
+    const uint32_t wave = blockIdx.x * (BS / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (BS / 64);
+    const uint32_t total = a.seg_pre[kSeg];
+    const uint32_t NL = (uint32_t) a.n_links;
+    uint32_t base = wave * gpw;  // next leaf-record group of this wave
+    int qn = 0;                  // queued candidates (wave-uniform)
+    // lane state: candidate, its link-frame triangle, the current edge and its walk
+    bool act = false;
+    uint32_t key = 0, tri = 0, own = 0, e = 0;
+    uint32_t* flag = a.flags;
+    int node0 = 0, prim0 = 0, sp = 0, ni = 0, iter = 0;
+    float3 A = {}, e1 = {}, e2 = {}, d1 = {}, P = {}, d = {}, inv = {};
+    float delta = 0.f, pad = 0.f;
+    MTBands bd = {};
+    // next owned edge whose segment meets the link box: its walk starts at the root; false if none
+    auto next_edge = [&]() -> bool {
+        const LinkInfo& L = a.robot->link[key % NL];
+        while (own) {
+            e = (uint32_t) __builtin_ctz(own);
+            own &= own - 1u;
+            P = e == 0 ? A : e == 1 ? add(A, e1) : add(A, e2);
+            d = e == 0 ? e1 : e == 1 ? d1 : neg(e2);
+            inv = safe_inv(d);
+            const float D = l1(d.x, d.y, d.z) * (1.f + 8.f * kEps32);
+            const float ad = 6.1f * kEps32 * D;
+            pad = 2.f * delta + 2.f * ad + 1e-6f * L.scale;
+            if (!slab(P, inv, L.lo[0], L.lo[1], L.lo[2], L.hi[0], L.hi[1], L.hi[2], pad)) continue;
+            const float S = l1(P.x - L.c[0], P.y - L.c[1], P.z - L.c[2]) * (1.f + 2.f * kEps32) + 3.f * delta + L.rv;
+            bd = make_bands(D, ad, S, delta + kEps32 * S, L.e1max, kEps32 * L.e1max, L.e2max, kEps32 * L.e2max);
+            sp = 0;
+            ni = 0;
+            iter = 0;
+            return true;
+        }
+        return false;
+    };
+    auto start = [&](uint2 c) -> bool {
+        key = c.x;
+        tri = c.y;
+        flag = a.flags + key / NL;
+        if (load_flag(flag)) return false;
+        if (COUNT) ctr.tris++;
+        const LinkInfo& L = a.robot->link[key % NL];
+        if (!L.tri_has_root) return false;
+        node0 = L.tri_node0;
+        prim0 = L.tri_prim0;
+        const float4 r0 = a.Ti[(size_t) key * 3], r1 = a.Ti[(size_t) key * 3 + 1], r2 = a.Ti[(size_t) key * 3 + 2];
+        const Tri t = a.sc.tri_prims[tri];
+        const LinkTri m = link_tri(t, r0, r1, r2);
+        A = m.A;
+        e1 = m.e1;
+        e2 = m.e2;
+        delta = m.delta;
+        d1 = rot(r0, r1, r2, sub(f3(t.c), f3(t.b)));
+        own = __float_as_uint(t.b.w) & 7u;
+        return next_edge();
+    };
+    for (;;) {
+        const uint64_t idle = __ballot(!act);
+        const int n_idle = __popcll(idle);
+        if (n_idle >= kRefill) {
+            // top the queue up with the live candidates of further leaf records
+            while (qn < n_idle && base < total) {
+                const uint32_t i = base + g;
+                bool live = false;
+                uint2 cand = make_uint2(0u, 0u);
+                if (g < gpw && i < total) {
+                    const uint2 c = a.flat[i];
+                    if (sub_k <= leaf_last(c.y) && !load_flag(a.flags + c.x / NL)) {
+                        live = true;
+                        cand = make_uint2(c.x, leaf_start(c.y) + sub_k);
+                    }
+                }
+                const uint64_t m = __ballot(live);
+                if (live) q[qn + __popcll(m & below)] = cand;
+                qn += __popcll(m);
+                base += n_waves * gpw;
+            }
+            __builtin_amdgcn_wave_barrier();
+            // idle lane number r (in lane order) takes queue entry qn - 1 - r
+            const int r = __popcll(idle & below);
+            const int take = min(n_idle, qn);
+            const bool mine = !act && r < take;
+            uint2 c = make_uint2(0u, 0u);
+            if (mine) c = q[qn - 1 - r];
+            qn -= take;
+            __builtin_amdgcn_wave_barrier();
+            if (mine) act = start(c);
+            if (qn == 0 && base >= total && __ballot(act) == 0) break;
+        }
+        for (int s = 0; s < kSteps && act; ++s) {
+            if ((++iter & 15) == 0 && load_flag(flag)) {
+                act = false;
+                break;
+            }
+            const BNode nd = a.rb.tri_nodes[node0 + ni];
+            if (COUNT) ctr.nodes++;
+            bool h0 = nd.c0 >= 0 && slab(P, inv, nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, pad);
+            bool h1 = nd.c1 >= 0 && slab(P, inv, nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, pad);
+            bool hit = false;
+#pragma unroll
+            for (int side = 0; side < 2; ++side) {
+                const bool hs = side ? h1 : h0;
+                const int cnt = side ? nd.n1 : nd.n0;
+                const int first = side ? nd.c1 : nd.c0;
+                if (hs && cnt > 0 && !hit) {
+                    for (int k = 0; k < cnt; ++k) {
+                        const Tri t = a.rb.tri_prims[prim0 + first + k];
+                        if (COUNT) ctr.prims++;
+                        const float3 ta = f3(t.a);
+                        const int res = mt32(P, d, ta, sub(f3(t.b), ta), sub(f3(t.c), ta), bd);
+                        if (res == kHit) {
+                            hit = true;
+                            break;
+                        }
+                        if (res == kUnsure) {
+                            if (COUNT) ctr.fp64++;
+                            defer(a, key, tri, (uint32_t) (prim0 + first + k), e);
+                        }
+                    }
+                    if (side) h1 = false; else h0 = false;
+                }
+            }
+            if (hit) {
+                atomicOr(flag, 1u);
+                act = false;
+                break;
+            }
+            if (h0 && h1) {
+                if (sp >= STACK) {
+                    atomicOr(a.overflow, 2u);
+                    act = next_edge();
+                    continue;
+                }
+                st[sp * BS] = nd.c1;
+                ++sp;
+                ni = nd.c0;
+            } else if (h0) {
+                ni = nd.c0;
+            } else if (h1) {
+                ni = nd.c1;
+            } else if (sp > 0) {
+                --sp;
+                ni = st[sp * BS];
+            } else {
+                act = next_edge();
+            }
+        }
+    }
+}
+
 // One direction (DIR 1: obstacle->robot, 2: robot->obstacle), pass `pass` of a.narrow_passes.  Pass
 // 0 runs over the candidates (grid-stride, each leaf record spread over `lanes` adjacent threads,
 // one scene triangle per thread); later passes resume the walks the previous pass handed off.
@@ -585,38 +749,42 @@ __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
         const uint32_t lanes = (uint32_t) a.leaf_lanes;
         const uint32_t gpw = 64u / lanes, g = (uint32_t) lane / lanes, sub_k = (uint32_t) lane - g * lanes;
         const uint32_t wave = blockIdx.x * (BS / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (BS / 64);
-        // robot->obstacle runs over the records of the poses obstacle->robot left unflagged
-        const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat;
-        const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg];
-        auto eval = [&](uint2 e) {
-            uint32_t* flag = a.flags + e.x / (uint32_t) a.n_links;
-            if (load_flag(flag)) return;
-            if (COUNT) ctr.tris++;
-            if (narrow_candidate<DIR, STACK, BS, COUNT>(a, e.x, e.y, 4u, 0, st, flag, ctr, hs)) atomicOr(flag, 1u);
-        };
-        for (uint32_t base = wave * gpw; base < total; base += n_waves * gpw) {
-            const uint32_t i = base + g;
-            bool live = false;
-            uint2 cand = make_uint2(0u, 0u);
-            if (g < gpw && i < total) {
-                const uint2 c = list[i];
-                if (sub_k <= leaf_last(c.y) && !load_flag(a.flags + c.x / (uint32_t) a.n_links)) {
-                    live = true;
-                    cand = make_uint2(c.x, leaf_start(c.y) + sub_k);
+        if (DIR == 1 && !hs.may) {
+            o2r_lanes<STACK, BS, COUNT>(a, q, st, ctr);
+        } else {
+            // robot->obstacle runs over the records of the poses obstacle->robot left unflagged
+            const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat;
+            const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg];
+            auto eval = [&](uint2 e) {
+                uint32_t* flag = a.flags + e.x / (uint32_t) a.n_links;
+                if (load_flag(flag)) return;
+                if (COUNT) ctr.tris++;
+                if (narrow_candidate<DIR, STACK, BS, COUNT>(a, e.x, e.y, 4u, 0, st, flag, ctr, hs)) atomicOr(flag, 1u);
+            };
+            for (uint32_t base = wave * gpw; base < total; base += n_waves * gpw) {
+                const uint32_t i = base + g;
+                bool live = false;
+                uint2 cand = make_uint2(0u, 0u);
+                if (g < gpw && i < total) {
+                    const uint2 c = list[i];
+                    if (sub_k <= leaf_last(c.y) && !load_flag(a.flags + c.x / (uint32_t) a.n_links)) {
+                        live = true;
+                        cand = make_uint2(c.x, leaf_start(c.y) + sub_k);
+                    }
+                }
+                const uint64_t m = __ballot(live);
+                if (live) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = cand;
+                qn += __popcll(m);
+                __builtin_amdgcn_wave_barrier();
+                while (qn >= 64) {
+                    const uint2 e = q[qn - 64 + lane];
+                    qn -= 64;
+                    __builtin_amdgcn_wave_barrier();
+                    eval(e);
                 }
             }
-            const uint64_t m = __ballot(live);
-            if (live) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = cand;
-            qn += __popcll(m);
-            __builtin_amdgcn_wave_barrier();
-            while (qn >= 64) {
-                const uint2 e = q[qn - 64 + lane];
-                qn -= 64;
-                __builtin_amdgcn_wave_barrier();
-                eval(e);
-            }
+            if (lane < qn) eval(q[lane]);
         }
-        if (lane < qn) eval(q[lane]);
     } else {
         const uint4* in = ((pass - 1) & 1 ? a.nspill[1] : a.nspill[0]);
         const uint32_t total = min(a.nspill_count[pass - 1], a.nspill_cap);
