@@ -323,6 +323,19 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                 st->graph_replayed = &ws == &c->main && c->dgraph.last_replayed ? 1u : 0u;
             }
             ws.timing_pending = false;
+            // a full hand-off queue costs time, not results: grow it (within the budget) for later calls
+            auto grow = [&](auto& buf, size_t elem, size_t max_n) {
+                const size_t want = std::min(buf.n * 2, max_n);
+                if (want <= buf.n) return;
+                try {
+                    budget_check(c, ws, budget, (want - buf.n) * elem);
+                } catch (const HipError&) {
+                    return;
+                }
+                buf.reserve(want);
+            };
+            if (sv[1] & 8u) grow(ws.nspill, sizeof(uint4), (size_t) 1 << 26);
+            if (sv[1] & 16u) grow(ws.spill, sizeof(uint2), (size_t) 1 << 27);
             return;
         }
         if (sv[1] & 1u) {

@@ -8,6 +8,8 @@
 // is queried with the link's boxes, and each candidate scene triangle is moved into the link frame,
 // where its owned edges trace the static link-triangle LBVH (obstacle->robot) and the triangle
 // itself queries the static link-edge LBVH (robot->obstacle).
+#include <type_traits>
+
 #include "band32.hpp"
 #include "cand.hpp"
 #include "exact64.hpp"
@@ -211,9 +213,11 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
                     for (int j = 0; j < sp; ++j) out_q[o + 1 + j] = make_uint2(key, (uint32_t) st[j * kBS]);
                     break;
                 }
-                // queue full: mark the reserved slots empty and finish this subtree here
+                // queue full: mark the reserved slots empty and finish this subtree here (bit 16 lets
+                // the host grow the queue for later calls)
                 for (uint64_t j = o; j < a.spill_cap && j < (uint64_t) o + want; ++j)
                     out_q[j] = make_uint2(0xffffffffu, 0u);
+                atomicOr(a.overflow, 16u);
                 spill_ok = false;
             }
             const BNode nd = a.sc.tri_nodes[ni];
@@ -354,6 +358,7 @@ struct NarrowSpill {
     uint32_t* count;
     uint32_t cap, budget;
     bool may;
+    uint32_t* full;  // gets bit 8 when a hand-off did not fit (the host then grows the queue)
 };
 
 // Appends the pending subtrees (ni and the stack) of a walk; false if the queue is full (the slots
@@ -369,6 +374,7 @@ __device__ __forceinline__ bool spill_walk(const NarrowSpill& sp_, uint32_t key,
         return true;
     }
     for (uint64_t k = o; k < sp_.cap && k < (uint64_t) o + want; ++k) sp_.out[k] = make_uint4(0xffffffffu, 0u, 0u, 0u);
+    atomicOr(sp_.full, 8u);
     return false;
 }
 
@@ -381,23 +387,201 @@ __device__ __forceinline__ void defer(const DiscreteArgs& a, uint32_t key, uint3
     else atomicOr(a.overflow, 4u);
 }
 
-// Obstacle edge j of scene triangle `tri` (link-frame origin P with abs. error <= delta per
-// component, direction d = R^T (Q - P) with rel. error <= 6 eps) against the link-triangle LBVH.
-template <int STACK, int BS, bool COUNT>
-__device__ __forceinline__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, uint32_t j, float3 P,
-                        float3 d, float delta, int* st, const uint32_t* flag, Ctr& ctr, int start,
-                        const NarrowSpill& hs) {
-    if (!L.tri_has_root) return 0;
-    const float3 inv = safe_inv(d);
+enum { kVisitDone = 0, kVisitMore = 1, kVisitHit = 2 };
+
+// Persistent narrow lanes: a wave takes new candidates once this many lanes are idle, and checks
+// after this many node visits of its active lanes (measured on MI355X; a build may override them).
+#ifndef RTCD_NARROW_REFILL
+#define RTCD_NARROW_REFILL 16
+#endif
+#ifndef RTCD_NARROW_STEPS
+#define RTCD_NARROW_STEPS 4
+#endif
+
+// Walk of obstacle edge e of a scene triangle (link-frame origin P with abs. error <= delta per
+// component, direction d = R^T (Q - P) with rel. error <= 6 eps) through the link-triangle LBVH.
+struct O2RWalk {
+    float3 P, d, inv;
+    float pad;
+    MTBands bd;
+    int node0, prim0;
+    uint32_t e;
+};
+
+// Sets up the walk; false if the segment misses the link box.
+__device__ __forceinline__ bool o2r_begin(const LinkInfo& L, float3 P, float3 d, float delta, uint32_t e, O2RWalk& w) {
+    w.P = P;
+    w.d = d;
+    w.e = e;
+    w.inv = safe_inv(d);
     const float D = l1(d.x, d.y, d.z) * (1.f + 8.f * kEps32);
     const float ad = 6.1f * kEps32 * D;
-    const float pad = 2.f * delta + 2.f * ad + 1e-6f * L.scale;
-    if (!slab(P, inv, L.lo[0], L.lo[1], L.lo[2], L.hi[0], L.hi[1], L.hi[2], pad)) return 0;
+    w.pad = 2.f * delta + 2.f * ad + 1e-6f * L.scale;
+    if (!slab(P, w.inv, L.lo[0], L.lo[1], L.lo[2], L.hi[0], L.hi[1], L.hi[2], w.pad)) return false;
     const float S = l1(P.x - L.c[0], P.y - L.c[1], P.z - L.c[2]) * (1.f + 2.f * kEps32) + 3.f * delta + L.rv;
-    const float as = delta + kEps32 * S;
-    const MTBands bd = make_bands(D, ad, S, as, L.e1max, kEps32 * L.e1max, L.e2max, kEps32 * L.e2max);
-    const BNode* nodes = a.rb.tri_nodes + L.tri_node0;
-    const Tri* prims = a.rb.tri_prims + L.tri_prim0;
+    w.bd = make_bands(D, ad, S, delta + kEps32 * S, L.e1max, kEps32 * L.e1max, L.e2max, kEps32 * L.e2max);
+    w.node0 = L.tri_node0;
+    w.prim0 = L.tri_prim0;
+    return true;
+}
+
+// One node visit: both child boxes, the link triangles of the children that are leaves (banded
+// FP32 test, unsure ones queued for FP64), then the next node from the children or the LDS stack.
+template <int STACK, int BS, bool COUNT>
+__device__ __forceinline__ int o2r_visit(const DiscreteArgs& a, const O2RWalk& w, uint32_t key, uint32_t tri, int& ni,
+                                         int* st, int& sp, Ctr& ctr) {
+    const BNode nd = a.rb.tri_nodes[w.node0 + ni];
+    if (COUNT) ctr.nodes++;
+    bool h0 = nd.c0 >= 0 && slab(w.P, w.inv, nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, w.pad);
+    bool h1 = nd.c1 >= 0 && slab(w.P, w.inv, nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, w.pad);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const bool hs = side ? h1 : h0;
+        const int cnt = side ? nd.n1 : nd.n0;
+        const int first = side ? nd.c1 : nd.c0;
+        if (hs && cnt > 0) {
+            for (int k = 0; k < cnt; ++k) {
+                const Tri t = a.rb.tri_prims[w.prim0 + first + k];
+                if (COUNT) ctr.prims++;
+                const float3 ta = f3(t.a);
+                const int r = mt32(w.P, w.d, ta, sub(f3(t.b), ta), sub(f3(t.c), ta), w.bd);
+                if (r == kHit) return kVisitHit;
+                if (r == kUnsure) {
+                    if (COUNT) ctr.fp64++;
+                    defer(a, key, tri, (uint32_t) (w.prim0 + first + k), w.e);
+                }
+            }
+            if (side) h1 = false; else h0 = false;
+        }
+    }
+    if (h0 && h1) {
+        if (sp >= STACK) {
+            atomicOr(a.overflow, 2u);
+            return kVisitDone;
+        }
+        st[sp * BS] = nd.c1;
+        ++sp;
+        ni = nd.c0;
+    } else if (h0) {
+        ni = nd.c0;
+    } else if (h1) {
+        ni = nd.c1;
+    } else {
+        if (sp == 0) return kVisitDone;
+        --sp;
+        ni = st[sp * BS];
+    }
+    return kVisitMore;
+}
+
+// Walk of a scene triangle (link frame A with abs. error <= delta, e1 = R^T (B - A), e2 = R^T (C - A)
+// with rel. error <= 6 eps) through the link-edge LBVH.
+struct R2OWalk {
+    float3 A, e1, e2, nrm;
+    float delta, lx, ly, lz, hx, hy, hz, nl, ne;
+    MTBands bd;
+    int node0, prim0;
+};
+
+// Sets up the walk; false if the triangle's box misses the link box.
+__device__ __forceinline__ bool r2o_begin(const LinkInfo& L, float3 A, float3 e1, float3 e2, float delta, R2OWalk& w) {
+    const float3 B = add(A, e1), C = add(A, e2);
+    const float pad = 2.f * delta + 12.f * kEps32 * (l1(e1.x, e1.y, e1.z) + l1(e2.x, e2.y, e2.z)) + 1e-6f * L.scale;
+    w.lx = fminf(fminf(A.x, B.x), C.x) - pad;
+    w.ly = fminf(fminf(A.y, B.y), C.y) - pad;
+    w.lz = fminf(fminf(A.z, B.z), C.z) - pad;
+    w.hx = fmaxf(fmaxf(A.x, B.x), C.x) + pad;
+    w.hy = fmaxf(fmaxf(A.y, B.y), C.y) + pad;
+    w.hz = fmaxf(fmaxf(A.z, B.z), C.z) + pad;
+    if (w.lx > L.hi[0] || w.hx < L.lo[0] || w.ly > L.hi[1] || w.hy < L.lo[1] || w.lz > L.hi[2] || w.hz < L.lo[2])
+        return false;
+    const float E1 = l1(e1.x, e1.y, e1.z) * (1.f + 8.f * kEps32);
+    const float E2 = l1(e2.x, e2.y, e2.z) * (1.f + 8.f * kEps32);
+    const float S = l1(A.x - L.c[0], A.y - L.c[1], A.z - L.c[2]) * (1.f + 2.f * kEps32) + 3.f * delta + L.rv;
+    w.bd = make_bands(L.dmax, kEps32 * L.dmax, S, delta + kEps32 * S, E1, 6.1f * kEps32 * E1, E2, 6.1f * kEps32 * E2);
+    // plane of the triangle: a node box that lies strictly on one side cannot hold a crossing edge.
+    // n = e1 x e2 and d0 = n.A carry errors bounded by 14 eps E1 E2 (n) and |n| delta (A); the
+    // margin in r2o_visit covers them for any box centre c: |Delta(n.(c - A))| <= 14 eps E1 E2 |c - A|_1 + |n|_1 delta.
+    w.nrm = make_float3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
+    w.nl = l1(w.nrm.x, w.nrm.y, w.nrm.z);
+    w.ne = 32.f * kEps32 * E1 * E2;
+    w.A = A;
+    w.e1 = e1;
+    w.e2 = e2;
+    w.delta = delta;
+    w.node0 = L.edge_node0;
+    w.prim0 = L.edge_prim0;
+    return true;
+}
+
+template <int STACK, int BS, bool COUNT>
+__device__ __forceinline__ int r2o_visit(const DiscreteArgs& a, const R2OWalk& w, uint32_t key, uint32_t tri, int& ni,
+                                         int* st, int& sp, Ctr& ctr) {
+    auto straddles = [&](float bx0, float by0, float bz0, float bx1, float by1, float bz1) {
+        const float cx = 0.5f * (bx0 + bx1) - w.A.x, cy = 0.5f * (by0 + by1) - w.A.y, cz = 0.5f * (bz0 + bz1) - w.A.z;
+        const float hx = 0.5f * (bx1 - bx0), hy = 0.5f * (by1 - by0), hz = 0.5f * (bz1 - bz0);
+        const float sd = w.nrm.x * cx + w.nrm.y * cy + w.nrm.z * cz;
+        const float rr = fabsf(w.nrm.x) * hx + fabsf(w.nrm.y) * hy + fabsf(w.nrm.z) * hz;
+        const float m = w.ne * (fabsf(cx) + fabsf(cy) + fabsf(cz) + hx + hy + hz) + (w.nl + w.ne) * 2.f * w.delta +
+                        4.f * kEps32 * (fabsf(sd) + rr);
+        return fabsf(sd) <= rr + m;
+    };
+    const BNode nd = a.rb.edge_nodes[w.node0 + ni];
+    if (COUNT) ctr.nodes++;
+    bool h0 = nd.c0 >= 0 &&
+              !(w.lx > nd.hi0x || w.hx < nd.lo0x || w.ly > nd.hi0y || w.hy < nd.lo0y || w.lz > nd.hi0z || w.hz < nd.lo0z) &&
+              straddles(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
+    bool h1 = nd.c1 >= 0 &&
+              !(w.lx > nd.hi1x || w.hx < nd.lo1x || w.ly > nd.hi1y || w.hy < nd.lo1y || w.lz > nd.hi1z || w.hz < nd.lo1z) &&
+              straddles(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const bool hs = side ? h1 : h0;
+        const int cnt = side ? nd.n1 : nd.n0;
+        const int first = side ? nd.c1 : nd.c0;
+        if (hs && cnt > 0) {
+            for (int k = 0; k < cnt; ++k) {
+                const Edge e = a.rb.edge_prims[w.prim0 + first + k];
+                if (COUNT) {
+                    ctr.prims++;
+                    ctr.edges++;
+                }
+                const float3 ep = f3(e.p);
+                const int r = mt32(ep, sub(f3(e.q), ep), w.A, w.e1, w.e2, w.bd);
+                if (r == kHit) return kVisitHit;
+                if (r == kUnsure) {
+                    if (COUNT) ctr.fp64++;
+                    defer(a, key, tri, (uint32_t) (w.prim0 + first + k), 3u);
+                }
+            }
+            if (side) h1 = false; else h0 = false;
+        }
+    }
+    if (h0 && h1) {
+        if (sp >= STACK) {
+            atomicOr(a.overflow, 2u);
+            return kVisitDone;
+        }
+        st[sp * BS] = nd.c1;
+        ++sp;
+        ni = nd.c0;
+    } else if (h0) {
+        ni = nd.c0;
+    } else if (h1) {
+        ni = nd.c1;
+    } else {
+        if (sp == 0) return kVisitDone;
+        --sp;
+        ni = st[sp * BS];
+    }
+    return kVisitMore;
+}
+
+// A walk from node `start` until it hits, ends, or hands its pending subtrees off (budget `hs`):
+// 1 on a hit.  `j` is the walk's kind in the hand-off and FP64 records (obstacle edge 0..2, or 3).
+template <int DIR, int STACK, int BS, bool COUNT, typename W>
+__device__ __forceinline__ int walk(const DiscreteArgs& a, const W& w, uint32_t key, uint32_t tri, uint32_t j, int start,
+                                   int* st, const uint32_t* flag, Ctr& ctr, const NarrowSpill& hs) {
     int sp = 0, ni = start, iter = 0;
     bool spill_ok = hs.may;
     for (;;) {
@@ -406,208 +590,89 @@ __device__ __forceinline__ int o2r_edge(const DiscreteArgs& a, const LinkInfo& L
             if (spill_walk<BS>(hs, key, tri, j, ni, st, sp)) return 0;
             spill_ok = false;
         }
-        const BNode nd = nodes[ni];
-        if (COUNT) ctr.nodes++;
-        bool h0 = nd.c0 >= 0 && slab(P, inv, nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, pad);
-        bool h1 = nd.c1 >= 0 && slab(P, inv, nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, pad);
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const bool hit = side ? h1 : h0;
-            const int cnt = side ? nd.n1 : nd.n0;
-            const int start = side ? nd.c1 : nd.c0;
-            if (hit && cnt > 0) {
-                for (int k = 0; k < cnt; ++k) {
-                    const Tri t = prims[start + k];
-                    if (COUNT) ctr.prims++;
-                    const float3 ta = f3(t.a);
-                    const int r = mt32(P, d, ta, sub(f3(t.b), ta), sub(f3(t.c), ta), bd);
-                    if (r == kHit) return 1;
-                    if (r == kUnsure) {
-                        if (COUNT) ctr.fp64++;
-                        defer(a, key, tri, (uint32_t) (L.tri_prim0 + start + k), j);
-                    }
-                }
-                if (side) h1 = false; else h0 = false;
-            }
-        }
-        if (h0 && h1) {
-            if (sp >= STACK) { atomicOr(a.overflow, 2u); return 0; }
-            st[sp * BS] = nd.c1;
-            ++sp;
-            ni = nd.c0;
-        } else if (h0) {
-            ni = nd.c0;
-        } else if (h1) {
-            ni = nd.c1;
-        } else {
-            if (sp == 0) return 0;
-            --sp;
-            ni = st[sp * BS];
-        }
+        int r;
+        if constexpr (DIR == 1) r = o2r_visit<STACK, BS, COUNT>(a, w, key, tri, ni, st, sp, ctr);
+        else r = r2o_visit<STACK, BS, COUNT>(a, w, key, tri, ni, st, sp, ctr);
+        if (r == kVisitHit) return 1;
+        if (r == kVisitDone) return 0;
     }
 }
 
-// Scene triangle (link frame A with abs. error <= delta, e1 = R^T (B - A), e2 = R^T (C - A) with
-// rel. error <= 6 eps) against the link-edge LBVH.
-template <int STACK, int BS, bool COUNT>
-__device__ __forceinline__ int r2o_tri(const DiscreteArgs& a, const LinkInfo& L, uint32_t key, uint32_t tri, float3 A, float3 e1,
-                       float3 e2, float delta, int* st, const uint32_t* flag, Ctr& ctr, int start,
-                       const NarrowSpill& hs) {
-    if (!L.edge_has_root) return 0;
-    const float3 B = add(A, e1), C = add(A, e2);
-    const float pad = 2.f * delta + 12.f * kEps32 * (l1(e1.x, e1.y, e1.z) + l1(e2.x, e2.y, e2.z)) + 1e-6f * L.scale;
-    const float lx = fminf(fminf(A.x, B.x), C.x) - pad, ly = fminf(fminf(A.y, B.y), C.y) - pad,
-                lz = fminf(fminf(A.z, B.z), C.z) - pad;
-    const float hx = fmaxf(fmaxf(A.x, B.x), C.x) + pad, hy = fmaxf(fmaxf(A.y, B.y), C.y) + pad,
-                hz = fmaxf(fmaxf(A.z, B.z), C.z) + pad;
-    if (lx > L.hi[0] || hx < L.lo[0] || ly > L.hi[1] || hy < L.lo[1] || lz > L.hi[2] || hz < L.lo[2]) return 0;
-    const float E1 = l1(e1.x, e1.y, e1.z) * (1.f + 8.f * kEps32);
-    const float E2 = l1(e2.x, e2.y, e2.z) * (1.f + 8.f * kEps32);
-    const float S = l1(A.x - L.c[0], A.y - L.c[1], A.z - L.c[2]) * (1.f + 2.f * kEps32) + 3.f * delta + L.rv;
-    const MTBands bd = make_bands(L.dmax, kEps32 * L.dmax, S, delta + kEps32 * S, E1, 6.1f * kEps32 * E1, E2,
-                                  6.1f * kEps32 * E2);
-    const BNode* nodes = a.rb.edge_nodes + L.edge_node0;
-    const Edge* prims = a.rb.edge_prims + L.edge_prim0;
-    // plane of the triangle: a node box that lies strictly on one side cannot hold a crossing edge.
-    // n = e1 x e2 and d0 = n.A carry errors bounded by 14 eps E1 E2 (n) and |n| delta (A); the
-    // margin below covers them for any box centre c: |Delta(n.(c - A))| <= 14 eps E1 E2 |c - A|_1 + |n|_1 delta.
-    const float3 nrm = make_float3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
-    const float nl = l1(nrm.x, nrm.y, nrm.z);
-    const float ne = 32.f * kEps32 * E1 * E2;
-    auto straddles = [&](float bx0, float by0, float bz0, float bx1, float by1, float bz1) {
-        const float cx = 0.5f * (bx0 + bx1) - A.x, cy = 0.5f * (by0 + by1) - A.y, cz = 0.5f * (bz0 + bz1) - A.z;
-        const float hx = 0.5f * (bx1 - bx0), hy = 0.5f * (by1 - by0), hz = 0.5f * (bz1 - bz0);
-        const float sd = nrm.x * cx + nrm.y * cy + nrm.z * cz;
-        const float rr = fabsf(nrm.x) * hx + fabsf(nrm.y) * hy + fabsf(nrm.z) * hz;
-        const float m = ne * (fabsf(cx) + fabsf(cy) + fabsf(cz) + hx + hy + hz) + (nl + ne) * 2.f * delta +
-                        4.f * kEps32 * (fabsf(sd) + rr);
-        return fabsf(sd) <= rr + m;
-    };
-    int sp = 0, ni = start, iter = 0;
-    bool spill_ok = hs.may;
-    for (;;) {
-        if ((++iter & 15) == 0 && load_flag(flag)) return 0;
-        if (spill_ok && (uint32_t) iter > hs.budget) {
-            if (spill_walk<BS>(hs, key, tri, 3u, ni, st, sp)) return 0;
-            spill_ok = false;
-        }
-        const BNode nd = nodes[ni];
-        if (COUNT) ctr.nodes++;
-        bool h0 = nd.c0 >= 0 && !(lx > nd.hi0x || hx < nd.lo0x || ly > nd.hi0y || hy < nd.lo0y || lz > nd.hi0z ||
-                                  hz < nd.lo0z) &&
-                  straddles(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
-        bool h1 = nd.c1 >= 0 && !(lx > nd.hi1x || hx < nd.lo1x || ly > nd.hi1y || hy < nd.lo1y || lz > nd.hi1z ||
-                                  hz < nd.lo1z) &&
-                  straddles(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
-#pragma unroll
-        for (int side = 0; side < 2; ++side) {
-            const bool hit = side ? h1 : h0;
-            const int cnt = side ? nd.n1 : nd.n0;
-            const int start = side ? nd.c1 : nd.c0;
-            if (hit && cnt > 0) {
-                for (int k = 0; k < cnt; ++k) {
-                    const Edge e = prims[start + k];
-                    if (COUNT) { ctr.prims++; ctr.edges++; }
-                    const float3 ep = f3(e.p);
-                    const int r = mt32(ep, sub(f3(e.q), ep), A, e1, e2, bd);
-                    if (r == kHit) return 1;
-                    if (r == kUnsure) {
-                        if (COUNT) ctr.fp64++;
-                        defer(a, key, tri, (uint32_t) (L.edge_prim0 + start + k), 3u);
-                    }
-                }
-                if (side) h1 = false; else h0 = false;
-            }
-        }
-        if (h0 && h1) {
-            if (sp >= STACK) { atomicOr(a.overflow, 2u); return 0; }
-            st[sp * BS] = nd.c1;
-            ++sp;
-            ni = nd.c0;
-        } else if (h0) {
-            ni = nd.c0;
-        } else if (h1) {
-            ni = nd.c1;
-        } else {
-            if (sp == 0) return 0;
-            --sp;
-            ni = st[sp * BS];
-        }
-    }
-}
-
-// One direction over all candidates (grid-stride).  DIR 1: obstacle->robot, 2: robot->obstacle.
-// Candidate (key, scene triangle) in direction DIR: edge j of the triangle (obstacle->robot) or the
-// triangle (robot->obstacle) against the link BVH from node `start`.  All edges when j == 4.
+// Candidate (key, scene triangle) resumed from a hand-off: edge j of the triangle (obstacle->robot)
+// or the triangle (robot->obstacle) against the link BVH from node `start`.
 template <int DIR, int STACK, int BS, bool COUNT>
-__device__ __forceinline__ int narrow_candidate(const DiscreteArgs& a, uint32_t key, uint32_t tri, uint32_t j,
+__device__ __forceinline__ int resume_candidate(const DiscreteArgs& a, uint32_t key, uint32_t tri, uint32_t j,
                                                 int start, int* st, uint32_t* flag, Ctr& ctr,
                                                 const NarrowSpill& hs) {
-    const uint32_t l = key % (uint32_t) a.n_links;
-    const LinkInfo& L = a.robot->link[l];
+    const LinkInfo& L = a.robot->link[key % (uint32_t) a.n_links];
     const float4 r0 = a.Ti[(size_t) key * 3], r1 = a.Ti[(size_t) key * 3 + 1], r2 = a.Ti[(size_t) key * 3 + 2];
     const Tri t = a.sc.tri_prims[tri];
     const LinkTri m = link_tri(t, r0, r1, r2);
-    if (DIR == 2) return r2o_tri<STACK, BS, COUNT>(a, L, key, tri, m.A, m.e1, m.e2, m.delta, st, flag, ctr, start, hs);
-    const uint32_t owned = __float_as_uint(t.b.w) & (j == 4 ? 7u : 1u << j);
-    // the owned edges A->B, B->C, C->A, one traversal site (the loop is kept rolled)
-    int hit = 0;
-#pragma clang loop unroll(disable)
-    for (uint32_t e = 0; e < 3 && !hit; ++e) {
-        if (!(owned & (1u << e))) continue;
-        const float3 P = e == 0 ? m.A : e == 1 ? add(m.A, m.e1) : add(m.A, m.e2);
-        const float3 d = e == 0 ? m.e1 : e == 1 ? rot(r0, r1, r2, sub(f3(t.c), f3(t.b))) : neg(m.e2);
-        hit = o2r_edge<STACK, BS, COUNT>(a, L, key, tri, e, P, d, m.delta, st, flag, ctr, start, hs);
+    if constexpr (DIR == 2) {
+        R2OWalk w;
+        if (!L.edge_has_root || !r2o_begin(L, m.A, m.e1, m.e2, m.delta, w)) return 0;
+        return walk<DIR, STACK, BS, COUNT>(a, w, key, tri, 3u, start, st, flag, ctr, hs);
+    } else {
+        // the edges A->B, B->C, C->A
+        const float3 P = j == 0 ? m.A : j == 1 ? add(m.A, m.e1) : add(m.A, m.e2);
+        const float3 d = j == 0 ? m.e1 : j == 1 ? rot(r0, r1, r2, sub(f3(t.c), f3(t.b))) : neg(m.e2);
+        O2RWalk w;
+        if (!L.tri_has_root || !o2r_begin(L, P, d, m.delta, j, w)) return 0;
+        return walk<DIR, STACK, BS, COUNT>(a, w, key, tri, j, start, st, flag, ctr, hs);
     }
-    return hit;
 }
 
-// Obstacle->robot pass without hand-off, with persistent lanes: every lane holds its own candidate
-// (scene triangle, pose-link) and, once that candidate is decided, takes the next one from the
-// wave's LDS queue, which the wave tops up from its share of the leaf records.  A wave that ran its
-// 64 candidates to completion together would keep every lane waiting for the longest walk: the
-// walks are short on average (a few link-BVH nodes per edge) but vary widely.  The walks, the tests
-// and their FP64 hand-over are those of o2r_edge.
-template <int STACK, int BS, bool COUNT>
-__device__ __forceinline__ void o2r_lanes(const DiscreteArgs& a, uint2* q, int* st, Ctr& ctr) {
-    constexpr int kRefill = 16;  // idle lanes that make the wave refill
-    constexpr int kSteps = 8;    // node visits between refill checks
+// Pass 0 of one direction with persistent lanes: every lane holds its own candidate (scene
+// triangle, pose-link) and, once that candidate is decided or handed off, takes the next one from
+// the wave's LDS queue, which the wave tops up with the live candidates of its share of the leaf
+// records (a leaf record spreads over `lanes` adjacent lanes, one scene triangle each; candidates of
+// poses already flagged never enter the queue).  A wave that ran 64 candidates to completion
+// together would keep every lane waiting for the longest walk: walks are short on average (a few
+// link-BVH nodes) but vary widely.  Obstacle->robot traces the owned edges of the triangle one after
+// the other; robot->obstacle walks the triangle once.
+template <int DIR, int STACK, int BS, bool COUNT, bool SPILL>
+__device__ __forceinline__ void narrow_lanes(const DiscreteArgs& a, const uint2* list, uint32_t total, uint2* q,
+                                             int* st, Ctr& ctr, const NarrowSpill& hs) {
+    constexpr int kRefill = RTCD_NARROW_REFILL;  // idle lanes that make the wave refill
+    constexpr int kSteps = RTCD_NARROW_STEPS;    // node visits between refill checks
     const int lane = (int) (threadIdx.x & 63);
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t lanes = (uint32_t) a.leaf_lanes;
     const uint32_t gpw = 64u / lanes, g = (uint32_t) lane / lanes, sub_k = (uint32_t) lane - g * lanes;
     const uint32_t wave = blockIdx.x * (BS / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (BS / 64);
-    const uint32_t total = a.seg_pre[kSeg];
     const uint32_t NL = (uint32_t) a.n_links;
     uint32_t base = wave * gpw;  // next leaf-record group of this wave
     int qn = 0;                  // queued candidates (wave-uniform)
-    // lane state: candidate, its link-frame triangle, the current edge and its walk
-    bool act = false;
-    uint32_t key = 0, tri = 0, own = 0, e = 0;
+    // lane state: the candidate and its current walk
+    bool act = false, spill_ok = false;
+    uint32_t key = 0, tri = 0;
     uint32_t* flag = a.flags;
-    int node0 = 0, prim0 = 0, sp = 0, ni = 0, iter = 0;
-    float3 A = {}, e1 = {}, e2 = {}, d1 = {}, P = {}, d = {}, inv = {};
-    float delta = 0.f, pad = 0.f;
-    MTBands bd = {};
-    // next owned edge whose segment meets the link box: its walk starts at the root; false if none
-    auto next_edge = [&]() -> bool {
-        const LinkInfo& L = a.robot->link[key % NL];
-        while (own) {
-            e = (uint32_t) __builtin_ctz(own);
-            own &= own - 1u;
-            P = e == 0 ? A : e == 1 ? add(A, e1) : add(A, e2);
-            d = e == 0 ? e1 : e == 1 ? d1 : neg(e2);
-            inv = safe_inv(d);
-            const float D = l1(d.x, d.y, d.z) * (1.f + 8.f * kEps32);
-            const float ad = 6.1f * kEps32 * D;
-            pad = 2.f * delta + 2.f * ad + 1e-6f * L.scale;
-            if (!slab(P, inv, L.lo[0], L.lo[1], L.lo[2], L.hi[0], L.hi[1], L.hi[2], pad)) continue;
-            const float S = l1(P.x - L.c[0], P.y - L.c[1], P.z - L.c[2]) * (1.f + 2.f * kEps32) + 3.f * delta + L.rv;
-            bd = make_bands(D, ad, S, delta + kEps32 * S, L.e1max, kEps32 * L.e1max, L.e2max, kEps32 * L.e2max);
-            sp = 0;
-            ni = 0;
-            iter = 0;
-            return true;
+    int sp = 0, ni = 0, iter = 0;
+    using W = std::conditional_t<DIR == 1, O2RWalk, R2OWalk>;
+    W w = {};
+    // obstacle->robot: the triangle in the link frame, its edge B->C and the edges not yet traced
+    float3 A = {}, e1 = {}, e2 = {}, d1 = {};
+    float delta = 0.f;
+    uint32_t own = 0;
+    auto begin_walk = [&]() {
+        sp = 0;
+        ni = 0;
+        iter = 0;
+        spill_ok = hs.may;
+    };
+    // obstacle->robot: the next owned edge whose segment meets the link box; false if none is left
+    auto next_edge = [&](const LinkInfo& L) -> bool {
+        if constexpr (DIR == 1) {
+            while (own) {
+                const uint32_t e = (uint32_t) __builtin_ctz(own);
+                own &= own - 1u;
+                const float3 P = e == 0 ? A : e == 1 ? add(A, e1) : add(A, e2);
+                const float3 d = e == 0 ? e1 : e == 1 ? d1 : neg(e2);
+                if (o2r_begin(L, P, d, delta, e, w)) {
+                    begin_walk();
+                    return true;
+                }
+            }
         }
         return false;
     };
@@ -618,19 +683,28 @@ __device__ __forceinline__ void o2r_lanes(const DiscreteArgs& a, uint2* q, int* 
         if (load_flag(flag)) return false;
         if (COUNT) ctr.tris++;
         const LinkInfo& L = a.robot->link[key % NL];
-        if (!L.tri_has_root) return false;
-        node0 = L.tri_node0;
-        prim0 = L.tri_prim0;
+        if (DIR == 1 ? !L.tri_has_root : !L.edge_has_root) return false;
         const float4 r0 = a.Ti[(size_t) key * 3], r1 = a.Ti[(size_t) key * 3 + 1], r2 = a.Ti[(size_t) key * 3 + 2];
         const Tri t = a.sc.tri_prims[tri];
         const LinkTri m = link_tri(t, r0, r1, r2);
-        A = m.A;
-        e1 = m.e1;
-        e2 = m.e2;
-        delta = m.delta;
-        d1 = rot(r0, r1, r2, sub(f3(t.c), f3(t.b)));
-        own = __float_as_uint(t.b.w) & 7u;
-        return next_edge();
+        if constexpr (DIR == 1) {
+            A = m.A;
+            e1 = m.e1;
+            e2 = m.e2;
+            delta = m.delta;
+            d1 = rot(r0, r1, r2, sub(f3(t.c), f3(t.b)));
+            own = __float_as_uint(t.b.w) & 7u;
+            return next_edge(L);
+        } else {
+            if (!r2o_begin(L, m.A, m.e1, m.e2, m.delta, w)) return false;
+            begin_walk();
+            return true;
+        }
+    };
+    // the lane's walk has ended without a hit: obstacle->robot goes on with the next owned edge
+    auto walk_ended = [&]() -> bool {
+        if constexpr (DIR == 1) return next_edge(a.robot->link[key % NL]);
+        return false;
     };
     for (;;) {
         const uint64_t idle = __ballot(!act);
@@ -642,7 +716,7 @@ __device__ __forceinline__ void o2r_lanes(const DiscreteArgs& a, uint2* q, int* 
                 bool live = false;
                 uint2 cand = make_uint2(0u, 0u);
                 if (g < gpw && i < total) {
-                    const uint2 c = a.flat[i];
+                    const uint2 c = list[i];
                     if (sub_k <= leaf_last(c.y) && !load_flag(a.flags + c.x / NL)) {
                         live = true;
                         cand = make_uint2(c.x, leaf_start(c.y) + sub_k);
@@ -670,121 +744,47 @@ __device__ __forceinline__ void o2r_lanes(const DiscreteArgs& a, uint2* q, int* 
                 act = false;
                 break;
             }
-            const BNode nd = a.rb.tri_nodes[node0 + ni];
-            if (COUNT) ctr.nodes++;
-            bool h0 = nd.c0 >= 0 && slab(P, inv, nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, pad);
-            bool h1 = nd.c1 >= 0 && slab(P, inv, nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, pad);
-            bool hit = false;
-#pragma unroll
-            for (int side = 0; side < 2; ++side) {
-                const bool hs = side ? h1 : h0;
-                const int cnt = side ? nd.n1 : nd.n0;
-                const int first = side ? nd.c1 : nd.c0;
-                if (hs && cnt > 0 && !hit) {
-                    for (int k = 0; k < cnt; ++k) {
-                        const Tri t = a.rb.tri_prims[prim0 + first + k];
-                        if (COUNT) ctr.prims++;
-                        const float3 ta = f3(t.a);
-                        const int res = mt32(P, d, ta, sub(f3(t.b), ta), sub(f3(t.c), ta), bd);
-                        if (res == kHit) {
-                            hit = true;
-                            break;
-                        }
-                        if (res == kUnsure) {
-                            if (COUNT) ctr.fp64++;
-                            defer(a, key, tri, (uint32_t) (prim0 + first + k), e);
-                        }
-                    }
-                    if (side) h1 = false; else h0 = false;
-                }
-            }
-            if (hit) {
-                atomicOr(flag, 1u);
-                act = false;
-                break;
-            }
-            if (h0 && h1) {
-                if (sp >= STACK) {
-                    atomicOr(a.overflow, 2u);
-                    act = next_edge();
+            if (SPILL && spill_ok && (uint32_t) iter > hs.budget) {
+                uint32_t j = 3u;
+                if constexpr (DIR == 1) j = w.e;
+                if (spill_walk<BS>(hs, key, tri, j, ni, st, sp)) {
+                    act = walk_ended();
                     continue;
                 }
-                st[sp * BS] = nd.c1;
-                ++sp;
-                ni = nd.c0;
-            } else if (h0) {
-                ni = nd.c0;
-            } else if (h1) {
-                ni = nd.c1;
-            } else if (sp > 0) {
-                --sp;
-                ni = st[sp * BS];
-            } else {
-                act = next_edge();
+                spill_ok = false;
+            }
+            int v;
+            if constexpr (DIR == 1) v = o2r_visit<STACK, BS, COUNT>(a, w, key, tri, ni, st, sp, ctr);
+            else v = r2o_visit<STACK, BS, COUNT>(a, w, key, tri, ni, st, sp, ctr);
+            if (v == kVisitHit) {
+                atomicOr(flag, 1u);
+                act = false;
+            } else if (v == kVisitDone) {
+                act = walk_ended();
             }
         }
     }
 }
 
 // One direction (DIR 1: obstacle->robot, 2: robot->obstacle), pass `pass` of a.narrow_passes.  Pass
-// 0 runs over the candidates (grid-stride, each leaf record spread over `lanes` adjacent threads,
-// one scene triangle per thread); later passes resume the walks the previous pass handed off.
+// 0 runs over the candidates with persistent lanes (robot->obstacle over the records of the poses
+// obstacle->robot left unflagged, when that ran first); later passes resume the walks the previous
+// pass handed off.
 template <int DIR, int STACK, int BS, bool COUNT>
 __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
     __shared__ int stk[STACK * BS];
     int* st = stk + threadIdx.x;
     const NarrowSpill hs{(pass & 1 ? a.nspill[1] : a.nspill[0]), a.nspill_count + pass, a.nspill_cap,
-                         pass == 0 ? a.narrow_budget : a.narrow_budget_next, pass < a.narrow_passes - 1};
+                         pass == 0 ? a.narrow_budget : a.narrow_budget_next, pass < a.narrow_passes - 1, a.overflow};
     const uint32_t stride = gridDim.x * BS;
     Ctr ctr;
     if (pass == 0) {
-        // Each wave reads leaf records (`lanes` adjacent lanes per record, one scene triangle each),
-        // keeps the candidates whose pose is not flagged yet in an LDS queue, and evaluates them 64 at
-        // a time with every lane active: most candidates belong to poses flagged early, and a wave
-        // that evaluated them in place would run the whole test for a few live lanes.
         __shared__ uint2 lq[BS / 64][128];
-        const int lane = (int) (threadIdx.x & 63);
-        uint2* q = lq[threadIdx.x >> 6];
-        int qn = 0;  // wave-uniform
-        const uint32_t lanes = (uint32_t) a.leaf_lanes;
-        const uint32_t gpw = 64u / lanes, g = (uint32_t) lane / lanes, sub_k = (uint32_t) lane - g * lanes;
-        const uint32_t wave = blockIdx.x * (BS / 64) + (threadIdx.x >> 6), n_waves = gridDim.x * (BS / 64);
-        if (DIR == 1 && !hs.may) {
-            o2r_lanes<STACK, BS, COUNT>(a, q, st, ctr);
-        } else {
-            // robot->obstacle runs over the records of the poses obstacle->robot left unflagged
-            const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat;
-            const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg];
-            auto eval = [&](uint2 e) {
-                uint32_t* flag = a.flags + e.x / (uint32_t) a.n_links;
-                if (load_flag(flag)) return;
-                if (COUNT) ctr.tris++;
-                if (narrow_candidate<DIR, STACK, BS, COUNT>(a, e.x, e.y, 4u, 0, st, flag, ctr, hs)) atomicOr(flag, 1u);
-            };
-            for (uint32_t base = wave * gpw; base < total; base += n_waves * gpw) {
-                const uint32_t i = base + g;
-                bool live = false;
-                uint2 cand = make_uint2(0u, 0u);
-                if (g < gpw && i < total) {
-                    const uint2 c = list[i];
-                    if (sub_k <= leaf_last(c.y) && !load_flag(a.flags + c.x / (uint32_t) a.n_links)) {
-                        live = true;
-                        cand = make_uint2(c.x, leaf_start(c.y) + sub_k);
-                    }
-                }
-                const uint64_t m = __ballot(live);
-                if (live) q[qn + __popcll(m & ((1ull << lane) - 1ull))] = cand;
-                qn += __popcll(m);
-                __builtin_amdgcn_wave_barrier();
-                while (qn >= 64) {
-                    const uint2 e = q[qn - 64 + lane];
-                    qn -= 64;
-                    __builtin_amdgcn_wave_barrier();
-                    eval(e);
-                }
-            }
-            if (lane < qn) eval(q[lane]);
-        }
+        const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat;
+        const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg];
+        // a pass that may hand off and one that never does (the walks of the latter keep fewer registers)
+        if (hs.may) narrow_lanes<DIR, STACK, BS, COUNT, true>(a, list, total, lq[threadIdx.x >> 6], st, ctr, hs);
+        else narrow_lanes<DIR, STACK, BS, COUNT, false>(a, list, total, lq[threadIdx.x >> 6], st, ctr, hs);
     } else {
         const uint4* in = ((pass - 1) & 1 ? a.nspill[1] : a.nspill[0]);
         const uint32_t total = min(a.nspill_count[pass - 1], a.nspill_cap);
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
             if (e.x == 0xffffffffu) continue;  // slot of a hand-off that did not fit
             uint32_t* flag = a.flags + e.x / (uint32_t) a.n_links;
             if (load_flag(flag)) continue;
-            if (narrow_candidate<DIR, STACK, BS, COUNT>(a, e.x, e.y, e.w, (int) e.z, st, flag, ctr, hs))
+            if (resume_candidate<DIR, STACK, BS, COUNT>(a, e.x, e.y, e.w, (int) e.z, st, flag, ctr, hs))
                 atomicOr(flag, 1u);
         }
     }
