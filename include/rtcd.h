@@ -139,6 +139,14 @@ rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const 
                                         uint32_t* d_out_seg, void* stream);
 /* W = pinv(B) for the uniform quadratic B-spline, [n_ctrl][n_wp] row-major. */
 rtcd_status rtcd_quadratic_weights(uint32_t n_ctrl, uint32_t n_wp, double* W);
+/* W = pinv(B), [n_ctrl][n_wp] row-major, for a degree-2 or degree-3 B-spline sampled at n_wp
+ * parameters uniform on its domain.  Replaces the weight files RTCD/data/weights/{quad,cubic}/
+ * {normal,repeated}/{nC}_{nT}.txt that batchOptixCurveRobot::loadCurveWeight reads
+ * (RTCD/Robot/batchCurveRobot.h:560-605; end-point types RTCD/Robot/batchRobotConfig.h:8-26):
+ * NORMAL = uniform knots, REPEATED = open-uniform (clamped) knots.  Quadratic weights of either kind
+ * can be passed to rtcd_check_quadratic as W. */
+enum { RTCD_END_NORMAL = 0, RTCD_END_REPEATED = 1 };
+rtcd_status rtcd_curve_weights(uint32_t degree, uint32_t end_points, uint32_t n_ctrl, uint32_t n_wp, double* W);
 
 /* ---- Serving: continuous batching over resident scenes (no RTCD counterpart; RTCD's closest
  * analogue is the NSTREAM ThreadPool of CollisionDetector::check, CollisionDetector.h:396-433). ---- */

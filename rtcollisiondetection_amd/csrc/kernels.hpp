@@ -132,5 +132,6 @@ void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStre
 // Quadratic pieces are split into this many sub-pieces (work items) each.
 constexpr int kSweepSub = 4;
 void quadratic_weights(int nc, int nw, double* W);
+void curve_weights(int degree, bool clamped, int nc, int nw, double* W);
 
 }  // namespace rtcd

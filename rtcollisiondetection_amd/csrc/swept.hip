@@ -468,17 +468,41 @@ void sweep_t(const SweepArgs& a, bool count, hipStream_t s) {
 }  // namespace
 
 // W = pinv(B) of the uniform quadratic B-spline sampled at u = linspace(0, nc - 2, nw).
-void quadratic_weights(int nc, int nw, double* W) {
-    const int nseg = nc - 2;
+// Basis matrix B [nw][nc] of a degree-p B-spline with nc control points at nw parameters uniform on
+// its domain [0, nc - p].  `clamped` false: uniform knots t_j = j - p (the reference's "normal" end
+// points); true: open-uniform knots 0 (p+1 times), 1, .., m - 1, m (p+1 times), m = nc - p (its
+// "repeated" end points, where the curve starts and ends on the first and last control point).
+// Cox-de Boor recursion on the knot span of each parameter (Piegl & Tiller, The NURBS Book, A2.2).
+static std::vector<double> bspline_basis(int p, bool clamped, int nc, int nw) {
+    const int m = nc - p;
+    std::vector<double> t((size_t) nc + p + 1);
+    for (int j = 0; j <= nc + p; ++j)
+        t[(size_t) j] = clamped ? (double) std::min(std::max(j - p, 0), m) : (double) (j - p);
     std::vector<double> B((size_t) nw * nc, 0.0);
+    double N[4], left[4], right[4];
     for (int i = 0; i < nw; ++i) {
-        double u = nw > 1 ? (double) nseg * i / (nw - 1) : 0.0;
-        int s = std::min((int) std::floor(u), nseg - 1);
-        double t = u - s;
-        B[(size_t) i * nc + s] = (1 - t) * (1 - t) / 2;
-        B[(size_t) i * nc + s + 1] = (-2 * t * t + 2 * t + 1) / 2;
-        B[(size_t) i * nc + s + 2] = t * t / 2;
+        const double u = nw > 1 ? (double) m * i / (nw - 1) : 0.0;
+        int k = p;  // span: t[k] <= u < t[k + 1], the last span for u = m
+        while (k < nc - 1 && t[(size_t) k + 1] <= u) ++k;
+        N[0] = 1.0;
+        for (int j = 1; j <= p; ++j) {
+            left[j] = u - t[(size_t) (k + 1 - j)];
+            right[j] = t[(size_t) (k + j)] - u;
+            double saved = 0.0;
+            for (int r = 0; r < j; ++r) {
+                const double tmp = N[r] / (right[r + 1] + left[j - r]);
+                N[r] = saved + right[r + 1] * tmp;
+                saved = left[j - r] * tmp;
+            }
+            N[j] = saved;
+        }
+        for (int j = 0; j <= p; ++j) B[(size_t) i * nc + (k - p + j)] = N[j];
     }
+    return B;
+}
+
+void curve_weights(int degree, bool clamped, int nc, int nw, double* W) {
+    const std::vector<double> B = bspline_basis(degree, clamped, nc, nw);
     // W = (B^T B)^{-1} B^T by Gauss-Jordan with partial pivoting on [B^T B | B^T]
     const int cols = nc + nw;
     std::vector<double> M((size_t) nc * cols, 0.0);
@@ -509,6 +533,8 @@ void quadratic_weights(int nc, int nw, double* W) {
     for (int r = 0; r < nc; ++r)
         for (int k = 0; k < nw; ++k) W[(size_t) r * nw + k] = M[(size_t) r * cols + nc + k];
 }
+
+void quadratic_weights(int nc, int nw, double* W) { curve_weights(2, false, nc, nw, W); }
 
 void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s) {
     if (!a.sc.edge_has_root || a.n_items == 0) return;
@@ -745,6 +771,17 @@ void on_main(rtcd_ctx* ctx, void* stream, F&& f) {
 }  // namespace
 
 extern "C" {
+
+rtcd_status rtcd_curve_weights(uint32_t degree, uint32_t end_points, uint32_t n_ctrl, uint32_t n_wp, double* W) {
+    if (!W || (degree != 2 && degree != 3) || end_points > RTCD_END_REPEATED || n_ctrl < degree + 1 || n_wp < n_ctrl ||
+        n_ctrl > 64 || n_wp > 4096)
+        return fail(RTCD_ERR_INVALID_ARG,
+                    "need degree 2 or 3, end points NORMAL or REPEATED, degree < n_ctrl <= 64, n_ctrl <= n_wp <= 4096, W non-null");
+    return guarded([&]() -> rtcd_status {
+        curve_weights((int) degree, end_points == RTCD_END_REPEATED, (int) n_ctrl, (int) n_wp, W);
+        return RTCD_OK;
+    });
+}
 
 rtcd_status rtcd_quadratic_weights(uint32_t n_ctrl, uint32_t n_wp, double* W) {
     if (!W || n_ctrl < 3 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)

@@ -167,6 +167,18 @@ def quadratic_weights(n_ctrl, n_wp):
     return W
 
 
+END_POINTS = {'normal': 0, 'repeated': 1}
+
+
+def curve_weights(degree, end_points, n_ctrl, n_wp):
+    """Least-squares fit matrix W = pinv(B) [n_ctrl, n_wp] of a quadratic (2) or cubic (3) B-spline;
+    end_points 'normal' (uniform knots) or 'repeated' (clamped knots), the two families of the
+    reference's RTCD/data/weights/{quad,cubic}/{normal,repeated} files."""
+    W = np.zeros((n_ctrl, n_wp), dtype=np.float64)
+    check(lib().rtcd_curve_weights(degree, END_POINTS[end_points], n_ctrl, n_wp, ptr(W)))
+    return W
+
+
 class CollisionDetector:
     """RTCD-shaped facade: CollisionDetector(ctx, scene, TYPE)."""
 

@@ -34,6 +34,65 @@ def test_quadratic_weights_match_reference_files(nc, nt):
     np.testing.assert_allclose(W, ref, atol=2e-6)
 
 
+@pytest.mark.parametrize('kind,end,nc,nt', [('quad', 'repeated', 4, 8), ('quad', 'repeated', 8, 16),
+                                            ('cubic', 'normal', 4, 16), ('cubic', 'normal', 16, 64),
+                                            ('cubic', 'repeated', 4, 16), ('cubic', 'repeated', 8, 32),
+                                            ('quad', 'normal', 8, 16)])
+def test_curve_weights_match_reference_files(kind, end, nc, nt):
+    from rtcollisiondetection_amd.detector import curve_weights
+    W = curve_weights(2 if kind == 'quad' else 3, end, nc, nt)
+    ref = np.loadtxt(os.path.join(GOLD, f'weights_{kind}_{end}_{nc}_{nt}.txt')).reshape(nc, nt)
+    np.testing.assert_allclose(W, ref, atol=2e-6)
+
+
+@pytest.mark.parametrize('degree', [2, 3])
+@pytest.mark.parametrize('end', ['normal', 'repeated'])
+def test_curve_weights_invariants(degree, end):
+    """W is a left inverse of the basis (W B = I: a fit reproduces any curve of the family exactly), its
+    rows sum to 1 (the basis is a partition of unity), and clamped curves start and end on their end
+    control points; the basis here is evaluated independently of the library (closed forms per span)."""
+    from rtcollisiondetection_amd.detector import curve_weights
+    nc, nt = 7, 29
+    W = curve_weights(degree, end, nc, nt)
+    m = nc - degree
+    us = np.linspace(0.0, m, nt)
+    B = np.zeros((nt, nc))
+    for r, u in enumerate(us):
+        if end == 'normal':
+            s = min(int(np.floor(u)), m - 1)
+            t = u - s
+            if degree == 2:
+                w = [(1 - t) ** 2 / 2, (-2 * t * t + 2 * t + 1) / 2, t * t / 2]
+            else:
+                w = [(1 - t) ** 3 / 6, (3 * t ** 3 - 6 * t * t + 4) / 6, (-3 * t ** 3 + 3 * t * t + 3 * t + 1) / 6, t ** 3 / 6]
+            B[r, s:s + degree + 1] = w
+        else:
+            # clamped: de Casteljau-free check through the Cox-de Boor definition on explicit knots
+            knots = np.concatenate([np.zeros(degree), np.arange(m + 1), np.full(degree, m)]).astype(float)
+            def N(i, p, x):
+                if p == 0:
+                    last = knots[i + 1] == m and x == m and knots[i] < m
+                    return 1.0 if (knots[i] <= x < knots[i + 1]) or last else 0.0
+                a = 0.0 if knots[i + p] == knots[i] else (x - knots[i]) / (knots[i + p] - knots[i]) * N(i, p - 1, x)
+                b = 0.0 if knots[i + p + 1] == knots[i + 1] else (knots[i + p + 1] - x) / (knots[i + p + 1] - knots[i + 1]) * N(i + 1, p - 1, x)
+                return a + b
+            B[r] = [N(i, degree, u) for i in range(nc)]
+    np.testing.assert_allclose(B.sum(axis=1), 1.0, atol=1e-12)
+    np.testing.assert_allclose(W @ B, np.eye(nc), atol=1e-9)
+    np.testing.assert_allclose(W.sum(axis=1), 1.0, atol=1e-9)
+    if end == 'repeated':
+        assert B[0, 0] == 1.0 and B[-1, -1] == 1.0
+
+
+def test_curve_weights_invalid():
+    from rtcollisiondetection_amd import _lib
+    from rtcollisiondetection_amd.detector import curve_weights
+    for args in ((4, 'normal', 8, 16), (2, 'normal', 8, 4), (3, 'repeated', 3, 16)):
+        with pytest.raises(_lib.RtcdError) as e:
+            curve_weights(*args)
+        assert e.value.status == 1
+
+
 def test_quadratic_weights_invalid():
     from rtcollisiondetection_amd import _lib
     from rtcollisiondetection_amd.detector import quadratic_weights

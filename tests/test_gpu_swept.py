@@ -56,6 +56,20 @@ def test_quadratic_matches_oracle(ctx, panda, orc_panda, shelf):
     assert np.array_equal(gseg, rseg)
 
 
+def test_quadratic_repeated_end_point_weights(ctx, panda, orc_panda, shelf):
+    """Control points fitted with the clamped ("repeated" end points) weights: same parity bar."""
+    from rtcollisiondetection_amd.detector import curve_weights
+    sc, sid, m = shelf
+    wp, _ = samplers.bspline_trajectories(96, panda.lower, panda.upper, n_ctrl=6, n_wp=24, seed=12)
+    W = curve_weights(2, 'repeated', 6, 24)
+    got, gseg = ctx.check_quadratic(sid, wp, 6, W, with_segments=True)
+    ref, rseg = O.check_quadratic(orc_panda, m, wp, W, with_segments=True)
+    print(f'quadratic (repeated): {int(ref.sum())}/96 trajectories collide')
+    assert 0 < ref.sum() < len(ref)
+    assert np.array_equal(got, ref)
+    assert np.array_equal(gseg, rseg)
+
+
 def test_quadratic_default_weights(ctx, panda, shelf):
     sc, sid, m = shelf
     wp, _ = samplers.bspline_trajectories(32, panda.lower, panda.upper, seed=6)
