@@ -137,6 +137,40 @@ rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* 
 rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_waypoints, uint32_t n_traj,
                                         uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
                                         uint32_t* d_out_seg, void* stream);
+/* ---- Geometry ingestion (host only): the files RTCD reads for robots and obstacles. ---- */
+/* OBJ mesh (RTCD/Meshes/mesh.h:311-365, meshModel::loadOBJ): polygons fan-triangulated, vertices
+ * deduplicated by exact value in first-use order (mesh.h:218-231).  With verts == tris == NULL only
+ * the counts are returned; otherwise *n_verts / *n_tris give the capacities ([V][3] floats, [F][3]
+ * indices) and receive the counts. */
+rtcd_status rtcd_obj_read(const char* path, float* verts, uint32_t* n_verts, int32_t* tris, uint32_t* n_tris);
+/* Edge list of a triangle mesh (mesh.h:200-216 addEdge and the loop edges of 378-620):
+ * UNIQUE = undirected edges in first-seen order, LOOP = the three edges of every triangle.  edges is
+ * [E][2] vertex indices, or NULL to get the count; *n_edges is its capacity on entry. */
+enum { RTCD_EDGES_UNIQUE = 0, RTCD_EDGES_LOOP = 1 };
+rtcd_status rtcd_mesh_edges(const int32_t* tris, uint32_t n_tris, int32_t mode, int32_t* edges, uint32_t* n_edges);
+/* URDF serial chain -> FK masks, as scripts/URDFParser/urdfParserUtils.py:527-605 derives the
+ * constants of RTCD/Robot/models/panda.h:16-44 (Rodrigues on each revolute/continuous joint's origin
+ * and axis; fixed joints before a movable one fold into its origin), joint limits, link names and the
+ * resolved path of each link's first <visual> mesh ("" if none). */
+typedef struct {
+    uint32_t dof, n_links;
+    float base[12];
+    float cos_mask[15][12], sin_mask[15][12], one_mask[15][12];
+    float lower[15], upper[15];
+    char link_name[16][64];
+    char link_mesh[16][512];
+    float link_mesh_scale[16][3];   /* <mesh scale>, applied first */
+    float link_mesh_xf[16][12];     /* then the <visual> origin, 3x4 row-major */
+} rtcd_urdf;
+rtcd_status rtcd_urdf_read(const char* path, rtcd_urdf* out);
+/* Sphere set of one link (mesh.h:825-848): lines "x y z r", '#' comments.  xyzr [S][4] or NULL to
+ * get the count; *n is its capacity on entry. */
+rtcd_status rtcd_spheres_read(const char* path, float* xyzr, uint32_t* n);
+/* rtcd_robot_load from files: the URDF's chain and link meshes (moved by each <visual> origin and
+ * mesh scale), and, when sphere_dir is given, <sphere_dir>/<link name>.txt for each link that has one.
+ * Replaces robot<DOF>(robotConfig) with the generated header (RTCD/Robot/robot.h:91-175). */
+rtcd_status rtcd_robot_load_urdf(rtcd_ctx* ctx, const char* urdf_path, const char* sphere_dir);
+
 /* W = pinv(B) for the uniform quadratic B-spline, [n_ctrl][n_wp] row-major. */
 rtcd_status rtcd_quadratic_weights(uint32_t n_ctrl, uint32_t n_wp, double* W);
 /* W = pinv(B), [n_ctrl][n_wp] row-major, for a degree-2 or degree-3 B-spline sampled at n_wp

@@ -21,7 +21,8 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
            'rtcd_check_linear', 'rtcd_check_linear_device', 'rtcd_check_quadratic', 'rtcd_check_quadratic_device',
            'rtcd_quadratic_weights', 'rtcd_curve_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
            'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_wait_timed', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param',
-           'rtcd_set_leaf_size', 'rtcd_set_leaf_sizes', 'rtcd_autotune']
+           'rtcd_set_leaf_size', 'rtcd_set_leaf_sizes', 'rtcd_autotune', 'rtcd_obj_read', 'rtcd_mesh_edges',
+           'rtcd_urdf_read', 'rtcd_spheres_read', 'rtcd_robot_load_urdf']
 TUNE_LEAF_SIZE = 1
 PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad_budget': 4,
           'broad_budget_small': 5, 'broad_budget_next': 6, 'broad_passes': 7, 'sweep_passes': 8, 'graphs': 9,
@@ -70,6 +71,17 @@ class ServeStats(C.Structure):
                 ('mean_ms', C.c_double), ('max_ms', C.c_double), ('span_ms', C.c_double),
                 ('queued_max', C.c_uint32), ('lane_cus', C.c_uint32), ('queue_p50_ms', C.c_double),
                 ('queue_p99_ms', C.c_double)]
+
+
+class Urdf(C.Structure):
+    _fields_ = [('dof', C.c_uint32), ('n_links', C.c_uint32), ('base', C.c_float * 12),
+                ('cos_mask', (C.c_float * 12) * 15), ('sin_mask', (C.c_float * 12) * 15),
+                ('one_mask', (C.c_float * 12) * 15), ('lower', C.c_float * 15), ('upper', C.c_float * 15),
+                ('link_name', (C.c_char * 64) * 16), ('link_mesh', (C.c_char * 512) * 16),
+                ('link_mesh_scale', (C.c_float * 3) * 16), ('link_mesh_xf', (C.c_float * 12) * 16)]
+
+
+EDGES_UNIQUE, EDGES_LOOP = 0, 1
 
 
 class TuneReport(C.Structure):
@@ -127,6 +139,12 @@ def lib():
         L.rtcd_set_leaf_size.argtypes = [vp, u32]
         L.rtcd_set_leaf_sizes.argtypes = [vp, u32, u32, u32]
         L.rtcd_autotune.argtypes = [vp, C.POINTER(Request), u32, C.POINTER(TuneReport)]
+        cp = C.c_char_p
+        L.rtcd_obj_read.argtypes = [cp, vp, C.POINTER(u32), vp, C.POINTER(u32)]
+        L.rtcd_mesh_edges.argtypes = [vp, u32, i32, vp, C.POINTER(u32)]
+        L.rtcd_urdf_read.argtypes = [cp, C.POINTER(Urdf)]
+        L.rtcd_spheres_read.argtypes = [cp, vp, C.POINTER(u32)]
+        L.rtcd_robot_load_urdf.argtypes = [vp, cp, cp]
         for name in EXPORTS:
             if name not in ('rtcd_destroy', 'rtcd_last_error'):
                 getattr(L, name).restype = C.c_int32

@@ -57,6 +57,14 @@ class Context:
         check(lib().rtcd_robot_load(self.h, C.byref(rd)))
         self.robot = desc
 
+    def load_robot_urdf(self, urdf_path, sphere_dir=None):
+        """Load a robot straight from a URDF, its link meshes and optional sphere files (rtcd_robot_load_urdf)."""
+        import os
+        from . import ingest
+        check(lib().rtcd_robot_load_urdf(self.h, os.fsencode(urdf_path),
+                                         os.fsencode(sphere_dir) if sphere_dir else None))
+        self.robot = ingest.load_urdf(urdf_path, sphere_dir)
+
     def create_scene(self, verts, tris):
         verts = _f32(verts).reshape(-1, 3)
         tris = np.ascontiguousarray(tris, dtype=np.int32).reshape(-1, 3)
