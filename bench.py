@@ -20,6 +20,12 @@ import os
 import sys
 import time
 
+# The serving stream runs 4 slots, each on its own stream.  The HIP runtime gives a process 4 hardware
+# queues by default and maps further streams onto them, so a slot can end up behind another stream's
+# work in one queue (measured: small-request p99 3.9-4.9 ms with 4 queues, 1.4-1.5 ms with 8).  The
+# runtime reads this before it initialises.
+os.environ.setdefault('GPU_MAX_HW_QUEUES', '8')
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -45,7 +51,7 @@ def _boxes8():
     return s.verts, s.tris
 
 
-def serve_bench(ctx, rb, scene_ids, rate=2000.0, duration=4.0, slots=3, lane_cus=0):
+def serve_bench(ctx, rb, scene_ids, rate=2000.0, duration=4.0, slots=4, lane_cus=0):
     """Serving stream: Poisson arrivals at `rate` req/s, 50/30/20 discrete/linear/quadratic, sizes
     log-uniform 1..8,192, three resident scenes; submit->result latency (and its queue part), and the
     same requests run one by one through the blocking entry points (the plain per-request launch).

@@ -209,6 +209,9 @@ typedef struct {
  * (FIFO among equal sizes) unless a unit has waited more than 20 ms, and only while the units'
  * estimated device bytes fit `mem_budget` (0 = 8 GiB).  With more than one slot, slot 0 is a lane for
  * requests of at most RTCD_PARAM_SERVE_LANE_MAX queries (it never ages and never takes larger ones).
+ * Each slot needs a hardware queue of its own: the HIP runtime maps streams onto GPU_MAX_HW_QUEUES
+ * queues per process (4 by default), so with the context's and other libraries' streams a slot may
+ * share a queue and wait behind unrelated work; set GPU_MAX_HW_QUEUES >= slots + 2 before HIP starts.
  * input/offsets are read when a unit launches: the caller keeps them and `out` alive until
  * rtcd_wait returns.  A request whose unit alone exceeds the budget is rejected with RTCD_ERR_OOM. */
 rtcd_status rtcd_serve_start(rtcd_ctx* ctx, uint32_t slots, uint64_t mem_budget);
