@@ -1,9 +1,12 @@
 // Geometry ingestion on the host: OBJ meshes, URDF kinematic chains and sphere sets, so that a robot
 // or an obstacle is loaded from the same files RTCD reads.
 //
-// * OBJ (RTCD/Meshes/mesh.h:311-365, meshModel::loadOBJ through tinyobjloader): `v` and `f` records,
-//   polygons fan-triangulated, vertices deduplicated by exact value in first-use order (mesh.h:218-231,
-//   addVertex with float3less: -0 and +0 are one vertex).
+// * OBJ (RTCD/Meshes/mesh.h:311-365, meshModel::loadOBJ through tinyobjloader, fetched unpinned from
+//   its release branch by cmake/configure_tiny_obj_loader.cmake): `v` and `f` records; quads split
+//   along the shorter diagonal as that loader does, larger polygons fan-triangulated (the loader's
+//   newer releases use earcut there, so those faces are not pinned to it); vertices deduplicated by
+//   exact value in first-use order (mesh.h:218-231, addVertex with float3less: -0 and +0 are one
+//   vertex).
 // * edges (mesh.h:200-216 addEdge, 378-620): unique undirected edges in first-seen order, or the loop
 //   edges (three per triangle).  Keys are 64-bit (the reference's int Cantor hash overflows above
 //   ~32k vertices).
@@ -107,9 +110,22 @@ Mesh parse_obj(const std::string& text, const std::string& name) {
                 idx.push_back((int32_t) r);
             }
             if (idx.size() < 3) throw std::invalid_argument(name + ":" + std::to_string(lineno) + ": face with < 3 vertices");
-            for (size_t k = 1; k + 1 < idx.size(); ++k) {  // fan
-                const int32_t a = use(idx[0]), b = use(idx[k]), c = use(idx[k + 1]);
-                m.f.insert(m.f.end(), {a, b, c});
+            if (idx.size() == 4) {
+                // a quad splits along its shorter diagonal, (0,1,2)+(0,2,3) or (0,1,3)+(1,2,3), with the
+                // squared lengths in float, as tinyobjloader's release branch triangulates quads
+                auto sq = [&](int32_t p, int32_t q) {
+                    const float dx = raw[(size_t) q * 3] - raw[(size_t) p * 3], dy = raw[(size_t) q * 3 + 1] - raw[(size_t) p * 3 + 1],
+                                dz = raw[(size_t) q * 3 + 2] - raw[(size_t) p * 3 + 2];
+                    return dx * dx + dy * dy + dz * dz;
+                };
+                const int32_t q0 = use(idx[0]), q1 = use(idx[1]), q2 = use(idx[2]), q3 = use(idx[3]);
+                if (sq(idx[0], idx[2]) < sq(idx[1], idx[3])) m.f.insert(m.f.end(), {q0, q1, q2, q0, q2, q3});
+                else m.f.insert(m.f.end(), {q0, q1, q3, q1, q2, q3});
+            } else {
+                for (size_t k = 1; k + 1 < idx.size(); ++k) {  // fan
+                    const int32_t a = use(idx[0]), b = use(idx[k]), c = use(idx[k + 1]);
+                    m.f.insert(m.f.end(), {a, b, c});
+                }
             }
         }
     }

@@ -1,9 +1,9 @@
 """Pose and trajectory generators (numpy PCG64, ``numpy.random.default_rng(seed)``).
 
 The reference samples Halton poses and KNN trajectories for its benchmarks
-(RTCD/Planner/sampler/sampler.h:69-82, RTCD/Utils/Test/poseUtils.h:115-141); the workloads here use
-uniform joint samples, bounded random walks and clamped quadratic B-splines instead.  ``halton``
-restates the reference's radical-inverse sampler for callers that want its sequence.
+(RTCD/Planner/sampler/sampler.h:69-82, RTCD/Utils/Test/poseUtils.h:115-141): ``halton_reference``,
+``benchmark_poses_discrete`` and ``knn_trajectories`` restate them; the headline workloads use uniform
+joint samples, bounded random walks and clamped quadratic B-splines.
 """
 import numpy as np
 
@@ -31,6 +31,72 @@ def halton(n, dof, skip=0):
                 k //= b
             out[i, d] = r
     return out
+
+
+_HALTON_BASES = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71, 73, 79, 83, 89, 97)
+
+
+def halton_reference(n, lo, hi, skip=0):
+    """RTCD's Halton poses, float32 [n, dof]: sample i, joint d is h(i, b) * (hi[d] - lo[d]) + lo[d] with
+    b the (d + skip)-th prime and h the radical inverse computed in float32 exactly as
+    localHaltonSingleNumber does (RTCD/Planner/sampler/sampler.h:11-24, createSamplesHalton 69-82):
+    integer quotient stored in a float, digit n0 - n1 * b in float, f = 1/b divided by b per digit.
+    Sample 0 is lo."""
+    lo = np.asarray(lo, dtype=np.float32)
+    hi = np.asarray(hi, dtype=np.float32)
+    dof = len(lo)
+    if skip + dof >= len(_HALTON_BASES):
+        raise ValueError('skip + dof must stay below 25')
+    out = np.zeros((n, dof), dtype=np.float32)
+    for d in range(dof):
+        b = _HALTON_BASES[d + skip]
+        fb = np.float32(b)
+        n0 = np.arange(n, dtype=np.int64)
+        hn = np.zeros(n, dtype=np.float32)
+        f = np.full(n, np.float32(1) / fb, dtype=np.float32)
+        while np.any(n0 > 0):
+            act = n0 > 0
+            n1 = (n0 // b).astype(np.float32)                       # float n1 = n0 / b (integer division)
+            r = (n0.astype(np.float32) - n1 * fb).astype(np.int64)  # int r = n0 - n1 * b (float arithmetic)
+            hn = np.where(act, hn + f * r.astype(np.float32), hn).astype(np.float32)
+            f = np.where(act, f / fb, f).astype(np.float32)
+            n0 = np.where(act, n1.astype(np.int64), n0)
+        out[:, d] = hn * (hi[d] - lo[d]) + lo[d]
+    return out
+
+
+def benchmark_poses_discrete(n, lower, upper):
+    """The pose set of RTCD's discrete benchmark (RTCD/Benchmark/Discrete/benchmark.cpp:56-57): its
+    randomMotions call passes (lowerBound, upperBound) into parameters named (upper, lower)
+    (RTCD/Utils/Test/poseUtils.h:16-38), so the Halton samples run from the upper limits down:
+    q = h * (lower - upper) + upper."""
+    return halton_reference(n, upper, lower)
+
+
+def knn_trajectories(points, k, n_traj_pts, weight=None):
+    """RTCD's KNN trajectories (RTCD/Utils/Test/poseUtils.h:115-141 genCSpaceKNNTraj): for every
+    configuration i and each of its k nearest configurations j (weighted squared L2 over the joints,
+    accumulated in float32 joint by joint as RTCD/Planner/KNN/knnCUDA.cu:7-76 does, ties to the lower
+    index, i itself first at distance 0), the straight line start + span * s, span = (end - start) /
+    (n_traj_pts - 1), s = 0..n_traj_pts-1, in float32.  Returns float32 [len(points) * k, n_traj_pts, dof]."""
+    P = np.ascontiguousarray(points, dtype=np.float32)
+    n, dof = P.shape
+    w = np.ones(dof, np.float32) if weight is None else np.asarray(weight, dtype=np.float32)
+    w2 = w * w
+    idx = np.zeros((n, k), dtype=np.int64)
+    for i0 in range(0, n, 512):
+        q = P[i0:i0 + 512]
+        ssd = np.zeros((len(q), n), dtype=np.float32)
+        for d in range(dof):
+            t = (P[None, :, d] - q[:, None, d]).astype(np.float32)
+            ssd = (ssd + t * t * w2[d]).astype(np.float32)
+        idx[i0:i0 + len(q)] = np.argsort(ssd, axis=1, kind='stable')[:, :k]
+    start = P[:, None, :]
+    end = P[idx]
+    span = ((end - start) / np.float32(n_traj_pts - 1)).astype(np.float32)
+    s = np.arange(n_traj_pts, dtype=np.float32)
+    traj = (start[:, :, None, :] + span[:, :, None, :] * s[None, None, :, None]).astype(np.float32)
+    return np.ascontiguousarray(traj.reshape(n * k, n_traj_pts, dof))
 
 
 def random_walk_trajectories(n, lower, upper, lengths=32, max_step=0.1, seed=4):

@@ -177,3 +177,40 @@ def scene_dense(seed_shelf=5, seed_blob=2):
         meshes.append((sh.verts[vid], np.vectorize(remap.get)(t).astype(np.int32)))
     meshes.append((bv, bt))
     return merge(meshes)
+
+
+# RTCD's benchmark scenes: cuRobo obstacle meshes placed by pure translations (scale 1), in insertion
+# order.  The tables restate RTCD/CollisionScenes/Scene/shelfSimple.h:8-24, shelf.h:8-54 and
+# denseShelf.h:8-72 (file, translation); the meshes come from data/curobo_scenes.npz, which
+# data/make_curobo_scenes.py writes from the reference's models/curobo/*.obj through the library's
+# OBJ reader.
+_SHELF = ('shelves', (0.45, 0.0, 0.4))
+_CUROBO_SCENES = {
+    'shelf_simple': (_SHELF, ('bin', (0.0, -0.6, 0.0)), ('bin', (0.0, 0.6, 0.0))),
+    'shelf': (_SHELF, ('bin', (0.0, -0.6, 0.0)), ('bin', (0.0, 0.6, 0.0)), ('Bob', (-0.4, 0.0, 0.0)),
+              ('Cow', (-0.4, 0.0, 0.0)), ('Fish', (-0.4, 0.0, 0.0)), ('Sheep', (-0.4, 0.0, 0.0)),
+              ('Snakeboard', (-0.4, 0.0, 0.0))),
+    'dense_shelf': (_SHELF, ('bin_dense', (0.0, -0.6, 0.0)), ('bin_dense', (0.0, 0.6, 0.0)),
+                    ('Bob', (-0.4, 0.0, 0.0)), ('Face1', (-0.4, 0.0, 0.0)), ('Face2', (-0.4, 0.0, 0.0)),
+                    ('Face3', (-0.4, 0.0, 0.0)), ('Cow', (-0.4, 0.0, 0.0)), ('Fish', (-0.4, 0.0, 0.0)),
+                    ('Sheep', (-0.4, 0.0, 0.0)), ('Snakeboard', (-0.4, 0.0, 0.0))),
+}
+CUROBO_SCENES = tuple(_CUROBO_SCENES)
+_curobo_meshes = None
+
+
+def curobo_scene(name):
+    """One of RTCD's benchmark scenes ('shelf_simple', 'shelf', 'dense_shelf').  Each mesh is moved as
+    RTCD's obstacle constructor moves it (RTCD/Meshes/mesh.h:258-265: scale, then the Eigen affine
+    transform, in float32), which for a translation is v + t rounded to float32."""
+    global _curobo_meshes
+    if _curobo_meshes is None:
+        import os
+        d = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data', 'curobo_scenes.npz'),
+                    allow_pickle=False)
+        _curobo_meshes = {k: d[k] for k in d.files}
+    meshes = []
+    for mesh, t in _CUROBO_SCENES[name]:
+        v = _curobo_meshes[mesh + '_verts']
+        meshes.append(((v + np.float32(t)).astype(np.float32), _curobo_meshes[mesh + '_tris']))
+    return merge(meshes)

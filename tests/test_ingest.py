@@ -17,17 +17,20 @@ REF = '/root/reference/models/franka_description'
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
 
 
-def test_obj_fan_negative_indices_and_dedup(tmp_path):
+def test_obj_polygons_negative_indices_and_dedup(tmp_path):
     p = tmp_path / 'm.obj'
     p.write_text('# comment\n'
-                 'v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\n'
-                 'v -0.0 0.0 0\n'            # same value as vertex 1 (-0 == +0)
+                 'v 0 0 0\nv 1 0 0\nv 3 3 0\nv 0 1 0\n'
+                 'v -0.0 0.0 0\n'              # same value as vertex 1 (-0 == +0)
                  'vn 0 0 1\nvt 0 0\n'
-                 'f 1/1/1 2/1/1 3/1/1 4/1/1\n'  # quad -> (1,2,3), (1,3,4)
-                 'f -1 -4 -3\n')               # vertex 5 (= 1), 2, 3
+                 'f 1/1/1 2/1/1 3/1/1 4/1/1\n'  # quad, diagonal 2-4 shorter -> (1,2,4), (2,3,4)
+                 'f 2 3 4 1\n'                  # the same quad from vertex 2: diagonal 2-4 -> (2,3,4), (2,4,1)
+                 'f -1 -4 -3\n'                 # vertex 5 (= 1), 2, 3
+                 'f 1 2 3 4 5\n')               # pentagon: fan from vertex 1
     v, t = ingest.read_obj(str(p))
-    np.testing.assert_array_equal(v, np.float32([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]]))
-    np.testing.assert_array_equal(t, [[0, 1, 2], [0, 2, 3], [0, 1, 2]])
+    np.testing.assert_array_equal(v, np.float32([[0, 0, 0], [1, 0, 0], [3, 3, 0], [0, 1, 0]]))
+    np.testing.assert_array_equal(t, [[0, 1, 3], [1, 2, 3], [1, 2, 3], [1, 3, 0], [0, 1, 2],
+                                      [0, 1, 2], [0, 2, 3], [0, 3, 0]])
 
 
 def test_obj_errors(tmp_path):
