@@ -23,8 +23,12 @@ import time
 # The serving stream runs 4 slots, each on its own stream.  The HIP runtime gives a process 4 hardware
 # queues by default and maps further streams onto them, so a slot can end up behind another stream's
 # work in one queue (measured: small-request p99 3.9-4.9 ms with 4 queues, 1.4-1.5 ms with 8).  The
-# runtime reads this before it initialises.
-os.environ.setdefault('GPU_MAX_HW_QUEUES', '8')
+# runtime reads this before it initialises; a larger value already set is kept.
+try:
+    _q = int(os.environ.get('GPU_MAX_HW_QUEUES', '0'))
+except ValueError:
+    _q = 0
+os.environ['GPU_MAX_HW_QUEUES'] = str(max(_q, 8))
 
 import numpy as np
 
@@ -89,6 +93,60 @@ def serve_bench(ctx, rb, scene_ids, rate=2000.0, duration=4.0, slots=4, lane_cus
             'plain_per_request_items_per_s': round(items / plain_wall, 1),
             'served_over_plain': round(plain_wall / wall, 2), 'mismatches_alone_vs_mix': mism,
             'queued_max': int(st['queued_max'])}
+
+
+# RTCD's published RTX 4070 figures for its own scenes at 4,096 poses or trajectories per batch
+# (read off its plots, about +-10 %): discrete us/pose (obstacle->robot, robot->obstacle, two-way),
+# us per single-pose batch, and us/trajectory for linear-16 and quadratic-8 curves.
+REF_RTX4070 = {
+    'shelf_simple': {'o2r_us_per_pose': 0.30, 'r2o_us_per_pose': 0.38, 'two_way_us_per_pose': 0.69,
+                     'linear16_us_per_traj': 7.6, 'quadratic8_us_per_traj': 19.0},
+    'shelf': {'o2r_us_per_pose': 0.36, 'r2o_us_per_pose': 0.48, 'two_way_us_per_pose': 0.85,
+              'linear16_us_per_traj': 11.4, 'quadratic8_us_per_traj': 27.0},
+    'dense_shelf': {'o2r_us_per_pose': 0.42, 'r2o_us_per_pose': 0.48, 'two_way_us_per_pose': 0.90,
+                    'single_pose_batch_us': {'o2r': 94, 'r2o': 51, 'two_way': 147},
+                    'linear16_us_per_traj': 21.0, 'quadratic8_us_per_traj': 54.5},
+}
+
+
+def reference_scene_bench(ctx, rb, reps=5):
+    """RTCD's own benchmark inputs on its three cuRobo scenes: 4,096 of its 8,192 Halton poses per batch
+    (each direction and both), single-pose batches, and 4,096 of its KNN trajectories (16 waypoints,
+    linear and quadratic with 8 control points).  Wall time of the blocking host-buffer entry points,
+    the way RTCD's Google-Benchmark loop times check() (median of `reps`)."""
+    from rtcollisiondetection_amd import samplers, scenes
+    q8k = samplers.benchmark_poses_discrete(8192, rb.lower, rb.upper)
+    q = q8k[np.random.default_rng(0).choice(8192, 4096, replace=False)]
+    pts = samplers.halton_reference(8192, rb.lower, rb.upper)
+    tr_all = samplers.knn_trajectories(pts, 4, 16)
+    tr = tr_all[np.random.default_rng(1).choice(len(tr_all), 4096, replace=False)]
+    offs = np.arange(0, (len(tr) + 1) * 16, 16, dtype=np.uint32)
+    W = samplers.quadratic_fit_weights(8, 16)
+
+    def med(f, r=reps):
+        f()
+        ts = []
+        for _ in range(r):
+            t = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts))
+
+    out = {}
+    for name in scenes.CUROBO_SCENES:
+        sc = scenes.curobo_scene(name)
+        sid = ctx.create_scene(sc.verts, sc.tris)
+        r = {'tris': int(sc.n_tris)}
+        for key, d in (('o2r', 1), ('r2o', 2), ('two_way', 3)):
+            r[f'{key}_us_per_pose'] = round(med(lambda: ctx.check_discrete(sid, q, d)) / len(q) * 1e6, 4)
+            r[f'{key}_single_pose_us'] = round(med(lambda: ctx.check_discrete(sid, q[:1], d), 20) * 1e6, 1)
+        r['collisions_two_way'] = int(ctx.check_discrete(sid, q, 3).astype(bool).sum())
+        r['linear16_us_per_traj'] = round(med(lambda: ctx.check_linear(sid, tr.reshape(-1, 7), offs)) / len(tr) * 1e6, 3)
+        r['quadratic8_us_per_traj'] = round(med(lambda: ctx.check_quadratic(sid, tr, 8, W)) / len(tr) * 1e6, 3)
+        r['rtx4070_published'] = REF_RTX4070[name]
+        out[name] = r
+        ctx.destroy_scene(sid)
+    return out
 
 
 def main():
@@ -276,6 +334,8 @@ def main():
             ctx.check_quadratic(sid, wp, 8)
         extra['dense_quadratic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
         log(f'[bench] extra sweeps { {k: v for k, v in extra.items() if "traj" in k or "ms_device" in k} }')
+        extra['reference_scenes'] = reference_scene_bench(ctx, rb)
+        log(f'[bench] reference scenes {extra["reference_scenes"]}')
         extra['serving'] = serve_bench(ctx, rb, [ctx.create_scene(*_boxes8()), sid2, sid])
         log(f'[bench] serving {extra["serving"]}')
 

@@ -164,7 +164,11 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.spill_count = ws.scalars.p + 8;
     a.budget = tasks < c->frontier_max_tasks ? c->broad_budget_small : c->broad_budget;
     a.budget_next = c->broad_budget_next;
-    a.n_passes = (int) c->broad_passes;
+    // batches of at most tiny_batch poses: a few launches cost more than any imbalance the hand-off
+    // passes remove (one pose: 158 -> 117 us on the 202k-triangle shelf with 2 broad passes and one
+    // robot->obstacle pass)
+    const bool tiny = n <= c->tiny_batch;
+    a.n_passes = tiny ? (int) std::min<uint32_t>(c->broad_passes, 2u) : (int) c->broad_passes;
     a.nspill_cap = (uint32_t) std::min<size_t>(ws.nspill.n / 2, 0x7fffffffu);
     a.nspill[0] = ws.nspill.p;
     a.nspill[1] = ws.nspill.p + a.nspill_cap;
@@ -192,7 +196,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.nspill_count = ws.scalars.p + 40;  // fresh hand-off counters for robot->obstacle
     a.narrow_budget = c->narrow_budget;
     a.narrow_budget_next = c->narrow_budget_next;
-    a.narrow_passes = (int) c->narrow_passes;
+    a.narrow_passes = tiny ? 1 : (int) c->narrow_passes;
     if (dir & 2) launch_narrow(a, 2, c->robot_depth + 1, ws.counting, s);
     launch_redecide(a, s);
     mark(5);
@@ -221,7 +225,8 @@ static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, c
             P(ws.unsure.p), ws.unsure.n, c->leaf_scene_tri, c->narrow_budget, c->narrow_budget_next, c->narrow_passes, c->o2r_budget,
             c->o2r_budget_next, c->o2r_passes,
             P(ws.scalars.p), P(ws.stats.p), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
-            c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth, c->narrow_bs};
+            c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth, c->narrow_bs,
+            c->tiny_batch};
 }
 
 void enqueue_discrete_main(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_poses, uint32_t n, uint32_t* d_out) {
@@ -860,6 +865,10 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
             return fail(RTCD_ERR_INVALID_ARG, "lane CUs must be 0 or a multiple of 32 up to 224");
         ctx->serve_lane_cus = (uint32_t) value;
         return RTCD_OK;
+    case RTCD_PARAM_TINY_BATCH:
+        if (value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "tiny batch must fit 32 bits");
+        ctx->tiny_batch = (uint32_t) value;
+        return RTCD_OK;
     case RTCD_PARAM_SERVE_LANE_KINDS:
         if (value < 1 || value > 7) return fail(RTCD_ERR_INVALID_ARG, "lane kinds must be a non-empty mask of bits 0..2");
         ctx->serve_lane_kinds = (uint32_t) value;
@@ -905,6 +914,7 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_SERVE_CHUNK_DISCRETE: *value = ctx->serve_chunk_discrete; return RTCD_OK;
     case RTCD_PARAM_SERVE_CHUNK_SWEEP: *value = ctx->serve_chunk_sweep; return RTCD_OK;
     case RTCD_PARAM_SERVE_LANE_KINDS: *value = ctx->serve_lane_kinds; return RTCD_OK;
+    case RTCD_PARAM_TINY_BATCH: *value = ctx->tiny_batch; return RTCD_OK;
     default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
 }

@@ -201,7 +201,7 @@ struct rtcd_ctx {
     uint32_t broad_budget_small = 16;           // the same for batches below frontier_max_tasks tasks
     uint32_t broad_budget_next = 16;            // the same in the passes after the first
     uint32_t broad_passes = 12;                 // broad hand-off passes (the last never hands off)
-    uint32_t sweep_passes = 8;                  // sweep hand-off passes
+    uint32_t sweep_passes = 12;                 // sweep hand-off passes
     uint32_t narrow_bs = 64;                    // work-group size of the discrete narrow passes
     // BVH leaf sizes (rtcd_config.leaf_size, rtcd_set_leaf_sizes): robot link BVHs, scene triangle
     // BVH (also the threads per broad-phase leaf record), scene edge BVH (sweeps)
@@ -248,6 +248,7 @@ struct rtcd_ctx {
     // serve_chunk_sweep trajectories
     uint32_t serve_lane_max = 1024, serve_lane_cus = 0, serve_chunk_discrete = 4096, serve_chunk_sweep = 1024;
     uint32_t serve_lane_kinds = 7;  // request kinds the lane takes: bit (kind - 1)
+    uint32_t tiny_batch = 4;  // discrete batches of at most this many poses run 2 broad passes, 1 narrow pass
     // serving loop
     ServeState* serve = nullptr;
     size_t serve_bytes = 0;  // device bytes held by serving slots

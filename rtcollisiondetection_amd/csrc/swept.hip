@@ -80,27 +80,48 @@ __global__ __launch_bounds__(kBS) void k_seg_maps(const uint32_t* __restrict__ o
     }
 }
 
-// P = W X for each (trajectory, sphere): same summation order as the oracle.
+// P = W X for each (trajectory, sphere): same summation order as the oracle (for each control point j
+// and coordinate, ((W[j,0] x_0 + W[j,1] x_1) + W[j,2] x_2) + ..., no contraction).  The waypoint loop
+// is the outer one, so each centre is read once per block of 8 control points, not once per point.
 __global__ __launch_bounds__(kBS) void k_fit(const double* __restrict__ c64, const float4* __restrict__ c32,
                                             const double* __restrict__ W, int n_traj, int nw, int nc, int S,
                                             double* __restrict__ P64, float4* __restrict__ P32) {
+#pragma clang fp contract(off)
+    constexpr int kJ = 8;
     int id = blockIdx.x * kBS + threadIdx.x;
     if (id >= n_traj * S) return;
     const int t = id / S, s = id - t * S;
     const float r = c32[(size_t) t * nw * S + s].w;
-    for (int j = 0; j < nc; ++j) {
-        double o[3];
-        for (int c = 0; c < 3; ++c) {
-#pragma clang fp contract(off)
-            double acc = W[(size_t) j * nw] * c64[(((size_t) t * nw + 0) * S + s) * 3 + c];
-            for (int k = 1; k < nw; ++k) acc = acc + W[(size_t) j * nw + k] * c64[(((size_t) t * nw + k) * S + s) * 3 + c];
-            o[c] = acc;
+    for (int j0 = 0; j0 < nc; j0 += kJ) {
+        const int nj = min(kJ, nc - j0);
+        double acc[kJ][3];
+        for (int k = 0; k < nw; ++k) {
+            const double* x = c64 + (((size_t) t * nw + k) * S + s) * 3;
+            const double x0 = x[0], x1 = x[1], x2 = x[2];
+#pragma unroll
+            for (int jj = 0; jj < kJ; ++jj) {
+                if (jj >= nj) break;
+                const double w = W[(size_t) (j0 + jj) * nw + k];
+                if (k == 0) {
+                    acc[jj][0] = w * x0;
+                    acc[jj][1] = w * x1;
+                    acc[jj][2] = w * x2;
+                } else {
+                    acc[jj][0] = acc[jj][0] + w * x0;
+                    acc[jj][1] = acc[jj][1] + w * x1;
+                    acc[jj][2] = acc[jj][2] + w * x2;
+                }
+            }
         }
-        const size_t pi = ((size_t) t * S + s) * nc + j;
-        P64[pi * 3] = o[0];
-        P64[pi * 3 + 1] = o[1];
-        P64[pi * 3 + 2] = o[2];
-        P32[pi] = make_float4((float) o[0], (float) o[1], (float) o[2], r);
+#pragma unroll
+        for (int jj = 0; jj < kJ; ++jj) {
+            if (jj >= nj) break;
+            const size_t pi = ((size_t) t * S + s) * nc + j0 + jj;
+            P64[pi * 3] = acc[jj][0];
+            P64[pi * 3 + 1] = acc[jj][1];
+            P64[pi * 3 + 2] = acc[jj][2];
+            P32[pi] = make_float4((float) acc[jj][0], (float) acc[jj][1], (float) acc[jj][2], r);
+        }
     }
 }
 

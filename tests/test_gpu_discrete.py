@@ -159,3 +159,27 @@ def test_device_api_matches_host_api(ctx, panda):
     ctx.sync()
     assert np.array_equal(dout.cpu().numpy().astype(np.uint32), host)
     ctx.destroy_scene(sid)
+
+
+def test_tiny_batches_fewer_passes(ctx, panda, orc_panda):
+    """Batches of at most RTCD_PARAM_TINY_BATCH poses run 2 broad passes and one robot->obstacle pass:
+    flags equal the full pass schedule and the oracle, for sizes around the threshold."""
+    sc = scenes.scene_dense()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    m = O.scene_mesh(sc)
+    q = samplers.uniform_poses(64, panda.lower, panda.upper, seed=77)
+    try:
+        assert ctx.get_param('tiny_batch') == 4
+        for n in (1, 2, 3, 4, 5, 8):
+            for d in (1, 2, 3):
+                ref = O.check_discrete(orc_panda, m, q[:n], d)
+                ctx.set_param('tiny_batch', 4)
+                a = ctx.check_discrete(sid, q[:n], d)
+                ctx.set_param('tiny_batch', 0)
+                b = ctx.check_discrete(sid, q[:n], d)
+                assert _mismatch(a, ref) == 0 and np.array_equal(a, b), (n, d)
+        whole = O.check_discrete(orc_panda, m, q, 3)
+        assert 0 < whole.sum() < len(q)
+    finally:
+        ctx.set_param('tiny_batch', 4)
+        ctx.destroy_scene(sid)
