@@ -12,7 +12,7 @@ thread_local std::string g_err;
 }  // namespace rtcd
 
 namespace {
-int stack_for(int depth) { return depth + 2 <= 32 ? 32 : 64; }
+int stack_for(int depth) { return stack_tier(depth); }
 
 void budget_check(const rtcd_ctx* c, const Workspace& ws, size_t budget, size_t extra) {
     if (budget && ws.bytes() + extra > budget)

@@ -874,7 +874,10 @@ static void broad_t(const DiscreteArgs& a, bool count, hipStream_t s) {
 }
 
 void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s) {
-    if (stack <= 32) broad_t<32>(a, count, s);
+    if (stack <= 20) broad_t<20>(a, count, s);
+    else if (stack <= 26) broad_t<26>(a, count, s);
+    else if (stack <= 32) broad_t<32>(a, count, s);
+    else if (stack <= 40) broad_t<40>(a, count, s);
     else if (stack <= 64) broad_t<64>(a, count, s);
     else throw std::runtime_error("BVH deeper than 64 levels");
 }

@@ -131,6 +131,15 @@ struct SweepArgs {
 void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s);
 // Quadratic pieces are split into this many sub-pieces (work items) each.
 constexpr int kSweepSub = 4;
+// LDS traversal stack (entries per thread) for a BVH whose longest chain of inner nodes is `depth`.  A
+// walk pushes only at inner nodes with two inner children, so it never holds more than depth - 1
+// entries; the tiers 20, 26, 32, 40, 64 are the largest stacks that keep 8, 6, 5, 4 and 2 work-groups
+// of 256 threads in 160 KB of LDS.  A tree deeper than 64 overflows the largest stack, which the
+// kernels report and the host turns into an error.
+inline int stack_tier(int depth) {
+    return depth <= 20 ? 20 : depth <= 26 ? 26 : depth <= 32 ? 32 : depth <= 40 ? 40 : 64;
+}
+
 void quadratic_weights(int nc, int nw, double* W);
 void curve_weights(int degree, bool clamped, int nc, int nw, double* W);
 

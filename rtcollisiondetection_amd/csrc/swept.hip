@@ -9,6 +9,8 @@
 // edge LBVH with its box, and each candidate edge is tested against the tube in FP32 with a
 // certified band (closest point + convexity lower bound); undecided pairs are re-decided by the
 // FP64 Bernstein-subdivision test that the oracle uses, bit for bit.
+#include <type_traits>
+
 #include "band32.hpp"
 #include "ctx.hpp"
 #include "exact64.hpp"
@@ -560,13 +562,16 @@ void quadratic_weights(int nc, int nw, double* W) { curve_weights(2, false, nc, 
 void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStream_t s) {
     if (!a.sc.edge_has_root || a.n_items == 0) return;
     if (stack > 64) throw std::runtime_error("BVH deeper than 64 levels");
-    if (degree == 1) {
-        if (stack <= 32) sweep_t<1, 32>(a, count, s);
-        else sweep_t<1, 64>(a, count, s);
-    } else {
-        if (stack <= 32) sweep_t<2, 32>(a, count, s);
-        else sweep_t<2, 64>(a, count, s);
-    }
+    auto go = [&](auto deg) {
+        constexpr int D = decltype(deg)::value;
+        if (stack <= 20) sweep_t<D, 20>(a, count, s);
+        else if (stack <= 26) sweep_t<D, 26>(a, count, s);
+        else if (stack <= 32) sweep_t<D, 32>(a, count, s);
+        else if (stack <= 40) sweep_t<D, 40>(a, count, s);
+        else sweep_t<D, 64>(a, count, s);
+    };
+    if (degree == 1) go(std::integral_constant<int, 1>{});
+    else go(std::integral_constant<int, 2>{});
 }
 
 }  // namespace rtcd
@@ -721,7 +726,7 @@ void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_tra
     a.n_ctrl = 2;
     sweep_work(c, ws, a, sweep_items(c, (size_t) std::max(n_seg, 0), false), budget);
     const int depth = sc.edge_bvh.depth;
-    if (n_seg > 0) launch_sweep(a, 1, depth + 2 <= 32 ? 32 : 64, ws.counting, s);
+    if (n_seg > 0) launch_sweep(a, 1, stack_tier(depth), ws.counting, s);
     sweep_end(ws);
 }
 
@@ -759,7 +764,7 @@ void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_
     a.P32 = ws.P32.p;
     sweep_work(c, ws, a, sweep_items(c, (size_t) a.n_seg, true), budget);
     const int depth = sc.edge_bvh.depth;
-    if (a.n_seg > 0) launch_sweep(a, 2, depth + 2 <= 32 ? 32 : 64, ws.counting, s);
+    if (a.n_seg > 0) launch_sweep(a, 2, stack_tier(depth), ws.counting, s);
     sweep_end(ws);
 }
 
