@@ -205,11 +205,13 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
 }
 
 void drop_graph(rtcd_ctx* c) {
-    if (c->dgraph.exec) (void) hipGraphExecDestroy(c->dgraph.exec);
-    c->dgraph.exec = nullptr;
-    c->dgraph.exec_key.clear();
-    c->dgraph.key.clear();
-    c->dgraph.failed_key.clear();
+    for (auto* g : {&c->dgraph, &c->sgraph}) {
+        if (g->exec) (void) hipGraphExecDestroy(g->exec);
+        g->exec = nullptr;
+        g->exec_key.clear();
+        g->key.clear();
+        g->failed_key.clear();
+    }
 }
 
 // Everything a captured discrete check bakes in: shape, the caller's buffers, every workspace,

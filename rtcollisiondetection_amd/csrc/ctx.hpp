@@ -112,6 +112,7 @@ struct Workspace {
     DevBuf<float4> P32;       // [n_traj][S][n_ctrl]
     DevBuf<Stats> stats;
     std::vector<double> W_host;  // last quadratic weights uploaded to W64
+    uint64_t W_version = 0;      // bumped whenever W64 gets other weights (part of a captured sweep's key)
     // timing: ev[0] start, ev[1..4] after FK, broad phase, gather, obstacle->robot narrow; ev[5] end
     hipEvent_t ev[6] = {};
     bool timing_pending = false;
@@ -240,7 +241,7 @@ struct rtcd_ctx {
         std::vector<uint64_t> failed_key; // a key whose capture failed: launched eagerly
         hipGraphExec_t exec = nullptr;
         bool last_replayed = false;
-    } dgraph;
+    } dgraph, sgraph;  // discrete checks, swept checks
     bool use_graphs = true;
     // serving policy, read by rtcd_serve_start: slot 0 serves only requests of at most serve_lane_max
     // queries (0: any request); with serve_lane_cus > 0 it runs on that many CUs and the other slots on
@@ -285,9 +286,10 @@ void rebuild_bvhs(rtcd_ctx* c);
 void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
                      uint32_t* d_out, size_t budget, rtcd_stats* st);
 void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_traj, const uint32_t* d_offs,
-                    uint32_t n_traj, uint32_t nw, uint32_t* d_out, uint32_t* d_seg, size_t budget);
+                    uint32_t n_traj, uint32_t nw, uint32_t* d_out, uint32_t* d_seg, size_t budget, bool events = true);
 void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
-                       uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget);
+                       uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget,
+                       bool events = true);
 // After ws.stream finished: re-runs (through `rerun`) when the FP64 queue overflowed.
 void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::function<void()>& rerun);
 // Device bytes a request of this shape needs in a fresh workspace (admission control).

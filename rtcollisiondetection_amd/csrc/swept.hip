@@ -632,13 +632,13 @@ static SweepArgs base_args(rtcd_ctx* c, Workspace& ws, SceneRes& sc, uint32_t* d
     return a;
 }
 
-static void sweep_begin(Workspace& ws) {
+static void sweep_begin(Workspace& ws, bool events) {
     ws.stats.reserve(1);
     ws.scalars.reserve(kScalars);
     if (ws.unsure.n < ((size_t) 1 << 16)) ws.unsure.reserve((size_t) 1 << 16);
     fill_u32(ws.scalars.p, kScalars, 0u, ws.stream);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, ws.stream);
-    RTCD_HIP(hipEventRecord(ws.ev[0], ws.stream));
+    if (events) RTCD_HIP(hipEventRecord(ws.ev[0], ws.stream));
 }
 
 // Hand-off queues of a sweep launch: two halves of the workspace candidate buffer, used in turn by
@@ -660,9 +660,9 @@ static void sweep_work(rtcd_ctx* c, Workspace& ws, SweepArgs& a, size_t items, s
     a.n_passes = (int) c->sweep_passes;
 }
 
-static void sweep_end(Workspace& ws) {
-    RTCD_HIP(hipEventRecord(ws.ev[5], ws.stream));
-    ws.timing_pending = true;
+static void sweep_end(Workspace& ws, bool events) {
+    if (events) RTCD_HIP(hipEventRecord(ws.ev[5], ws.stream));
+    ws.timing_pending = events;
 }
 
 void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::function<void()>& rerun) {
@@ -699,15 +699,16 @@ void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::functio
             for (int p = 0; p + 1 < (int) c->sweep_passes; ++p) tot += sc[8 + p];
             st->leaf_candidates = tot;
         }
+        st->graph_replayed = &ws == &c->main && c->sgraph.last_replayed ? 1u : 0u;
     }
     ws.timing_pending = false;
 }
 
 void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_traj, const uint32_t* d_offs,
-                    uint32_t n_traj, uint32_t nw, uint32_t* d_out, uint32_t* d_seg, size_t budget) {
+                    uint32_t n_traj, uint32_t nw, uint32_t* d_out, uint32_t* d_seg, size_t budget, bool events) {
     const hipStream_t s = ws.stream;
     const int n_seg = (int) (nw - n_traj);
-    sweep_begin(ws);
+    sweep_begin(ws, events);
     fill_u32(d_out, n_traj, 0u, s);
     if (d_seg && n_seg > 0) fill_u32(d_seg, (size_t) n_seg, 0u, s);
     enqueue_centers(c, ws, d_traj, (int) nw);
@@ -727,22 +728,29 @@ void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_tra
     sweep_work(c, ws, a, sweep_items(c, (size_t) std::max(n_seg, 0), false), budget);
     const int depth = sc.edge_bvh.depth;
     if (n_seg > 0) launch_sweep(a, 1, stack_tier(depth), ws.counting, s);
-    sweep_end(ws);
+    sweep_end(ws, events);
 }
 
-void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
-                       uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget) {
-    const hipStream_t s = ws.stream;
-    const int S = c->n_spheres;
-    const int npc = (int) nc - 2;
+// The fit weights of a quadratic check in ws.W64: the caller's W, or the default uniform-knot fit.
+// Uploaded only when they differ from the last ones, so a captured check never copies them.
+static void prepare_weights(Workspace& ws, uint32_t nc, uint32_t nw, const double* W) {
     std::vector<double> Wh((size_t) nc * nw);
     if (W) std::copy(W, W + Wh.size(), Wh.begin());
     else quadratic_weights((int) nc, (int) nw, Wh.data());
-    sweep_begin(ws);
     if (ws.W_host != Wh) {
-        ws.W_host = Wh;  // kept alive in the workspace for the asynchronous copy
-        ws.W64.upload(ws.W_host, s);
+        ws.W_host = std::move(Wh);  // kept alive in the workspace for the asynchronous copy
+        ws.W64.upload(ws.W_host, ws.stream);
+        ++ws.W_version;
     }
+}
+
+void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
+                       uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget, bool events) {
+    const hipStream_t s = ws.stream;
+    const int S = c->n_spheres;
+    const int npc = (int) nc - 2;
+    prepare_weights(ws, nc, nw, W);
+    sweep_begin(ws, events);
     fill_u32(d_out, n_traj, 0u, s);
     if (d_seg) fill_u32(d_seg, (size_t) n_traj * npc, 0u, s);
     enqueue_centers(c, ws, d_wp, (int) (n_traj * nw));
@@ -765,7 +773,70 @@ void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_
     sweep_work(c, ws, a, sweep_items(c, (size_t) a.n_seg, true), budget);
     const int depth = sc.edge_bvh.depth;
     if (a.n_seg > 0) launch_sweep(a, 2, stack_tier(depth), ws.counting, s);
-    sweep_end(ws);
+    sweep_end(ws, events);
+}
+
+// Everything a captured swept check bakes in: kind and shape, the caller's buffers, every workspace,
+// scene and robot buffer with its capacity, the fit weights' version and the sweep parameters.
+static std::vector<uint64_t> sweep_key(const rtcd_ctx* c, const Workspace& ws, const SceneRes& sc, int degree,
+                                       const void* d_in, const uint32_t* d_offs, uint32_t n_traj, uint32_t nw,
+                                       uint32_t nc, const uint32_t* d_out, const uint32_t* d_seg) {
+    auto P = [](const void* p) { return (uint64_t) (uintptr_t) p; };
+    return {(uint64_t) degree, P(&sc), P(sc.edge_bvh.nodes.p), P(sc.edges.p), P(c->d_robot.p), P(c->r_spheres.p),
+            P(c->r_sphere_link.p), P(c->sphere_radius.p), P(ws.stream), P(d_in), P(d_offs), P(d_out), P(d_seg),
+            n_traj, nw, nc, ws.W_version, P(ws.W64.p), P(ws.c64.p), ws.c64.n, P(ws.c32.p), ws.c32.n,
+            P(ws.seg_traj.p), ws.seg_traj.n, P(ws.seg_wp.p), ws.seg_wp.n, P(ws.P64.p), ws.P64.n, P(ws.P32.p),
+            ws.P32.n, P(ws.cand.p), ws.cand.n, P(ws.unsure.p), ws.unsure.n, P(ws.scalars.p), P(ws.stats.p),
+            c->sweep_budget, c->sweep_passes};
+}
+
+// A swept check on the context's workspace: the first call with a key launches eagerly, the second
+// captures the launches into a graph (every buffer then has its size), later ones replay it.  Other
+// workspaces, counting launches and contexts with graphs off always launch eagerly.
+template <typename K, typename E>
+static void sweep_main(rtcd_ctx* c, Workspace& ws, K&& key, E&& enqueue) {
+    auto& g = c->sgraph;
+    g.last_replayed = false;
+    if (&ws != &c->main || !c->use_graphs || ws.counting) {
+        enqueue(true);
+        return;
+    }
+    const hipStream_t s = ws.stream;
+    std::vector<uint64_t> k = key();
+    if (!(g.exec && g.exec_key == k) && g.key == k && g.failed_key != k) {
+        if (g.exec) (void) hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.exec_key.clear();
+        g.failed_key = k;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            bool ok = true;
+            try {
+                enqueue(false);
+            } catch (...) {
+                ok = false;
+            }
+            hipGraph_t graph = nullptr;
+            if (hipStreamEndCapture(s, &graph) != hipSuccess) ok = false;
+            hipGraphExec_t exec = nullptr;
+            if (ok && graph && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                g.exec = exec;
+                g.exec_key = k;
+                g.failed_key.clear();
+            }
+            if (graph) (void) hipGraphDestroy(graph);
+        }
+        (void) hipGetLastError();
+    }
+    if (g.exec && g.exec_key == k) {
+        RTCD_HIP(hipEventRecord(ws.ev[0], s));
+        RTCD_HIP(hipGraphLaunch(g.exec, s));
+        RTCD_HIP(hipEventRecord(ws.ev[5], s));
+        ws.timing_pending = true;
+        g.last_replayed = true;
+        return;
+    }
+    enqueue(true);
+    g.key = key();
 }
 
 }  // namespace rtcd
@@ -830,7 +901,11 @@ rtcd_status rtcd_check_linear_device(rtcd_ctx* ctx, uint32_t scene_id, const flo
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
         if (n_traj == 0) return RTCD_OK;
         on_main(ctx, stream, [&](Workspace& ws) {
-            enqueue_linear(ctx, ws, *sc, d_traj, d_offsets, n_traj, n_wp_total, d_out, d_out_seg, 0);
+            sweep_main(ctx, ws,
+                       [&]() { return sweep_key(ctx, ws, *sc, 1, d_traj, d_offsets, n_traj, n_wp_total, 0, d_out, d_out_seg); },
+                       [&](bool ev) {
+                           enqueue_linear(ctx, ws, *sc, d_traj, d_offsets, n_traj, n_wp_total, d_out, d_out_seg, 0, ev);
+                       });
         });
         hipStream_t st = stream ? (hipStream_t) stream : ctx->main.stream;
         auto scp = sc;
@@ -867,11 +942,13 @@ rtcd_status rtcd_check_linear(rtcd_ctx* ctx, uint32_t scene_id, const float* tra
         b.h_offsets.upload(offsets, n_traj + 1, s);
         b.h_out.reserve(n_traj);
         b.h_seg.reserve(std::max<uint32_t>(n_seg, 1));
-        auto run = [&]() {
-            enqueue_linear(ctx, b, *sc, b.h_poses.p, b.h_offsets.p, n_traj, nw, b.h_out.p, out_seg ? b.h_seg.p : nullptr,
-                           0);
-        };
-        run();
+        uint32_t* seg = out_seg ? b.h_seg.p : nullptr;
+        auto run = [&]() { enqueue_linear(ctx, b, *sc, b.h_poses.p, b.h_offsets.p, n_traj, nw, b.h_out.p, seg, 0); };
+        sweep_main(ctx, b,
+                   [&]() { return sweep_key(ctx, b, *sc, 1, b.h_poses.p, b.h_offsets.p, n_traj, nw, 0, b.h_out.p, seg); },
+                   [&](bool ev) {
+                       enqueue_linear(ctx, b, *sc, b.h_poses.p, b.h_offsets.p, n_traj, nw, b.h_out.p, seg, 0, ev);
+                   });
         finish_sweep(ctx, b, &ctx->last, run);
         RTCD_HIP(hipMemcpyAsync(out, b.h_out.p, sizeof(uint32_t) * n_traj, hipMemcpyDeviceToHost, s));
         if (out_seg && n_seg)
@@ -894,7 +971,12 @@ rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const 
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
         if (n_traj == 0) return RTCD_OK;
         on_main(ctx, stream, [&](Workspace& ws) {
-            enqueue_quadratic(ctx, ws, *sc, d_wp, n_traj, n_wp, n_ctrl, W, d_out, d_out_seg, 0);
+            prepare_weights(ws, n_ctrl, n_wp, W);
+            sweep_main(ctx, ws,
+                       [&]() { return sweep_key(ctx, ws, *sc, 2, d_wp, nullptr, n_traj, n_wp, n_ctrl, d_out, d_out_seg); },
+                       [&](bool ev) {
+                           enqueue_quadratic(ctx, ws, *sc, d_wp, n_traj, n_wp, n_ctrl, W, d_out, d_out_seg, 0, ev);
+                       });
         });
         hipStream_t st = stream ? (hipStream_t) stream : ctx->main.stream;
         auto scp = sc;
@@ -931,11 +1013,14 @@ rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* 
         b.h_poses.upload(wp, (size_t) n_traj * n_wp * ctx->robot.dof, s);
         b.h_out.reserve(n_traj);
         b.h_seg.reserve(n_traj * npc);
-        auto run = [&]() {
-            enqueue_quadratic(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, out_seg ? b.h_seg.p : nullptr,
-                              0);
-        };
-        run();
+        uint32_t* seg = out_seg ? b.h_seg.p : nullptr;
+        auto run = [&]() { enqueue_quadratic(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, seg, 0); };
+        prepare_weights(b, n_ctrl, n_wp, W);
+        sweep_main(ctx, b,
+                   [&]() { return sweep_key(ctx, b, *sc, 2, b.h_poses.p, nullptr, n_traj, n_wp, n_ctrl, b.h_out.p, seg); },
+                   [&](bool ev) {
+                       enqueue_quadratic(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, seg, 0, ev);
+                   });
         finish_sweep(ctx, b, &ctx->last, run);
         RTCD_HIP(hipMemcpyAsync(out, b.h_out.p, sizeof(uint32_t) * n_traj, hipMemcpyDeviceToHost, s));
         if (out_seg) RTCD_HIP(hipMemcpyAsync(out_seg, b.h_seg.p, sizeof(uint32_t) * n_traj * npc, hipMemcpyDeviceToHost, s));

@@ -90,3 +90,82 @@ def test_device_replays_per_direction(panda, orc_panda, direction):
         assert modes[0] == 0 and modes[-2:] == [1, 1], modes
     finally:
         a.close()
+
+
+def test_sweep_host_replays_match_eager_and_oracle(panda, orc_panda):
+    from rtcollisiondetection_amd.detector import curve_weights
+    a, b = _ctx(panda), _ctx(panda)
+    try:
+        b.set_param('graphs', 0)
+        sc = scenes.scene_shelf()
+        sa, sb = a.create_scene(sc.verts, sc.tris), b.create_scene(sc.verts, sc.tris)
+        m = O.scene_mesh(sc)
+        w, o = samplers.random_walk_trajectories(200, panda.lower, panda.upper, lengths=(4, 24), seed=61)
+        wp, _ = samplers.bspline_trajectories(96, panda.lower, panda.upper, n_ctrl=8, n_wp=16, seed=62)
+        ref_l = O.check_linear(orc_panda, m, w, o)
+        W = samplers.quadratic_fit_weights(8, 16)
+        ref_q = O.check_quadratic(orc_panda, m, wp, W)
+        assert 0 < ref_l.sum() < len(ref_l) and 0 < ref_q.sum() < len(ref_q)
+        other_w, other_o = samplers.random_walk_trajectories(50, panda.lower, panda.upper, lengths=8, seed=63)
+        other_ref = b.check_linear(sb, other_w, other_o)
+        modes = []
+        for r in range(5):
+            got_l, seg_l = a.check_linear(sa, w, o, with_segments=True)
+            modes.append(a.stats()['graph_replayed'])
+            assert np.array_equal(got_l, ref_l) and np.array_equal(seg_l, O.check_linear(orc_panda, m, w, o, with_segments=True)[1]), r
+            assert np.array_equal(b.check_linear(sb, other_w, other_o), other_ref), r  # eager work in between
+        print('linear graph_replayed per call:', modes)
+        assert modes[0] == 0 and modes[-2:] == [1, 1], modes
+        modes = []
+        for r in range(5):
+            got_q = a.check_quadratic(sa, wp, 8, W)
+            modes.append(a.stats()['graph_replayed'])
+            assert np.array_equal(got_q, ref_q), r
+        print('quadratic graph_replayed per call:', modes)
+        assert modes[0] == 0 and modes[-2:] == [1, 1], modes
+        # other weights: eager once, then replayed again; flags follow the weights
+        Wr = curve_weights(2, 'repeated', 8, 16)
+        ref_r = O.check_quadratic(orc_panda, m, wp, Wr)
+        got = [a.check_quadratic(sa, wp, 8, Wr) for _ in range(3)]
+        assert all(np.array_equal(g, ref_r) for g in got)
+        assert a.stats()['graph_replayed'] == 1
+        # new waypoint values in the same buffers: the replay reads them (the quadratic calls above grew
+        # the staging buffer, so the linear shape is captured again first)
+        for _ in range(2):
+            assert np.array_equal(a.check_linear(sa, w, o), ref_l)
+        rng = np.random.default_rng(64)
+        for _ in range(2):
+            w2 = np.clip(w + rng.uniform(-0.1, 0.1, w.shape), panda.lower, panda.upper).astype(np.float32)
+            got2 = a.check_linear(sa, w2, o)
+            assert a.stats()['graph_replayed'] == 1
+            assert np.array_equal(got2, O.check_linear(orc_panda, m, w2, o))
+    finally:
+        a.close()
+        b.close()
+
+
+def test_sweep_device_replays(panda, orc_panda):
+    import torch
+    a = _ctx(panda)
+    try:
+        sc = scenes.scene_dense()
+        sid = a.create_scene(sc.verts, sc.tris)
+        m = O.scene_mesh(sc)
+        wp, _ = samplers.bspline_trajectories(256, panda.lower, panda.upper, n_ctrl=8, n_wp=16, seed=65)
+        ref = O.check_quadratic(orc_panda, m, wp, samplers.quadratic_fit_weights(8, 16))
+        dwp = torch.from_numpy(wp).cuda()
+        dout = torch.zeros(len(wp), dtype=torch.int32, device='cuda')
+        modes = []
+        for r in range(5):
+            dout.fill_(7)
+            from rtcollisiondetection_amd import _lib
+            import ctypes as C
+            _lib.check(_lib.lib().rtcd_check_quadratic_device(a.h, sid, C.c_void_p(dwp.data_ptr()), len(wp), 16, 8, None,
+                                                              C.c_void_p(dout.data_ptr()), None, None))
+            a.sync()
+            modes.append(a.stats()['graph_replayed'])
+            assert np.array_equal(dout.cpu().numpy().astype(bool), ref.astype(bool)), r
+        print('device quadratic graph_replayed per call:', modes)
+        assert modes[0] == 0 and modes[-2:] == [1, 1], modes
+    finally:
+        a.close()
