@@ -195,7 +195,7 @@ struct rtcd_ctx {
     DevBuf<int32_t> r_sphere_link;
     DevBuf<float> sphere_radius;
     // launch-shape parameters (rtcd_set_param)
-    uint32_t sweep_budget = 16;                 // work units per sweep task before it hands off its subtrees
+    uint32_t sweep_budget = 32;                 // work units per sweep task before it hands off its subtrees
     uint64_t frontier_work = (uint64_t) 1 << 23;     // discrete: target (task, frontier entry) threads
     uint64_t frontier_max_tasks = (uint64_t) 1 << 17; // discrete: split batches with fewer tasks
     uint32_t broad_budget = 64;                 // discrete: broad node visits before a walk hands off

@@ -311,105 +311,150 @@ __device__ __forceinline__ bool cull_ok(const Cull& c, float bx0, float by0, flo
 // the FP64 queue.
 template <int DEG, int STACK, bool COUNT>
 __global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, int pass) {
+    constexpr int kRefill = 16;  // idle lanes that make a wave take new items
+    constexpr int kSteps = 4;    // walk steps between refill checks
     __shared__ int stk[STACK * kBS];
+    __shared__ uint32_t cursor;  // next item of the block's range
     int* st = stk + threadIdx.x;
+    if (threadIdx.x == 0) cursor = 0;
+    __syncthreads();
     const int S = a.robot_n_spheres_host;
     const uint2* in_q = pass > 0 ? ((pass - 1) & 1 ? a.spill[1] : a.spill[0]) : nullptr;
     uint2* out_q = (pass & 1 ? a.spill[1] : a.spill[0]);
     uint32_t* out_cnt = a.spill_count + pass;
     const bool may_spill = pass < a.n_passes - 1;
     const uint32_t n_tasks = pass == 0 ? a.n_items : min(a.spill_count[pass - 1], a.spill_cap);
-    const uint32_t stride = gridDim.x * kBS;
+    // block b owns one contiguous range of the items (pass 0: (segment or sub-piece, sphere), sphere
+    // fastest, so that neighbouring lanes sweep one segment's spheres; later passes: the handed-off
+    // subtrees).  Lanes are persistent: a lane whose walk ends takes the block's next item.
+    const uint32_t chunk = (n_tasks + gridDim.x - 1) / gridDim.x;
+    const uint32_t beg = min(blockIdx.x * chunk, n_tasks);
+    const uint32_t span = min(beg + chunk, n_tasks) - beg;
     uint32_t c_nodes = 0, c_prims = 0, c_fp64 = 0, c_items = 0, c_certs = 0;
-    for (uint32_t t = blockIdx.x * kBS + threadIdx.x; t < n_tasks; t += stride) {
-        uint32_t it = t;
-        int ni = 0;
+    // lane state: the item, its cull volume and its walk
+    bool act = false, spill_ok = false;
+    uint32_t it = 0, work = 0;
+    int ni = 0, sp = 0, iter = 0;
+    uint32_t* flag = a.out;
+    Item m{};
+    Cull cu{};
+    auto setup = [&](uint32_t t) -> bool {
+        it = t;
+        ni = 0;
         if (pass > 0) {
             const uint2 r = in_q[t];
-            if (r.x == 0xffffffffu) continue;  // slot of a hand-off that did not fit
+            if (r.x == 0xffffffffu) return false;  // slot of a hand-off that did not fit
             it = r.x;
             ni = (int) r.y;
         }
-        const Item m = make_item<DEG>(a, it, S);
-        uint32_t* flag = a.out_seg ? a.out_seg + m.g : a.out + m.traj;
-        if (load_flag(flag)) continue;
+        m = make_item<DEG>(a, it, S);
+        flag = a.out_seg ? a.out_seg + m.g : a.out + m.traj;
+        if (load_flag(flag)) return false;
         if (COUNT && pass == 0) c_items++;
-        const Cull cu = make_cull<DEG>(m, a.sc.scale);
-        bool spill_ok = may_spill;
-        int hit = 0, sp = 0, iter = 0;
-        uint32_t work = 0;
-        for (;;) {
-            if ((++iter & 7) == 0 && load_flag(flag)) break;
-            if (spill_ok && work > a.budget) {
-                const uint32_t want = (uint32_t) sp + 1u;
-                const uint32_t o = atomicAdd(out_cnt, want);
-                if ((uint64_t) o + want <= a.spill_cap) {
-                    out_q[o] = make_uint2(it, (uint32_t) ni);
-                    for (int j = 0; j < sp; ++j) out_q[o + 1 + j] = make_uint2(it, (uint32_t) st[j * kBS]);
-                    break;
-                }
-                // queue full: mark the reserved slots empty and finish this subtree here
-                for (uint64_t j = o; j < a.spill_cap && j < (uint64_t) o + want; ++j)
-                    out_q[j] = make_uint2(0xffffffffu, 0u);
-                spill_ok = false;
+        cu = make_cull<DEG>(m, a.sc.scale);
+        spill_ok = may_spill;
+        sp = 0;
+        iter = 0;
+        work = 0;
+        return true;
+    };
+    // one node visit; false when the walk has ended (hit, done, handed off or abandoned)
+    auto step = [&]() -> bool {
+        if ((++iter & 7) == 0 && load_flag(flag)) return false;
+        if (spill_ok && work > a.budget) {
+            const uint32_t want = (uint32_t) sp + 1u;
+            const uint32_t o = atomicAdd(out_cnt, want);
+            if ((uint64_t) o + want <= a.spill_cap) {
+                out_q[o] = make_uint2(it, (uint32_t) ni);
+                for (int j = 0; j < sp; ++j) out_q[o + 1 + j] = make_uint2(it, (uint32_t) st[j * kBS]);
+                return false;
             }
-            const BNode nd = a.sc.edge_nodes[ni];
-            ++work;
-            if (COUNT) c_nodes++;
-            bool h0 = nd.c0 >= 0 && cull_ok(cu, nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
-            bool h1 = nd.c1 >= 0 && cull_ok(cu, nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
+            // queue full: mark the reserved slots empty and finish this subtree here
+            for (uint64_t j = o; j < a.spill_cap && j < (uint64_t) o + want; ++j) out_q[j] = make_uint2(0xffffffffu, 0u);
+            spill_ok = false;
+        }
+        const BNode nd = a.sc.edge_nodes[ni];
+        ++work;
+        if (COUNT) c_nodes++;
+        bool h0 = nd.c0 >= 0 && cull_ok(cu, nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
+        bool h1 = nd.c1 >= 0 && cull_ok(cu, nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
+        bool hit = false;
 #pragma unroll
-            for (int side = 0; side < 2 && !hit; ++side) {
-                const bool hb = side ? h1 : h0;
-                const int cnt = side ? nd.n1 : nd.n0;
-                const int start = side ? nd.c1 : nd.c0;
-                if (hb && cnt > 0) {
-                    for (int k = 0; k < cnt && !hit; ++k) {
-                        const Edge e = a.sc.edge_prims[start + k];
-                        if (COUNT) c_prims++;
-                        const float3 A = f3(e.p), B = f3(e.q);
-                        uint32_t nc = 0;
-                        int cls;
-                        if (DEG == 1) {
-                            cls = cap_cert(m.q0, m.q1, A, B, m.r, m.r, m.ein);
-                            nc = 1;
-                        } else {
-                            cls = quad_cert(m.q0, m.q1, m.q2, A, B, m.r, m.ein, nc);
-                        }
-                        work += nc;
-                        if (COUNT) c_certs += nc;
-                        if (cls == kHit) {
-                            hit = 1;
-                        } else if (cls == kUnsure) {
-                            if (COUNT) c_fp64++;
-                            const uint32_t slot = atomicAdd(a.unsure_count, 1u);
-                            if (slot < a.unsure_cap) a.unsure[slot] = make_uint4(m.g, (uint32_t) m.s, (uint32_t) (start + k), 0u);
-                            else atomicOr(a.overflow, 4u);
-                        }
+        for (int side = 0; side < 2 && !hit; ++side) {
+            const bool hb = side ? h1 : h0;
+            const int cnt = side ? nd.n1 : nd.n0;
+            const int first = side ? nd.c1 : nd.c0;
+            if (hb && cnt > 0) {
+                for (int k = 0; k < cnt && !hit; ++k) {
+                    const Edge e = a.sc.edge_prims[first + k];
+                    if (COUNT) c_prims++;
+                    const float3 A = f3(e.p), B = f3(e.q);
+                    uint32_t nc = 0;
+                    int cls;
+                    if (DEG == 1) {
+                        cls = cap_cert(m.q0, m.q1, A, B, m.r, m.r, m.ein);
+                        nc = 1;
+                    } else {
+                        cls = quad_cert(m.q0, m.q1, m.q2, A, B, m.r, m.ein, nc);
                     }
-                    if (side) h1 = false; else h0 = false;
+                    work += nc;
+                    if (COUNT) c_certs += nc;
+                    if (cls == kHit) {
+                        hit = true;
+                    } else if (cls == kUnsure) {
+                        if (COUNT) c_fp64++;
+                        const uint32_t slot = atomicAdd(a.unsure_count, 1u);
+                        if (slot < a.unsure_cap) a.unsure[slot] = make_uint4(m.g, (uint32_t) m.s, (uint32_t) (first + k), 0u);
+                        else atomicOr(a.overflow, 4u);
+                    }
                 }
-            }
-            if (hit) break;
-            if (h0 && h1) {
-                if (sp >= STACK) { atomicOr(a.overflow, 2u); break; }
-                st[sp * kBS] = nd.c1;
-                ++sp;
-                ni = nd.c0;
-            } else if (h0) {
-                ni = nd.c0;
-            } else if (h1) {
-                ni = nd.c1;
-            } else {
-                if (sp == 0) break;
-                --sp;
-                ni = st[sp * kBS];
+                if (side) h1 = false; else h0 = false;
             }
         }
         if (hit) {
             atomicOr(a.out + m.traj, 1u);
             if (a.out_seg) atomicOr(a.out_seg + m.g, 1u);
+            return false;
         }
+        if (h0 && h1) {
+            if (sp >= STACK) {
+                atomicOr(a.overflow, 2u);
+                return false;
+            }
+            st[sp * kBS] = nd.c1;
+            ++sp;
+            ni = nd.c0;
+        } else if (h0) {
+            ni = nd.c0;
+        } else if (h1) {
+            ni = nd.c1;
+        } else {
+            if (sp == 0) return false;
+            --sp;
+            ni = st[sp * kBS];
+        }
+        return true;
+    };
+    const int lane = (int) (threadIdx.x & 63);
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (;;) {
+        uint64_t idle = __ballot(!act);
+        if (__popcll(idle) >= kRefill) {
+            bool exhausted;
+            do {
+                const int n_idle = __popcll(idle);
+                const int leader = __ffsll((unsigned long long) idle) - 1;
+                uint32_t b0 = 0;
+                if (lane == leader) b0 = atomicAdd(&cursor, (uint32_t) n_idle);
+                b0 = (uint32_t) __shfl((int) b0, leader, 64);
+                exhausted = (uint64_t) b0 + (uint64_t) n_idle >= span;
+                const uint32_t r = (uint32_t) __popcll(idle & below);
+                if (!act && (uint64_t) b0 + r < span) act = setup(beg + b0 + r);
+                idle = __ballot(!act);
+            } while (!exhausted && __popcll(idle) >= kRefill);
+            if (exhausted && idle == ~0ull) break;
+        }
+        for (int s2 = 0; s2 < kSteps && act; ++s2) act = step();
     }
     if (COUNT) {
         atomicAdd(&a.stats->node_fetches, (unsigned long long) c_nodes);
