@@ -71,6 +71,13 @@ def serve_bench(ctx, rb, scene_ids, rate=2000.0, duration=4.0, slots=4, lane_cus
         st = srv.stats()
     finally:
         srv.stop()
+    # the same requests submitted as fast as possible: throughput of the loop when the device, not the
+    # arrival process, is the limit (compared with running them one by one, below)
+    srv = Server(ctx, slots=slots)
+    try:
+        _, sat_wall = run_stream(srv, reqs, realtime=False)
+    finally:
+        srv.stop()
     pct = lambda a, q: round(float(np.percentile(a, q)), 3) if len(a) else None
     lat = np.array([x[2] for x in res])
     d1k = [x for x in res if x[0] == _lib.REQ_DISCRETE and x[1] <= 1024]
@@ -91,7 +98,9 @@ def serve_bench(ctx, rb, scene_ids, rate=2000.0, duration=4.0, slots=4, lane_cus
             'p99_ms_discrete_512_1024': pct(band, 99), 'n_discrete_512_1024': int(len(band)),
             'served_items_per_s': round(items / wall, 1), 'wall_s': round(wall, 3),
             'plain_per_request_items_per_s': round(items / plain_wall, 1),
-            'served_over_plain': round(plain_wall / wall, 2), 'mismatches_alone_vs_mix': mism,
+            'served_over_plain': round(plain_wall / wall, 2),
+            'saturated_items_per_s': round(items / sat_wall, 1), 'saturated_over_plain': round(plain_wall / sat_wall, 2),
+            'mismatches_alone_vs_mix': mism,
             'queued_max': int(st['queued_max'])}
 
 

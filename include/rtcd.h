@@ -40,7 +40,7 @@ typedef struct {
     int32_t device;          /* HIP device index */
     uint32_t skip_base_link; /* 1: link 0 (the fixed base) is not checked in discrete modes
                                 (RTCD/Robot/robot.h:17,113-115 zeroes its OBB); default 1 */
-    uint32_t leaf_size;      /* BVH leaf size (1..16) of every BVH; 0 = per-BVH defaults (robot links 4,
+    uint32_t leaf_size;      /* BVH leaf size (1..16) of every BVH; 0 = per-BVH defaults (robot links 2,
                                 scene triangles 8, scene edges 2) */
     uint64_t mem_budget;     /* bytes of device memory the context may hold for work buffers; 0 = no limit */
 } rtcd_config;

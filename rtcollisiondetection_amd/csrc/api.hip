@@ -385,9 +385,9 @@ rtcd_status rtcd_create(const rtcd_config* cfg, rtcd_ctx** out) {
         if (c.leaf_size) {
             ctx->leaf_robot = ctx->leaf_scene_tri = ctx->leaf_scene_edge = c.leaf_size;
         } else {
-            // measured on the dense scene: broad phase and narrow passes favour triangle leaves of 8
-            // (robot 4), the sweeps edge leaves of 2
-            ctx->leaf_robot = 4;
+            // measured on the dense scenes: the broad phase favours scene-triangle leaves of 8, the
+            // narrow passes robot leaves of 2, the sweeps scene-edge leaves of 2
+            ctx->leaf_robot = 2;
             ctx->leaf_scene_tri = 8;
             ctx->leaf_scene_edge = 2;
         }

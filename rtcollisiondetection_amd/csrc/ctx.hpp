@@ -198,7 +198,7 @@ struct rtcd_ctx {
     uint32_t sweep_budget = 32;                 // work units per sweep task before it hands off its subtrees
     uint64_t frontier_work = (uint64_t) 1 << 23;     // discrete: target (task, frontier entry) threads
     uint64_t frontier_max_tasks = (uint64_t) 1 << 17; // discrete: split batches with fewer tasks
-    uint32_t broad_budget = 64;                 // discrete: broad node visits before a walk hands off
+    uint32_t broad_budget = 16;                 // discrete: broad node visits before a walk hands off
     uint32_t broad_budget_small = 16;           // the same for batches below frontier_max_tasks tasks
     uint32_t broad_budget_next = 16;            // the same in the passes after the first
     uint32_t broad_passes = 12;                 // broad hand-off passes (the last never hands off)
@@ -206,7 +206,7 @@ struct rtcd_ctx {
     uint32_t narrow_bs = 64;                    // work-group size of the discrete narrow passes
     // BVH leaf sizes (rtcd_config.leaf_size, rtcd_set_leaf_sizes): robot link BVHs, scene triangle
     // BVH (also the threads per broad-phase leaf record), scene edge BVH (sweeps)
-    uint32_t leaf_robot = 4, leaf_scene_tri = 8, leaf_scene_edge = 2;
+    uint32_t leaf_robot = 2, leaf_scene_tri = 8, leaf_scene_edge = 2;
     uint32_t narrow_budget = 16;                // robot->obstacle: link-BVH nodes before a walk hands off
     uint32_t narrow_budget_next = 8;            // the same in later passes
     uint32_t narrow_passes = 6;                 // robot->obstacle passes (the last never hands off)
