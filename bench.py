@@ -343,6 +343,29 @@ def main():
             ctx.check_quadratic(sid, wp, 8)
         extra['dense_quadratic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
         log(f'[bench] extra sweeps { {k: v for k, v in extra.items() if "traj" in k or "ms_device" in k} }')
+        # the plain obstacle->robot case: 1,024 uniform poses (seed 0) against the 8 boxes (96 triangles),
+        # one batched call versus one blocking call per pose
+        c1s = scenes.scene_boxes8()
+        c1id = ctx.create_scene(c1s.verts, c1s.tris)
+        c1q = samplers.uniform_poses(1024, rb.lower, rb.upper, seed=0)
+        c1_flags = ctx.check_discrete(c1id, c1q, 1)
+        t = time.perf_counter()
+        for _ in range(5):
+            ctx.check_discrete(c1id, c1q, 1)
+        c1_batched = 1024 * 5 / (time.perf_counter() - t)
+        ctx.set_param('graphs', 0)
+        ctx.check_discrete(c1id, c1q[:1], 1)
+        t = time.perf_counter()
+        for i in range(256):
+            ctx.check_discrete(c1id, c1q[i:i + 1], 1)
+        c1_plain = 256 / (time.perf_counter() - t)
+        ctx.set_param('graphs', 1)
+        extra['c1_boxes8_o2r'] = {'poses': 1024, 'scene_tris': int(len(c1s.tris)),
+                                  'collisions': int(np.count_nonzero(c1_flags)),
+                                  'batched_poses_per_s': round(c1_batched, 1),
+                                  'plain_per_query_poses_per_s': round(c1_plain, 1),
+                                  'batched_over_plain': round(c1_batched / c1_plain, 1)}
+        log(f'[bench] boxes8 obstacle->robot {extra["c1_boxes8_o2r"]}')
         extra['reference_scenes'] = reference_scene_bench(ctx, rb)
         log(f'[bench] reference scenes {extra["reference_scenes"]}')
         extra['serving'] = serve_bench(ctx, rb, [ctx.create_scene(*_boxes8()), sid2, sid])
@@ -365,6 +388,17 @@ def main():
         cpu = {'value': round(sample / cdt, 1), 'unit': 'poses/s', 'cores': cores, 'kind': 'port',
                'sample': f'first {sample} of the 65,536 poses, dense scene, two-way, FP64 oracle with BVHs',
                'mismatches_vs_gpu': mism}
+        if 'c1_boxes8_o2r' in extra:
+            # the plain obstacle->robot case on the CPU too (the whole 1,024 poses, one thread per core)
+            c1s = scenes.scene_boxes8()
+            c1q = samplers.uniform_poses(1024, rb.lower, rb.upper, seed=0)
+            t = time.perf_counter()
+            c1_ref = O.check_discrete(orb, O.scene_mesh(c1s), c1q, 1, nthreads=cores)
+            c1_dt = time.perf_counter() - t
+            c1_sid = ctx.create_scene(c1s.verts, c1s.tris)
+            c1_got = ctx.check_discrete(c1_sid, c1q, 1)
+            cpu['c1_boxes8_o2r'] = {'poses_per_s': round(1024 / c1_dt, 1), 'cores': cores,
+                                    'mismatches_vs_gpu': int(np.sum(c1_ref.astype(bool) != c1_got.astype(bool)))}
         log(f'[bench] cpu baseline {cpu}')
 
     out = {
