@@ -121,15 +121,15 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
 
     ws.seg_count.reserve((size_t) kCandSegments * kSegStride);
     ws.seg_pre.reserve(kCandSegments + 1);
-    fill_u32(d_out, n, 0u, s);
-    fill_u32(ws.scalars.p, kScalars, 0u, s);
-    fill_u32(ws.seg_count.p, (size_t) kCandSegments * kSegStride, 0u, s);
+    ws.pin_status.reserve(kPublish);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, s);
     auto mark = [&](int k) {
         if (events) RTCD_HIP(hipEventRecord(ws.ev[k], s));
     };
     mark(0);
-    launch_fk(c->d_robot.p, c->robot.dof, NL, d_poses, (int) n, ws.T64.p, ws.Ti.p, ws.wbox.p, s);
+    // the pose kernel also clears the flags, the status words and the segment counters
+    launch_fk(c->d_robot.p, c->robot.dof, NL, d_poses, (int) n, ws.T64.p, ws.Ti.p, ws.wbox.p,
+              FkClear{d_out, (int) n, ws.scalars.p, ws.seg_count.p}, s);
     DiscreteArgs a{};
     a.robot = c->d_robot.p;
     a.rb = RobotBvh{c->r_tri_nodes.p, c->r_tris.p, c->r_edge_nodes.p, c->r_edges.p};
@@ -187,8 +187,9 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     // pass then skips
     if (dir & 1) launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
     mark(4);
-    if (dir == 3) {
-        // the broad records are no longer needed: the compacted list reuses their buffer
+    if (dir == 3 && !tiny) {
+        // the broad records are no longer needed: the compacted list reuses their buffer (a tiny
+        // batch skips the launch: its robot->obstacle lanes pass over the records of flagged poses)
         a.r2o_list = ws.cand.p;
         a.r2o_count = ws.scalars.p + 3;
         launch_compact_unflagged(a, s);
@@ -199,6 +200,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.narrow_passes = tiny ? 1 : (int) c->narrow_passes;
     if (dir & 2) launch_narrow(a, 2, c->robot_depth + 1, ws.counting, s);
     launch_redecide(a, s);
+    launch_publish(ws.scalars.p, ws.pin_status.d, d_out, ws.pub_flags, n, s);
     mark(5);
     ws.timing_pending = events;
     ws.stages_recorded = events;
@@ -226,7 +228,7 @@ static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, c
             P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.spill.p), ws.spill.n, P(ws.nspill.p), ws.nspill.n,
             P(ws.unsure.p), ws.unsure.n, c->leaf_scene_tri, c->narrow_budget, c->narrow_budget_next, c->narrow_passes, c->o2r_budget,
             c->o2r_budget_next, c->o2r_passes,
-            P(ws.scalars.p), P(ws.stats.p), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
+            P(ws.scalars.p), P(ws.stats.p), P(ws.pin_status.d), P(ws.pub_flags), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
             c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth, c->narrow_bs,
             c->tiny_batch};
 }
@@ -282,7 +284,9 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                      uint32_t* d_out, size_t budget, rtcd_stats* st) {
     const hipStream_t s = ws.stream;
     for (int attempt = 0; attempt < 4; ++attempt) {
-        std::vector<uint32_t> sv = ws.scalars.download(6, s);
+        // the last kernel of the call published the status words into pinned host memory
+        RTCD_HIP(hipStreamSynchronize(s));
+        std::vector<uint32_t> sv(ws.pin_status.h, ws.pin_status.h + 6);
         sv.push_back(0);
         if ((sv[1] & 1u) || ws.counting) {
             // total leaf candidates and the fullest segment (sizes the re-run on overflow)
@@ -788,11 +792,29 @@ rtcd_status rtcd_check_discrete(rtcd_ctx* ctx, uint32_t scene_id, int32_t direct
         Workspace& ws = ctx->main;
         ws.counting = ctx->counting;
         const hipStream_t s = ws.stream;
-        ws.h_poses.reserve((size_t) n * ctx->robot.dof);
+        const size_t nq = (size_t) n * ctx->robot.dof;
         ws.h_out.reserve(n);
-        RTCD_HIP(hipMemcpyAsync(ws.h_poses.p, poses, sizeof(float) * n * ctx->robot.dof, hipMemcpyHostToDevice, s));
+        if (n <= kPinnedBatch) {
+            // small batch: the pose kernel reads the poses from pinned memory and the publish kernel
+            // writes the flags back there, so the call makes no copy and synchronises once
+            ws.pin_poses.reserve(nq);
+            ws.pin_flags.reserve(n);
+            std::memcpy(ws.pin_poses.h, poses, sizeof(float) * nq);
+            ws.pub_flags = ws.pin_flags.d;
+            try {
+                enqueue_discrete_main(ctx, *sc, direction, ws.pin_poses.d, n, ws.h_out.p);
+                finish_discrete(ctx, ws, *sc, direction, ws.pin_poses.d, n, ws.h_out.p, 0, &ctx->last);
+            } catch (...) {
+                ws.pub_flags = nullptr;
+                throw;
+            }
+            ws.pub_flags = nullptr;
+            std::memcpy(out, ws.pin_flags.h, sizeof(uint32_t) * n);
+            return RTCD_OK;
+        }
+        ws.h_poses.reserve(nq);
+        RTCD_HIP(hipMemcpyAsync(ws.h_poses.p, poses, sizeof(float) * nq, hipMemcpyHostToDevice, s));
         enqueue_discrete_main(ctx, *sc, direction, ws.h_poses.p, n, ws.h_out.p);
-        RTCD_HIP(hipStreamSynchronize(s));
         finish_discrete(ctx, ws, *sc, direction, ws.h_poses.p, n, ws.h_out.p, 0, &ctx->last);
         RTCD_HIP(hipMemcpyAsync(out, ws.h_out.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
         RTCD_HIP(hipStreamSynchronize(s));

@@ -82,6 +82,10 @@ namespace rtcd {
 
 // Every buffer one in-flight check needs, bound to one stream.  The blocking and *_device entry
 // points use the context's main workspace; the serving loop owns one workspace per slot.
+// Host-buffer discrete batches up to this many poses read their poses from, and return their flags
+// through, pinned host memory (no copies; about 28 KB of poses at most).
+constexpr uint32_t kPinnedBatch = 1024;
+
 struct Workspace {
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -104,6 +108,12 @@ struct Workspace {
     // host staging
     DevBuf<float> h_poses;
     DevBuf<uint32_t> h_offsets, h_out, h_seg;
+    // pinned host memory the kernels use directly: status words published at the end of each discrete
+    // call, and the poses / flags of small host-buffer batches (pub_flags: where the publish kernel
+    // copies the flags, or null)
+    HostMapped<uint32_t> pin_status, pin_flags;
+    HostMapped<float> pin_poses;
+    uint32_t* pub_flags = nullptr;
     // sweeps
     DevBuf<double> c64;       // sphere centres [W][S][3]
     DevBuf<float4> c32;       // [W][S] (x, y, z, r)

@@ -53,6 +53,41 @@ struct DevBuf {
     size_t bytes() const { return n * sizeof(T); }
 };
 
+// Pinned, coherent host memory that kernels read and write directly (h on the host, d on the
+// device).  Small batches go through it: the pose kernel reads the caller's poses from it and the
+// last kernel writes the status words (and small flag arrays) into it, so that a call needs no copy
+// and one stream synchronisation.  Grows, never shrinks; contents are not preserved.
+template <typename T>
+struct HostMapped {
+    T* h = nullptr;
+    T* d = nullptr;
+    size_t n = 0;
+    HostMapped() = default;
+    HostMapped(const HostMapped&) = delete;
+    HostMapped& operator=(const HostMapped&) = delete;
+    ~HostMapped() { release(); }
+    void reserve(size_t count) {
+        if (count <= n) return;
+        release();
+        const size_t c = count + count / 4 + 16;
+        RTCD_HIP(hipHostMalloc((void**) &h, c * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent));
+        void* dp = nullptr;
+        const hipError_t e = hipHostGetDevicePointer(&dp, h, 0);
+        if (e != hipSuccess) {
+            (void) hipHostFree(h);
+            h = nullptr;
+            throw HipError(e, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+        }
+        d = (T*) dp;
+        n = c;
+    }
+    void release() {
+        if (h) (void) hipHostFree(h);
+        h = d = nullptr;
+        n = 0;
+    }
+};
+
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned) ((n + bs - 1) / bs); }
 
 // ---- scan.hip ----

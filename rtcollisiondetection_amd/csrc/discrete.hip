@@ -59,10 +59,34 @@ __device__ __forceinline__ LinkTri link_tri(const Tri& t, const float4& r0, cons
 }
 
 // ------------------------------------------------------------------------------------------ FK
+// Row r of the world AABB of link box L under the link transform T (rounded outwards).
+__device__ __forceinline__ void link_box_row(const double* T, const LinkInfo& L, int r, float& lo, float& hi) {
+    const double c0 = L.c[0], c1 = L.c[1], c2 = L.c[2];
+    const double h0 = 0.5 * ((double) L.hi[0] - L.lo[0]), h1 = 0.5 * ((double) L.hi[1] - L.lo[1]),
+                 h2 = 0.5 * ((double) L.hi[2] - L.lo[2]);
+    const double wc = T[r * 4 + 0] * c0 + T[r * 4 + 1] * c1 + T[r * 4 + 2] * c2 + T[r * 4 + 3];
+    const double wh = fabs(T[r * 4 + 0]) * h0 + fabs(T[r * 4 + 1]) * h1 + fabs(T[r * 4 + 2]) * h2;
+    const double pad = 1e-6 * (1.0 + fabs(wc) + wh);
+    lo = (float) (wc - wh - pad);
+    hi = (float) (wc + wh + pad);
+}
+
+// The per-call clears the pose kernel does on the side (it runs first, so nothing reads these
+// before it ends): the caller's flags, the status words and the candidate-segment counters.
+__device__ __forceinline__ void fk_clear(const FkClear& z, int p, bool pose_lane) {
+    if (pose_lane && p < z.n) z.flags[p] = 0u;
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < kScalars; i += kBS) z.scalars[i] = 0u;
+        for (int b = threadIdx.x; b < kSeg; b += kBS) z.seg_count[(size_t) b * kSegStride] = 0u;
+    }
+}
+
+// One thread per pose: T64, Ti and wbox of every link.
 __global__ __launch_bounds__(kBS) void k_fk(const RobotDev* __restrict__ R, const float* __restrict__ poses, int n,
                                            double* __restrict__ T64, float4* __restrict__ Ti,
-                                           float4* __restrict__ wbox) {
-    int p = blockIdx.x * kBS + threadIdx.x;
+                                           float4* __restrict__ wbox, FkClear z) {
+    const int p = blockIdx.x * kBS + threadIdx.x;
+    fk_clear(z, p, true);
     if (p >= n) return;
     const int dof = R->dof, NL = dof + 1;
     double T[12];
@@ -87,21 +111,66 @@ __global__ __launch_bounds__(kBS) void k_fk(const RobotDev* __restrict__ R, cons
         for (int j = 0; j < 3; ++j)
             Ti[pl * 3 + j] = make_float4((float) T[0 * 4 + j], (float) T[1 * 4 + j], (float) T[2 * 4 + j],
                                          (float) T[j * 4 + 3]);
-        // world AABB of the link box, rounded outwards
-        const LinkInfo& L = R->link[l];
-        double c[3] = {L.c[0], L.c[1], L.c[2]};
-        double h[3] = {0.5 * ((double) L.hi[0] - L.lo[0]), 0.5 * ((double) L.hi[1] - L.lo[1]),
-                       0.5 * ((double) L.hi[2] - L.lo[2])};
         float lo[3], hi[3];
-        for (int r = 0; r < 3; ++r) {
-            double wc = T[r * 4 + 0] * c[0] + T[r * 4 + 1] * c[1] + T[r * 4 + 2] * c[2] + T[r * 4 + 3];
-            double wh = fabs(T[r * 4 + 0]) * h[0] + fabs(T[r * 4 + 1]) * h[1] + fabs(T[r * 4 + 2]) * h[2];
-            double pad = 1e-6 * (1.0 + fabs(wc) + wh);
-            lo[r] = (float) (wc - wh - pad);
-            hi[r] = (float) (wc + wh + pad);
-        }
+        for (int r = 0; r < 3; ++r) link_box_row(T, R->link[l], r, lo[r], hi[r]);
         wbox[pl * 2] = make_float4(lo[0], lo[1], lo[2], 0.f);
         wbox[pl * 2 + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
+    }
+}
+
+// One wave per pose, for small batches, where the serial FP64 chain of a single thread is the whole
+// latency: lane i < dof evaluates the sine and cosine of joint i, lane k < 12 owns entry k of the
+// running transform (the products of the chain), lanes 0..2 own the rows of Ti and of the world box.
+// Every value comes from the same expressions in the same order as in k_fk, so both kernels write
+// identical bits.
+__global__ __launch_bounds__(kBS) void k_fk_wave(const RobotDev* __restrict__ R, const float* __restrict__ poses,
+                                                int n, double* __restrict__ T64, float4* __restrict__ Ti,
+                                                float4* __restrict__ wbox, FkClear z) {
+    const int lane = threadIdx.x & 63;
+    const int p = blockIdx.x * (kBS / 64) + (threadIdx.x >> 6);  // uniform per wave
+    fk_clear(z, p, lane == 0);
+    if (p >= n) return;
+    const int dof = R->dof, NL = dof + 1;
+    double sj = 0.0, cj = 0.0;
+    if (lane < dof) x64::sincos((double) poses[(size_t) p * dof + lane], &sj, &cj);
+    const int k = lane < 12 ? lane : 0;  // lanes 12.. shadow entry 0
+    const int r = k >> 2, cc = k & 3;
+    double T = (double) R->base[k];
+    for (int l = 0; l < NL; ++l) {
+        if (l > 0) {
+            const int i = l - 1;
+            const double s = __shfl(sj, i, 64), c = __shfl(cj, i, 64);
+            const double a0 = __shfl(T, r * 4 + 0, 64), a1 = __shfl(T, r * 4 + 1, 64), a2 = __shfl(T, r * 4 + 2, 64),
+                         a3 = __shfl(T, r * 4 + 3, 64);
+            {
+#pragma clang fp contract(off)
+                // column cc of the joint transform (rows 0..2), then entry (r, cc) of T * L
+                const int m = i * 12 + cc;
+                const double L0 = (c * (double) R->cmask[m] + s * (double) R->smask[m]) + (double) R->omask[m];
+                const double L1 =
+                    (c * (double) R->cmask[m + 4] + s * (double) R->smask[m + 4]) + (double) R->omask[m + 4];
+                const double L2 =
+                    (c * (double) R->cmask[m + 8] + s * (double) R->smask[m + 8]) + (double) R->omask[m + 8];
+                double v = (a0 * L0 + a1 * L1) + a2 * L2;
+                if (cc == 3) v = v + a3;
+                T = v;
+            }
+        }
+        double Tf[12];
+        for (int j = 0; j < 12; ++j) Tf[j] = __shfl(T, j, 64);
+        const size_t pl = (size_t) p * NL + l;
+        if (lane < 12) T64[pl * 12 + lane] = T;
+        float lo = 0.f, hi = 0.f;
+        if (lane < 3) {
+            Ti[pl * 3 + lane] = make_float4((float) Tf[0 * 4 + lane], (float) Tf[1 * 4 + lane], (float) Tf[2 * 4 + lane],
+                                            (float) Tf[lane * 4 + 3]);
+            link_box_row(Tf, R->link[l], lane, lo, hi);
+        }
+        const float lo1 = __shfl(lo, 1, 64), lo2 = __shfl(lo, 2, 64), hi1 = __shfl(hi, 1, 64), hi2 = __shfl(hi, 2, 64);
+        if (lane == 0) {
+            wbox[pl * 2] = make_float4(lo, lo1, lo2, 0.f);
+            wbox[pl * 2 + 1] = make_float4(hi, hi1, hi2, 0.f);
+        }
     }
 }
 
@@ -266,25 +335,28 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
 __global__ __launch_bounds__(kBS) void k_seg_scan(const uint32_t* __restrict__ seg_count, uint32_t seg_cap,
                                                  uint32_t* __restrict__ seg_pre) {
     constexpr int per = kSeg / kBS;
-    __shared__ uint32_t part[kBS];
+    __shared__ uint32_t part[kBS / 64];
     uint32_t loc[per], acc = 0;
     for (int k = 0; k < per; ++k) {
         loc[k] = min(seg_count[(size_t) (threadIdx.x * per + k) * kSegStride], seg_cap);
         acc += loc[k];
     }
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t run = 0;
-        for (int k = 0; k < kBS; ++k) {
-            const uint32_t x = part[k];
-            part[k] = run;
-            run += x;
-        }
-        seg_pre[kSeg] = run;
+    // inclusive scan of the per-thread sums within each wave, then across the waves
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = acc;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
     }
+    if (lane == 63) part[wave] = inc;
     __syncthreads();
-    uint32_t run = part[threadIdx.x];
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < kBS / 64; ++w) {
+        before += w < wave ? part[w] : 0u;
+        all += part[w];
+    }
+    if (threadIdx.x == 0) seg_pre[kSeg] = all;
+    uint32_t run = before + inc - acc;
     for (int k = 0; k < per; ++k) {
         seg_pre[threadIdx.x * per + k] = run;
         run += loc[k];
@@ -840,6 +912,14 @@ __global__ __launch_bounds__(kBS) void k_redecide(DiscreteArgs a) {
     }
 }
 
+// Status words and small flag arrays into pinned host memory (see launch_publish).
+__global__ __launch_bounds__(kBS) void k_publish(const uint32_t* __restrict__ scalars, uint32_t* status_out,
+                                                const uint32_t* __restrict__ flags, uint32_t* flags_out, uint32_t n) {
+    if (flags_out)
+        for (uint32_t i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) flags_out[i] = flags[i];
+    if (blockIdx.x == 0 && threadIdx.x < kPublish) status_out[threadIdx.x] = scalars[threadIdx.x];
+}
+
 int device_cus() {
     static int cus = 0;
     if (!cus) {
@@ -853,9 +933,15 @@ int device_cus() {
 }  // namespace
 
 void launch_fk(const RobotDev* robot, int, int, const float* poses, int n, double* T64, float4* Ti, float4* wbox,
-               hipStream_t s) {
+               const FkClear& z, hipStream_t s) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_fk, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, robot, poses, n, T64, Ti, wbox);
+    // a wave per pose while the batch leaves most of the device idle (one thread per pose would run
+    // the whole FP64 chain serially), a thread per pose beyond
+    if ((size_t) n * 64 <= (size_t) device_cus() * kBS * 4)
+        hipLaunchKernelGGL(k_fk_wave, dim3(blocks_for((size_t) n * 64, kBS)), dim3(kBS), 0, s, robot, poses, n, T64, Ti,
+                           wbox, z);
+    else
+        hipLaunchKernelGGL(k_fk, dim3(blocks_for(n, kBS)), dim3(kBS), 0, s, robot, poses, n, T64, Ti, wbox, z);
     RTCD_LAUNCH_CHECK();
 }
 
@@ -928,6 +1014,13 @@ void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s) {
 
 void launch_redecide(const DiscreteArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_redecide, dim3((unsigned) device_cus() * 2u), dim3(kBS), 0, s, a);
+    RTCD_LAUNCH_CHECK();
+}
+
+void launch_publish(const uint32_t* scalars, uint32_t* status_out, const uint32_t* flags, uint32_t* flags_out,
+                    uint32_t n, hipStream_t s) {
+    const unsigned nb = flags_out ? std::max(1u, std::min(blocks_for(n, kBS), 64u)) : 1u;
+    hipLaunchKernelGGL(k_publish, dim3(nb), dim3(kBS), 0, s, scalars, status_out, flags, flags_out, n);
     RTCD_LAUNCH_CHECK();
 }
 

@@ -79,9 +79,17 @@ struct DiscreteArgs {
 // subtrees handed off by the previous pass.
 constexpr int kMaxPasses = 16;
 
+// Buffers the pose kernel clears for the rest of a discrete call: flags [n], the status words
+// [kScalars] and the candidate-segment counters (word 0 of each kSegStride group).
+struct FkClear {
+    uint32_t* flags;
+    int n;
+    uint32_t* scalars;
+    uint32_t* seg_count;
+};
 // FK for n poses: T64, Ti, wbox for every link (exact FP64 sequence shared with the oracle).
 void launch_fk(const RobotDev* robot, int dof, int n_links, const float* poses, int n, double* T64, float4* Ti,
-               float4* wbox, hipStream_t s);
+               float4* wbox, const FkClear& z, hipStream_t s);
 // Broad phase: one pass appending leaf candidates (wave-aggregated atomics).
 void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
 // Gathers the candidate segments, in order, into one flat array (a.flat, a.seg_pre).
@@ -93,6 +101,11 @@ void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStr
 void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s);
 // FP64 re-decision of the queued tests.
 void launch_redecide(const DiscreteArgs& a, hipStream_t s);
+// Copies the first kPublish status words, and flags[0, n) when flags_out is set, into pinned host
+// memory (device addresses of HostMapped buffers).
+constexpr int kPublish = 8;
+void launch_publish(const uint32_t* scalars, uint32_t* status_out, const uint32_t* flags, uint32_t* flags_out,
+                    uint32_t n, hipStream_t s);
 
 // ---- swept volumes ----
 struct SweepArgs {
