@@ -25,7 +25,10 @@ void budget_check(const rtcd_ctx* c, const Workspace& ws, size_t budget, size_t 
 namespace rtcd {
 
 // Frontier levels of a BVH: level 0 = the root's children, each next level expands every inner
-// entry into its two children (leaves carry over), up to 1024 entries.
+// entry into its two children (leaves carry over), up to kFrontierMax entries.  Small batches start
+// their walks deep (down to the all-leaves level, where the broad phase is one flat box test per
+// leaf and link instead of a chain of dependent node fetches); large ones near the root.
+constexpr size_t kFrontierMax = (size_t) 1 << 16;
 static void build_frontiers(const std::vector<BNode>& nodes, std::vector<FEntry>& all,
                             std::vector<std::pair<int, int>>& levels) {
     all.clear();
@@ -60,7 +63,7 @@ static void build_frontiers(const std::vector<BNode>& nodes, std::vector<FEntry>
                 if (c.c >= 0) nxt.push_back(c);
             }
         }
-        if (!inner || nxt.size() > 1024) break;
+        if (!inner || nxt.size() > kFrontierMax) break;
         cur.swap(nxt);
     }
 }
@@ -199,8 +202,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.narrow_budget_next = c->narrow_budget_next;
     a.narrow_passes = tiny ? 1 : (int) c->narrow_passes;
     if (dir & 2) launch_narrow(a, 2, c->robot_depth + 1, ws.counting, s);
-    launch_redecide(a, s);
-    launch_publish(ws.scalars.p, ws.pin_status.d, d_out, ws.pub_flags, n, s);
+    launch_redecide(a, tiny, ws.scalars.p, ws.pin_status.d, ws.pub_flags, s);
     mark(5);
     ws.timing_pending = events;
     ws.stages_recorded = events;

@@ -126,11 +126,19 @@ __global__ __launch_bounds__(kBS) void k_fk(const RobotDev* __restrict__ R, cons
 __global__ __launch_bounds__(kBS) void k_fk_wave(const RobotDev* __restrict__ R, const float* __restrict__ poses,
                                                 int n, double* __restrict__ T64, float4* __restrict__ Ti,
                                                 float4* __restrict__ wbox, FkClear z) {
+    // the joint masks in LDS: the chain then waits on LDS, not on a global load per joint
+    __shared__ float cm[12 * kMaxDof], sm[12 * kMaxDof], om[12 * kMaxDof];
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * (kBS / 64) + (threadIdx.x >> 6);  // uniform per wave
     fk_clear(z, p, lane == 0);
-    if (p >= n) return;
     const int dof = R->dof, NL = dof + 1;
+    for (int i = threadIdx.x; i < 12 * dof; i += kBS) {
+        cm[i] = R->cmask[i];
+        sm[i] = R->smask[i];
+        om[i] = R->omask[i];
+    }
+    __syncthreads();
+    if (p >= n) return;
     double sj = 0.0, cj = 0.0;
     if (lane < dof) x64::sincos((double) poses[(size_t) p * dof + lane], &sj, &cj);
     const int k = lane < 12 ? lane : 0;  // lanes 12.. shadow entry 0
@@ -146,11 +154,9 @@ __global__ __launch_bounds__(kBS) void k_fk_wave(const RobotDev* __restrict__ R,
 #pragma clang fp contract(off)
                 // column cc of the joint transform (rows 0..2), then entry (r, cc) of T * L
                 const int m = i * 12 + cc;
-                const double L0 = (c * (double) R->cmask[m] + s * (double) R->smask[m]) + (double) R->omask[m];
-                const double L1 =
-                    (c * (double) R->cmask[m + 4] + s * (double) R->smask[m + 4]) + (double) R->omask[m + 4];
-                const double L2 =
-                    (c * (double) R->cmask[m + 8] + s * (double) R->smask[m + 8]) + (double) R->omask[m + 8];
+                const double L0 = (c * (double) cm[m] + s * (double) sm[m]) + (double) om[m];
+                const double L1 = (c * (double) cm[m + 4] + s * (double) sm[m + 4]) + (double) om[m + 4];
+                const double L2 = (c * (double) cm[m + 8] + s * (double) sm[m + 8]) + (double) om[m + 8];
                 double v = (a0 * L0 + a1 * L1) + a2 * L2;
                 if (cc == 3) v = v + a3;
                 T = v;
@@ -879,9 +885,9 @@ __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
 }
 
 // FP64 re-decision of the queued tests, exactly as the oracle computes them.
-__global__ __launch_bounds__(kBS) void k_redecide(DiscreteArgs a) {
+__device__ __forceinline__ void redecide_range(const DiscreteArgs& a, uint32_t first, uint32_t stride) {
     const uint32_t total = min(*a.unsure_count, a.unsure_cap);
-    for (uint32_t i = blockIdx.x * kBS + threadIdx.x; i < total; i += gridDim.x * kBS) {
+    for (uint32_t i = first; i < total; i += stride) {
         const uint4 u = a.unsure[i];
         const uint32_t p = u.x / (uint32_t) a.n_links;
         uint32_t* flag = a.flags + p;
@@ -911,13 +917,27 @@ __global__ __launch_bounds__(kBS) void k_redecide(DiscreteArgs a) {
         if (x64::seg_tri(P, Q, A, B, C)) atomicOr(flag, 1u);
     }
 }
+__global__ __launch_bounds__(kBS) void k_redecide(DiscreteArgs a) {
+    redecide_range(a, blockIdx.x * kBS + threadIdx.x, gridDim.x * kBS);
+}
 
-// Status words and small flag arrays into pinned host memory (see launch_publish).
+// Status words and small flag arrays into pinned host memory (see launch_redecide).
 __global__ __launch_bounds__(kBS) void k_publish(const uint32_t* __restrict__ scalars, uint32_t* status_out,
                                                 const uint32_t* __restrict__ flags, uint32_t* flags_out, uint32_t n) {
     if (flags_out)
         for (uint32_t i = blockIdx.x * kBS + threadIdx.x; i < n; i += gridDim.x * kBS) flags_out[i] = flags[i];
     if (blockIdx.x == 0 && threadIdx.x < kPublish) status_out[threadIdx.x] = scalars[threadIdx.x];
+}
+
+// Small batches: the whole re-decision queue in one block, which then publishes (its own flag
+// updates are visible to it after the barrier; everything else was written by earlier kernels).
+__global__ __launch_bounds__(kBS) void k_redecide_publish(DiscreteArgs a, const uint32_t* __restrict__ scalars,
+                                                         uint32_t* status_out, uint32_t* flags_out) {
+    redecide_range(a, threadIdx.x, kBS);
+    __syncthreads();
+    if (flags_out)
+        for (uint32_t i = threadIdx.x; i < (uint32_t) a.n_poses; i += kBS) flags_out[i] = load_flag(a.flags + i);
+    if (threadIdx.x < kPublish) status_out[threadIdx.x] = scalars[threadIdx.x];
 }
 
 int device_cus() {
@@ -1012,15 +1032,18 @@ void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s) {
     RTCD_LAUNCH_CHECK();
 }
 
-void launch_redecide(const DiscreteArgs& a, hipStream_t s) {
+void launch_redecide(const DiscreteArgs& a, bool one_block, const uint32_t* scalars, uint32_t* status_out,
+                     uint32_t* flags_out, hipStream_t s) {
+    if (one_block) {
+        hipLaunchKernelGGL(k_redecide_publish, dim3(1), dim3(kBS), 0, s, a, scalars, status_out, flags_out);
+        RTCD_LAUNCH_CHECK();
+        return;
+    }
     hipLaunchKernelGGL(k_redecide, dim3((unsigned) device_cus() * 2u), dim3(kBS), 0, s, a);
     RTCD_LAUNCH_CHECK();
-}
-
-void launch_publish(const uint32_t* scalars, uint32_t* status_out, const uint32_t* flags, uint32_t* flags_out,
-                    uint32_t n, hipStream_t s) {
+    const uint32_t n = (uint32_t) a.n_poses;
     const unsigned nb = flags_out ? std::max(1u, std::min(blocks_for(n, kBS), 64u)) : 1u;
-    hipLaunchKernelGGL(k_publish, dim3(nb), dim3(kBS), 0, s, scalars, status_out, flags, flags_out, n);
+    hipLaunchKernelGGL(k_publish, dim3(nb), dim3(kBS), 0, s, scalars, status_out, a.flags, flags_out, n);
     RTCD_LAUNCH_CHECK();
 }
 

@@ -99,13 +99,12 @@ void launch_gather(const DiscreteArgs& a, hipStream_t s);
 void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStream_t s);
 // Compacts the gathered records of still-unflagged poses into a.r2o_list (after obstacle->robot).
 void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s);
-// FP64 re-decision of the queued tests.
-void launch_redecide(const DiscreteArgs& a, hipStream_t s);
-// Copies the first kPublish status words, and flags[0, n) when flags_out is set, into pinned host
-// memory (device addresses of HostMapped buffers).
+// FP64 re-decision of the queued tests, then a copy of the first kPublish status words (and of the
+// flags when flags_out is set) into pinned host memory (device addresses of HostMapped buffers).
+// one_block: the whole queue in one work-group that also publishes (small batches: one launch).
 constexpr int kPublish = 8;
-void launch_publish(const uint32_t* scalars, uint32_t* status_out, const uint32_t* flags, uint32_t* flags_out,
-                    uint32_t n, hipStream_t s);
+void launch_redecide(const DiscreteArgs& a, bool one_block, const uint32_t* scalars, uint32_t* status_out,
+                     uint32_t* flags_out, hipStream_t s);
 
 // ---- swept volumes ----
 struct SweepArgs {
