@@ -148,7 +148,17 @@ def reference_scene_bench(ctx, rb, reps=5):
         r = {'tris': int(sc.n_tris)}
         for key, d in (('o2r', 1), ('r2o', 2), ('two_way', 3)):
             r[f'{key}_us_per_pose'] = round(med(lambda: ctx.check_discrete(sid, q, d)) / len(q) * 1e6, 4)
-            r[f'{key}_single_pose_us'] = round(med(lambda: ctx.check_discrete(sid, q[:1], d), 20) * 1e6, 1)
+            # one-pose batches over 64 different poses (the reference's loop also steps through its
+            # poses): median and 90th percentile of the blocking call
+            ctx.check_discrete(sid, q[:1], d)
+            ctx.check_discrete(sid, q[:1], d)
+            ts = []
+            for i in range(64):
+                t = time.perf_counter()
+                ctx.check_discrete(sid, q[i + 1:i + 2], d)
+                ts.append(time.perf_counter() - t)
+            r[f'{key}_single_pose_us'] = round(float(np.median(ts)) * 1e6, 1)
+            r[f'{key}_single_pose_p90_us'] = round(float(np.percentile(ts, 90)) * 1e6, 1)
         r['collisions_two_way'] = int(ctx.check_discrete(sid, q, 3).astype(bool).sum())
         r['linear16_us_per_traj'] = round(med(lambda: ctx.check_linear(sid, tr.reshape(-1, 7), offs)) / len(tr) * 1e6, 3)
         r['quadratic8_us_per_traj'] = round(med(lambda: ctx.check_quadratic(sid, tr, 8, W)) / len(tr) * 1e6, 3)

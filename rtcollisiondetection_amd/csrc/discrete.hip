@@ -557,9 +557,12 @@ __device__ __forceinline__ int o2r_visit(const DiscreteArgs& a, const O2RWalk& w
 struct R2OWalk {
     float3 A, e1, e2, nrm;
     float delta, lx, ly, lz, hx, hy, hz, nl, ne;
+    float sg;  // orientation of the triangle's shadow on the plane of the axes other than ax (0: none)
     MTBands bd;
-    int node0, prim0;
+    int node0, prim0, ax;
 };
+
+__device__ __forceinline__ float pick(const float3& p, int k) { return k == 0 ? p.x : k == 1 ? p.y : p.z; }
 
 // Sets up the walk; false if the triangle's box misses the link box.
 __device__ __forceinline__ bool r2o_begin(const LinkInfo& L, float3 A, float3 e1, float3 e2, float delta, R2OWalk& w) {
@@ -583,6 +586,17 @@ __device__ __forceinline__ bool r2o_begin(const LinkInfo& L, float3 A, float3 e1
     w.nrm = make_float3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
     w.nl = l1(w.nrm.x, w.nrm.y, w.nrm.z);
     w.ne = 32.f * kEps32 * E1 * E2;
+    // shadow of the triangle on the plane of the two axes other than its normal's largest one; its
+    // orientation counts only where the rounded 2D area is larger than its error bound
+    const float ax_ = fabsf(w.nrm.x), ay_ = fabsf(w.nrm.y), az_ = fabsf(w.nrm.z);
+    w.ax = ax_ >= ay_ && ax_ >= az_ ? 0 : ay_ >= az_ ? 1 : 2;
+    {
+        const int iu = w.ax == 2 ? 0 : w.ax + 1, iv = w.ax == 0 ? 2 : w.ax - 1;
+        const float Au = pick(A, iu), Av = pick(A, iv);
+        const float p1 = (pick(B, iu) - Au) * (pick(C, iv) - Av), p2 = (pick(B, iv) - Av) * (pick(C, iu) - Au);
+        const float area = p1 - p2, err = 5.f * kEps32 * (fabsf(p1) + fabsf(p2));
+        w.sg = area > err ? 1.f : area < -err ? -1.f : 0.f;
+    }
     w.A = A;
     w.e1 = e1;
     w.e2 = e2;
@@ -604,14 +618,38 @@ __device__ __forceinline__ int r2o_visit(const DiscreteArgs& a, const R2OWalk& w
                         4.f * kEps32 * (fabsf(sd) + rr);
         return fabsf(sd) <= rr + m;
     };
+    // A robot edge inside a node box crosses the triangle at a point of the box, so the box meets the
+    // triangle's shadow on the (u, v) plane.  The box, grown by delta (the error of the triangle's
+    // vertices), entirely outside one edge line of the shadow holds no crossing.  The box corner that
+    // maximises the edge function and the signs are exact; m covers its rounding and the growth.
+    auto shadow_misses = [&](float bx0, float by0, float bz0, float bx1, float by1, float bz1) {
+        if (w.sg == 0.f) return false;
+        const int iu = w.ax == 2 ? 0 : w.ax + 1, iv = w.ax == 0 ? 2 : w.ax - 1;
+        const float3 lo = make_float3(bx0, by0, bz0), hi = make_float3(bx1, by1, bz1);
+        const float lu = pick(lo, iu), lv = pick(lo, iv), hu = pick(hi, iu), hv = pick(hi, iv);
+        const float Au = pick(w.A, iu), Av = pick(w.A, iv);
+        const float Bu = Au + pick(w.e1, iu), Bv = Av + pick(w.e1, iv);
+        const float Cu = Au + pick(w.e2, iu), Cv = Av + pick(w.e2, iv);
+        auto outside = [&](float Pu, float Pv, float Qu, float Qv) {
+            const float du = (Qu - Pu) * w.sg, dv = (Qv - Pv) * w.sg;
+            // sg * E(x) = du (x_v - P_v) - dv (x_u - P_u), largest at this corner
+            const float xu = dv > 0.f ? lu : hu, xv = du > 0.f ? hv : lv;
+            const float t1 = du * (xv - Pv), t2 = dv * (xu - Pu);
+            const float m = 5.f * kEps32 * (fabsf(t1) + fabsf(t2)) + 1.01f * (fabsf(du) + fabsf(dv)) * w.delta;
+            return (t1 - t2) + m < 0.f;
+        };
+        return outside(Au, Av, Bu, Bv) || outside(Bu, Bv, Cu, Cv) || outside(Cu, Cv, Au, Av);
+    };
     const BNode nd = a.rb.edge_nodes[w.node0 + ni];
     if (COUNT) ctr.nodes++;
     bool h0 = nd.c0 >= 0 &&
               !(w.lx > nd.hi0x || w.hx < nd.lo0x || w.ly > nd.hi0y || w.hy < nd.lo0y || w.lz > nd.hi0z || w.hz < nd.lo0z) &&
-              straddles(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
+              straddles(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z) &&
+              !shadow_misses(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z);
     bool h1 = nd.c1 >= 0 &&
               !(w.lx > nd.hi1x || w.hx < nd.lo1x || w.ly > nd.hi1y || w.hy < nd.lo1y || w.lz > nd.hi1z || w.hz < nd.lo1z) &&
-              straddles(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
+              straddles(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z) &&
+              !shadow_misses(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z);
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const bool hs = side ? h1 : h0;
