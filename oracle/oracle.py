@@ -55,6 +55,9 @@ def lib():
         L.orc_check_linear.argtypes = [C.c_void_p, C.c_void_p, _f32p, _u32p, C.c_int, _u32p, C.c_void_p, C.c_int]
         L.orc_check_quadratic.argtypes = [C.c_void_p, C.c_void_p, _f32p, C.c_int, C.c_int, C.c_int, _f64p, _u32p,
                                           C.c_void_p, C.c_int]
+        L.orc_check_cubic.argtypes = [C.c_void_p, C.c_void_p, _f32p, C.c_int, C.c_int, C.c_int, _f64p, _u32p,
+                                      C.c_void_p, C.c_int]
+        L.orc_cubic_piece.argtypes = [_f64p, _f64p]
         L.orc_fit_ctrl.argtypes = [_f64p, C.c_int, C.c_int, _f64p, _f64p]
         _lib = L
     return _lib
@@ -104,6 +107,14 @@ def tube_decide(ctrl, r, A, B):
     ctrl = np.ascontiguousarray(np.asarray(ctrl, dtype=np.float64).reshape(-1))
     deg = len(ctrl) // 3 - 1
     return lib().orc_tube_decide(deg, ctrl, float(r), _d3(A), _d3(B))
+
+
+def cubic_piece(P):
+    """Bezier controls [4, 3] of the uniform cubic B-spline span with control points P [4, 3]."""
+    P = np.ascontiguousarray(np.asarray(P, dtype=np.float64).reshape(4, 3))
+    out = np.zeros((4, 3))
+    lib().orc_cubic_piece(P, out)
+    return out
 
 
 def fit_ctrl(W, X):
@@ -193,4 +204,18 @@ def check_quadratic(robot, mesh, wp, W, with_segments=False, nthreads=None):
     nt = nthreads or os.cpu_count() or 1
     lib().orc_check_quadratic(robot.h, mesh.h, wp, n, nwp, nc, W, out,
                               seg.ctypes.data_as(C.c_void_p) if seg is not None else None, nt)
+    return (out, seg) if with_segments else out
+
+
+def check_cubic(robot, mesh, wp, W, with_segments=False, nthreads=None):
+    """Cubic B-spline sweeps: as check_quadratic with n_ctrl - 3 uniform cubic pieces per trajectory."""
+    wp = np.ascontiguousarray(wp, dtype=np.float32)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    n, nwp = wp.shape[0], wp.shape[1]
+    nc = W.shape[0]
+    out = np.zeros(n, dtype=np.uint32)
+    seg = np.zeros((n, nc - 3), dtype=np.uint32) if with_segments else None
+    nt = nthreads or os.cpu_count() or 1
+    lib().orc_check_cubic(robot.h, mesh.h, wp, n, nwp, nc, W, out,
+                          seg.ctypes.data_as(C.c_void_p) if seg is not None else None, nt)
     return (out, seg) if with_segments else out

@@ -134,13 +134,15 @@ int orc_seg_tri(const double* P, const double* Q, const double* A, const double*
  *   EA(t) = |C(t) - A|^2 - r^2 <= 0,   EB(t) = |C(t) - B|^2 - r^2 <= 0,
  *   G(t) = dd |w|^2 - (w.D)^2 - r^2 dd <= 0  with  0 <= p(t) = w.D <= dd  (dd > 0)
  * holds.  All four are polynomials in Bernstein form (|.|^2 terms of degree 2n, scaled by 3 when
- * n = 2 to keep the coefficients division-free); the decision subdivides [0,1] at midpoints
+ * n = 2 and by 10 when n = 3 to keep the coefficients division-free: the degree-2n Bernstein
+ * coefficients of a product of two degree-n polynomials are sums of C(n,i) C(n,j) / C(2n,i+j) times
+ * the coefficient products); the decision subdivides [0,1] at midpoints
  * (de Casteljau) depth first: endpoint coefficients are polynomial values and certify a hit, the
  * coefficient range bounds the polynomial and excludes an interval.  Undecided intervals at depth
  * ORC_TUBE_DMAX are dropped, and more than ORC_TUBE_BUDGET intervals end the search with a miss.
  * ---------------------------------------------------------------------------------------------- */
 static void dc_split(const double* c, int m, double* L, double* R) {
-    double b[5];
+    double b[7] = {0, 0, 0, 0, 0, 0, 0};
     for (int i = 0; i <= m; ++i) b[i] = c[i];
     L[0] = b[0];
     R[m] = b[m];
@@ -152,7 +154,7 @@ static void dc_split(const double* c, int m, double* L, double* R) {
 }
 
 typedef struct {
-    double G[5], P[3], EA[5], EB[5];
+    double G[7], P[4], EA[7], EB[7];
     int depth;
 } tnode;
 
@@ -169,7 +171,7 @@ static double maxv(const double* c, int m) {
 
 int orc_tube_decide(int deg, const double* ctrl, double r, const double* A, const double* B) {
     const int n = deg, m = 2 * deg;
-    double w[3][3], wb[3][3], D[3], p[3];
+    double w[4][3], wb[4][3], D[3], p[4];
     sub3(B, A, D);
     for (int i = 0; i <= n; ++i) {
         sub3(ctrl + 3 * i, A, w[i]);
@@ -177,8 +179,25 @@ int orc_tube_decide(int deg, const double* ctrl, double r, const double* A, cons
     }
     const double dd = dot3(D, D);
     for (int i = 0; i <= n; ++i) p[i] = dot3(w[i], D);
-    double Sw[5], Sb[5], Sp[5], k;
-    if (n == 1) {
+    double Sw[7], Sb[7], Sp[7], k;
+    if (n == 3) {
+        Sw[0] = 10.0 * dot3(w[0], w[0]); Sw[1] = 10.0 * dot3(w[0], w[1]);
+        Sw[2] = 4.0 * dot3(w[0], w[2]) + 6.0 * dot3(w[1], w[1]);
+        Sw[3] = dot3(w[0], w[3]) + 9.0 * dot3(w[1], w[2]);
+        Sw[4] = 4.0 * dot3(w[1], w[3]) + 6.0 * dot3(w[2], w[2]);
+        Sw[5] = 10.0 * dot3(w[2], w[3]); Sw[6] = 10.0 * dot3(w[3], w[3]);
+        Sb[0] = 10.0 * dot3(wb[0], wb[0]); Sb[1] = 10.0 * dot3(wb[0], wb[1]);
+        Sb[2] = 4.0 * dot3(wb[0], wb[2]) + 6.0 * dot3(wb[1], wb[1]);
+        Sb[3] = dot3(wb[0], wb[3]) + 9.0 * dot3(wb[1], wb[2]);
+        Sb[4] = 4.0 * dot3(wb[1], wb[3]) + 6.0 * dot3(wb[2], wb[2]);
+        Sb[5] = 10.0 * dot3(wb[2], wb[3]); Sb[6] = 10.0 * dot3(wb[3], wb[3]);
+        Sp[0] = 10.0 * (p[0] * p[0]); Sp[1] = 10.0 * (p[0] * p[1]);
+        Sp[2] = 4.0 * (p[0] * p[2]) + 6.0 * (p[1] * p[1]);
+        Sp[3] = p[0] * p[3] + 9.0 * (p[1] * p[2]);
+        Sp[4] = 4.0 * (p[1] * p[3]) + 6.0 * (p[2] * p[2]);
+        Sp[5] = 10.0 * (p[2] * p[3]); Sp[6] = 10.0 * (p[3] * p[3]);
+        k = 10.0;
+    } else if (n == 1) {
         Sw[0] = dot3(w[0], w[0]); Sw[1] = dot3(w[0], w[1]); Sw[2] = dot3(w[1], w[1]);
         Sb[0] = dot3(wb[0], wb[0]); Sb[1] = dot3(wb[0], wb[1]); Sb[2] = dot3(wb[1], wb[1]);
         Sp[0] = p[0] * p[0]; Sp[1] = p[0] * p[1]; Sp[2] = p[1] * p[1];
@@ -749,4 +768,54 @@ void orc_check_quadratic(const orc_robot* rb, const orc_mesh* scene, const float
     int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg, int nthreads) {
     sweep_ctx c = {rb, scene, wp, NULL, out, out_seg, n_wp, n_ctrl, W};
     par_for(n_traj, nthreads, quad_one, &c);
+}
+
+/* Bezier controls of span p of the uniform cubic B-spline with control points P (the span
+ * OPTIX_PRIMITIVE_TYPE_ROUND_CUBIC_BSPLINE draws, RTCD/Robot/batchCurveRobot.h:203-204):
+ * ((P0 + 4 P1) + P2) / 6, (2 P1 + P2) / 3, (P1 + 2 P2) / 3, ((P1 + 4 P2) + P3) / 6, with the
+ * divisions as products by the rounded constants 1/6 and 1/3. */
+static const double ORC_SIXTH = 1.0 / 6.0, ORC_THIRD = 1.0 / 3.0;
+void orc_cubic_piece(const double* P, double* ctrl) {
+    for (int cc = 0; cc < 3; ++cc) {
+        const double p0 = P[cc], p1 = P[3 + cc], p2 = P[6 + cc], p3 = P[9 + cc];
+        ctrl[cc] = ((p0 + 4.0 * p1) + p2) * ORC_SIXTH;
+        ctrl[3 + cc] = (2.0 * p1 + p2) * ORC_THIRD;
+        ctrl[6 + cc] = (p1 + 2.0 * p2) * ORC_THIRD;
+        ctrl[9 + cc] = ((p1 + 4.0 * p2) + p3) * ORC_SIXTH;
+    }
+}
+
+static void cubic_one(void* c, int i) {
+    sweep_ctx* x = (sweep_ctx*) c;
+    const orc_robot* rb = x->rb;
+    int nw = x->n_wp, nc = x->n_ctrl, npc = nc - 3;
+    double* cen = (double*) malloc(sizeof(double) * 3 * rb->ns * nw);
+    for (int k = 0; k < nw; ++k) orc_sphere_centers(rb, x->traj + ((size_t) i * nw + k) * rb->dof, cen + 3 * rb->ns * k);
+    double* X = (double*) malloc(sizeof(double) * 3 * nw);
+    double* P = (double*) malloc(sizeof(double) * 3 * nc);
+    uint32_t* seg = (uint32_t*) calloc(npc > 0 ? npc : 1, sizeof(uint32_t));
+    for (int s = 0; s < rb->ns; ++s) {
+        for (int k = 0; k < nw; ++k)
+            for (int cc = 0; cc < 3; ++cc) X[3 * k + cc] = cen[3 * rb->ns * k + 3 * s + cc];
+        orc_fit_ctrl(x->W, nc, nw, X, P);
+        for (int p = 0; p < npc; ++p) {
+            if (seg[p]) continue;
+            double ctrl[12];
+            orc_cubic_piece(P + 3 * p, ctrl);
+            seg[p] = (uint32_t) tube_vs_scene(x->sc, 3, ctrl, rb->sph[4 * s + 3]);
+        }
+    }
+    uint32_t any = 0;
+    for (int p = 0; p < npc; ++p) {
+        any |= seg[p];
+        if (x->out_seg) x->out_seg[(size_t) i * npc + p] = seg[p];
+    }
+    x->out[i] = any;
+    free(cen); free(X); free(P); free(seg);
+}
+
+void orc_check_cubic(const orc_robot* rb, const orc_mesh* scene, const float* wp, int n_traj,
+    int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg, int nthreads) {
+    sweep_ctx c = {rb, scene, wp, NULL, out, out_seg, n_wp, n_ctrl, W};
+    par_for(n_traj, nthreads, cubic_one, &c);
 }

@@ -20,7 +20,7 @@
  * Exact contract (the reference leaves these to closed OptiX intersectors, so they are fixed here):
  *   - segment vs triangle: closed segment, closed triangle, coplanar (det == 0) is a miss, decided by
  *     the division-free scaled Moller-Trumbore predicate of orc_seg_tri;
- *   - swept sphere: the solid tube { x : min_t |x - C(t)| <= r } of a degree-1 or degree-2 Bezier
+ *   - swept sphere: the solid tube { x : min_t |x - C(t)| <= r } of a degree-1, -2 or -3 Bezier
  *     C(t), decided by orc_tube_decide (Bernstein subdivision, depth/budget capped);
  *   - every FP64 operation has a fixed evaluation order and no FMA contraction, so a device
  *     implementation of the same sequence reproduces the results bit for bit.
@@ -49,7 +49,7 @@ int orc_seg_tri(const double* P, const double* Q, const double* A, const double*
 /* Same, also returning det, u, v, t (scaled Moller-Trumbore quantities) in q4. */
 int orc_seg_tri_q(const double* P, const double* Q, const double* A, const double* B, const double* C, double* q4);
 
-/* 1 if segment AB comes within r of the Bezier curve of degree deg (1 or 2) with control points
+/* 1 if segment AB comes within r of the Bezier curve of degree deg (1, 2 or 3) with control points
  * ctrl[(deg+1)*3].  ORC_TUBE_DMAX / ORC_TUBE_BUDGET bound the subdivision. */
 #define ORC_TUBE_DMAX 40
 #define ORC_TUBE_BUDGET 2048
@@ -89,6 +89,15 @@ void orc_check_linear(const orc_robot* rb, const orc_mesh* scene, const float* t
  * ((P_i+P_{i+1})/2, P_{i+1}, (P_{i+1}+P_{i+2})/2).  out_seg (nullable): [n_traj][n_ctrl-2]. */
 void orc_check_quadratic(const orc_robot* rb, const orc_mesh* scene, const float* wp, int n_traj,
     int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg, int nthreads);
+
+/* Cubic B-spline sweep (RTCD's CUBIC_CURVE robot: RTCD/Robot/batchRobotConfig.h:19,
+ * batchCurveRobot.h:203-204, a ROUND_CUBIC_BSPLINE curve GAS): as orc_check_quadratic with uniform
+ * cubic pieces i = 0..n_ctrl-4, each the Bezier of orc_cubic_piece(P + 3 i).  out_seg (nullable):
+ * [n_traj][n_ctrl-3]. */
+void orc_check_cubic(const orc_robot* rb, const orc_mesh* scene, const float* wp, int n_traj,
+    int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg, int nthreads);
+/* Bezier controls ctrl[12] of the uniform cubic B-spline span with control points P[4][3]. */
+void orc_cubic_piece(const double* P, double* ctrl);
 
 /* Control points of one sphere path: X[n_wp][3] -> P[n_ctrl][3] (sweptVolume.cu:22-83). */
 void orc_fit_ctrl(const double* W, int n_ctrl, int n_wp, const double* X, double* P);

@@ -224,6 +224,66 @@ def test_quadratic_with_collinear_even_ctrl_equals_capsule():
         assert O.tube_decide(np.stack([c0, c1, c2]), r, A, B) == O.tube_decide(np.stack([c0, c2]), r, A, B)
 
 
+def _cubic_seg_dist(ctrl, A, B, n=40001):
+    t = np.linspace(0, 1, n)[:, None]
+    C = ((1 - t) ** 3 * ctrl[0] + 3 * t * (1 - t) ** 2 * ctrl[1] + 3 * t ** 2 * (1 - t) * ctrl[2] +
+         t ** 3 * ctrl[3])
+    D = B - A
+    s = np.clip(((C - A) @ D) / (D @ D), 0, 1)[:, None]
+    return np.sqrt(np.min(np.sum((C - (A + s * D)) ** 2, axis=1)))
+
+
+def test_cubic_tube_matches_dense_sampling():
+    rng = np.random.default_rng(21)
+    n_hit = 0
+    for _ in range(300):
+        ctrl = rng.uniform(-1, 1, (4, 3))
+        A = rng.uniform(-1, 1, 3)
+        B = A + rng.uniform(-1, 1, 3)
+        r = rng.uniform(0.05, 0.5)
+        dist = _cubic_seg_dist(ctrl, A, B)
+        # sampling overestimates the distance by at most L * dt / 2, |C'| <= 3 * ctrl polygon length
+        L = 3 * sum(np.linalg.norm(ctrl[i + 1] - ctrl[i]) for i in range(3))
+        tol = L / 40000 + 1e-9
+        got = O.tube_decide(ctrl, r, A, B)
+        if dist < r - 1e-9:
+            assert got == 1
+        elif dist - tol > r + 1e-9:
+            assert got == 0
+        n_hit += got
+    assert 30 < n_hit < 270
+
+
+def test_cubic_of_elevated_quadratic_equals_quadratic():
+    # a quadratic (q0, q1, q2) is the cubic (q0, (q0 + 2 q1) / 3, (2 q1 + q2) / 3, q2)
+    rng = np.random.default_rng(22)
+    for _ in range(200):
+        q = rng.uniform(-1, 1, (3, 3))
+        cub = np.stack([q[0], (q[0] + 2 * q[1]) / 3, (2 * q[1] + q[2]) / 3, q[2]])
+        A = rng.uniform(-1, 1, 3)
+        B = A + rng.uniform(-1, 1, 3)
+        r = rng.uniform(0.05, 0.5)
+        dist = _curve_seg_dist(q, A, B)
+        L = 2 * (np.linalg.norm(q[1] - q[0]) + np.linalg.norm(q[2] - q[1]))
+        if abs(dist - r) < L / 20000 + 1e-6:
+            continue
+        assert O.tube_decide(cub, r, A, B) == O.tube_decide(q, r, A, B)
+
+
+def test_cubic_piece_is_the_uniform_bspline_span():
+    # span of the uniform cubic B-spline: ((1-t)^3 P0 + (3t^3 - 6t^2 + 4) P1 + (-3t^3 + 3t^2 + 3t + 1) P2
+    # + t^3 P3) / 6 (closed form of the uniform basis), against the Bezier form of orc_cubic_piece
+    rng = np.random.default_rng(23)
+    t = np.linspace(0, 1, 33)[:, None]
+    for _ in range(50):
+        P = rng.uniform(-1, 1, (4, 3))
+        span = ((1 - t) ** 3 * P[0] + (3 * t ** 3 - 6 * t ** 2 + 4) * P[1] +
+                (-3 * t ** 3 + 3 * t ** 2 + 3 * t + 1) * P[2] + t ** 3 * P[3]) / 6
+        b = O.cubic_piece(P)
+        bez = ((1 - t) ** 3 * b[0] + 3 * t * (1 - t) ** 2 * b[1] + 3 * t ** 2 * (1 - t) * b[2] + t ** 3 * b[3])
+        assert np.abs(span - bez).max() < 1e-14
+
+
 def test_tube_degenerate_cases():
     # zero-length query segment: point inside / outside the capsule
     assert O.tube_decide([[0, 0, 0], [1, 0, 0]], 0.1, [0.5, 0.05, 0], [0.5, 0.05, 0]) == 1
