@@ -137,6 +137,17 @@ rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* 
 rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_waypoints, uint32_t n_traj,
                                         uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
                                         uint32_t* d_out_seg, void* stream);
+
+/* Cubic B-spline sweep (RTCD CUBIC_CURVE, the default curve type of RTCD/Robot/batchRobotConfig.h:19:
+ * a ROUND_CUBIC_BSPLINE curve GAS, RTCD/Robot/batchCurveRobot.h:203-204, 379-402,
+ * RTCD/CollisionDetector/CollisionDetector.h:585-586): as rtcd_check_quadratic with a uniform cubic
+ * B-spline of n_ctrl >= 4 control points, n_ctrl - 3 pieces per trajectory (piece i: control points
+ * i..i+3).  W NULL: the fit of RTCD/data/weights/cubic/normal.  out_seg (nullable) [n_traj][n_ctrl-3]. */
+rtcd_status rtcd_check_cubic(rtcd_ctx* ctx, uint32_t scene_id, const float* waypoints, uint32_t n_traj,
+                             uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg);
+rtcd_status rtcd_check_cubic_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_waypoints, uint32_t n_traj,
+                                    uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
+                                    uint32_t* d_out_seg, void* stream);
 /* ---- Geometry ingestion (host only): the files RTCD reads for robots and obstacles. ---- */
 /* OBJ mesh (RTCD/Meshes/mesh.h:311-365, meshModel::loadOBJ): polygons fan-triangulated, vertices
  * deduplicated by exact value in first-use order (mesh.h:218-231).  With verts == tris == NULL only
@@ -178,7 +189,7 @@ rtcd_status rtcd_quadratic_weights(uint32_t n_ctrl, uint32_t n_wp, double* W);
  * {normal,repeated}/{nC}_{nT}.txt that batchOptixCurveRobot::loadCurveWeight reads
  * (RTCD/Robot/batchCurveRobot.h:560-605; end-point types RTCD/Robot/batchRobotConfig.h:8-26):
  * NORMAL = uniform knots, REPEATED = open-uniform (clamped) knots.  Quadratic weights of either kind
- * can be passed to rtcd_check_quadratic as W. */
+ * can be passed to rtcd_check_quadratic as W, cubic ones to rtcd_check_cubic. */
 enum { RTCD_END_NORMAL = 0, RTCD_END_REPEATED = 1 };
 rtcd_status rtcd_curve_weights(uint32_t degree, uint32_t end_points, uint32_t n_ctrl, uint32_t n_wp, double* W);
 

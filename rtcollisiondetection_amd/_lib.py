@@ -19,7 +19,7 @@ O2R, R2O, TWO_WAY = 1, 2, 3
 EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 'rtcd_scene_create',
            'rtcd_scene_destroy', 'rtcd_scene_info', 'rtcd_check_discrete', 'rtcd_check_discrete_device', 'rtcd_sync',
            'rtcd_check_linear', 'rtcd_check_linear_device', 'rtcd_check_quadratic', 'rtcd_check_quadratic_device',
-           'rtcd_quadratic_weights', 'rtcd_curve_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
+           'rtcd_check_cubic', 'rtcd_check_cubic_device', 'rtcd_quadratic_weights', 'rtcd_curve_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
            'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_wait_timed', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param',
            'rtcd_set_leaf_size', 'rtcd_set_leaf_sizes', 'rtcd_autotune', 'rtcd_obj_read', 'rtcd_mesh_edges',
            'rtcd_urdf_read', 'rtcd_spheres_read', 'rtcd_robot_load_urdf']
@@ -123,6 +123,8 @@ def lib():
         L.rtcd_check_linear_device.argtypes = [vp, u32, vp, vp, u32, u32, vp, vp, vp]
         L.rtcd_check_quadratic.argtypes = [vp, u32, vp, u32, u32, u32, vp, vp, vp]
         L.rtcd_check_quadratic_device.argtypes = [vp, u32, vp, u32, u32, u32, vp, vp, vp, vp]
+        L.rtcd_check_cubic.argtypes = [vp, u32, vp, u32, u32, u32, vp, vp, vp]
+        L.rtcd_check_cubic_device.argtypes = [vp, u32, vp, u32, u32, u32, vp, vp, vp, vp]
         L.rtcd_quadratic_weights.argtypes = [u32, u32, vp]
         L.rtcd_curve_weights.argtypes = [u32, u32, u32, u32, vp]
         L.rtcd_set_counting.argtypes = [vp, i32]

@@ -297,9 +297,15 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                      uint32_t* d_out, size_t budget, rtcd_stats* st);
 void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_traj, const uint32_t* d_offs,
                     uint32_t n_traj, uint32_t nw, uint32_t* d_out, uint32_t* d_seg, size_t budget, bool events = true);
-void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
-                       uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget,
-                       bool events = true);
+// B-spline sweeps of degree 2 (quadratic) or 3 (cubic): n_ctrl - degree uniform pieces per trajectory.
+void enqueue_curve(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
+                   uint32_t nc, int degree, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget,
+                   bool events = true);
+inline void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj,
+                              uint32_t nw, uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg,
+                              size_t budget, bool events = true) {
+    enqueue_curve(c, ws, sc, d_wp, n_traj, nw, nc, 2, W, d_out, d_seg, budget, events);
+}
 // After ws.stream finished: re-runs (through `rerun`) when the FP64 queue overflowed.
 void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::function<void()>& rerun);
 // Device bytes a request of this shape needs in a fresh workspace (admission control).

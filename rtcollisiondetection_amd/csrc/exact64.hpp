@@ -128,7 +128,7 @@ constexpr int kTubeBudget = 2048;
 
 __host__ __device__ inline void dc_split(const double* c, int m, double* L, double* R) {
 #pragma clang fp contract(off)
-    double b[5];
+    double b[7];
     for (int i = 0; i <= m; ++i) b[i] = c[i];
     L[0] = b[0];
     R[m] = b[m];
@@ -140,7 +140,7 @@ __host__ __device__ inline void dc_split(const double* c, int m, double* L, doub
 }
 
 struct TNode {
-    double G[5], P[3], EA[5], EB[5];
+    double G[7], P[4], EA[7], EB[7];
     int depth;
 };
 
@@ -155,16 +155,17 @@ __host__ __device__ inline double maxv(const double* c, int m) {
     return r;
 }
 
-// Segment AB within r of the Bezier curve C(t) (degree 1 or 2, control points ctrl[(deg+1)*3]).
+// Segment AB within r of the Bezier curve C(t) (degree 1, 2 or 3, control points ctrl[(deg+1)*3]).
 // With w(t) = C(t) - A, D = B - A, dd = D.D, the segment meets the solid tube iff for some t in [0,1]
 //   EA = |C - A|^2 - r^2 <= 0, or EB = |C - B|^2 - r^2 <= 0, or
 //   G = dd |w|^2 - (w.D)^2 - r^2 dd <= 0 with 0 <= p = w.D <= dd (dd > 0).
-// The polynomials are kept in Bernstein form ((|.|^2 terms scaled by 3 for degree 2) and [0,1] is
+// The polynomials are kept in Bernstein form (|.|^2 terms scaled by 3 for degree 2, by 10 for degree 3:
+// a product of two degree-n polynomials has degree-2n coefficients sum C(n,i) C(n,j) / C(2n,i+j) a_i b_j) and [0,1] is
 // bisected depth first; endpoint coefficients certify a hit, coefficient ranges exclude an interval.
 __host__ __device__ inline int tube(int deg, const double* ctrl, double r, const double* A, const double* B) {
 #pragma clang fp contract(off)
     const int n = deg, m = 2 * deg;
-    double w[3][3], wb[3][3], D[3], p[3];
+    double w[4][3], wb[4][3], D[3], p[4];
     sub3(B, A, D);
     for (int i = 0; i <= n; ++i) {
         sub3(ctrl + 3 * i, A, w[i]);
@@ -172,8 +173,25 @@ __host__ __device__ inline int tube(int deg, const double* ctrl, double r, const
     }
     const double dd = dot3(D, D);
     for (int i = 0; i <= n; ++i) p[i] = dot3(w[i], D);
-    double Sw[5], Sb[5], Sp[5], k;
-    if (n == 1) {
+    double Sw[7], Sb[7], Sp[7], k;
+    if (n == 3) {
+        Sw[0] = 10.0 * dot3(w[0], w[0]); Sw[1] = 10.0 * dot3(w[0], w[1]);
+        Sw[2] = 4.0 * dot3(w[0], w[2]) + 6.0 * dot3(w[1], w[1]);
+        Sw[3] = dot3(w[0], w[3]) + 9.0 * dot3(w[1], w[2]);
+        Sw[4] = 4.0 * dot3(w[1], w[3]) + 6.0 * dot3(w[2], w[2]);
+        Sw[5] = 10.0 * dot3(w[2], w[3]); Sw[6] = 10.0 * dot3(w[3], w[3]);
+        Sb[0] = 10.0 * dot3(wb[0], wb[0]); Sb[1] = 10.0 * dot3(wb[0], wb[1]);
+        Sb[2] = 4.0 * dot3(wb[0], wb[2]) + 6.0 * dot3(wb[1], wb[1]);
+        Sb[3] = dot3(wb[0], wb[3]) + 9.0 * dot3(wb[1], wb[2]);
+        Sb[4] = 4.0 * dot3(wb[1], wb[3]) + 6.0 * dot3(wb[2], wb[2]);
+        Sb[5] = 10.0 * dot3(wb[2], wb[3]); Sb[6] = 10.0 * dot3(wb[3], wb[3]);
+        Sp[0] = 10.0 * (p[0] * p[0]); Sp[1] = 10.0 * (p[0] * p[1]);
+        Sp[2] = 4.0 * (p[0] * p[2]) + 6.0 * (p[1] * p[1]);
+        Sp[3] = p[0] * p[3] + 9.0 * (p[1] * p[2]);
+        Sp[4] = 4.0 * (p[1] * p[3]) + 6.0 * (p[2] * p[2]);
+        Sp[5] = 10.0 * (p[2] * p[3]); Sp[6] = 10.0 * (p[3] * p[3]);
+        k = 10.0;
+    } else if (n == 1) {
         Sw[0] = dot3(w[0], w[0]); Sw[1] = dot3(w[0], w[1]); Sw[2] = dot3(w[1], w[1]);
         Sb[0] = dot3(wb[0], wb[0]); Sb[1] = dot3(wb[0], wb[1]); Sb[2] = dot3(wb[1], wb[1]);
         Sp[0] = p[0] * p[0]; Sp[1] = p[0] * p[1]; Sp[2] = p[1] * p[1];
@@ -226,6 +244,21 @@ __host__ __device__ inline int tube(int deg, const double* ctrl, double r, const
         L.depth = R.depth = cur.depth + 1;
         stack[sp++] = R;
         cur = L;
+    }
+}
+
+// Bezier controls ctrl[12] of the uniform cubic B-spline span with control points P[4][3]:
+// ((P0 + 4 P1) + P2) / 6, (2 P1 + P2) / 3, (P1 + 2 P2) / 3, ((P1 + 4 P2) + P3) / 6, the divisions as
+// products by the rounded constants 1/6 and 1/3 (the oracle's orc_cubic_piece, operation for operation).
+__host__ __device__ inline void cubic_piece(const double* P, double* ctrl) {
+#pragma clang fp contract(off)
+    constexpr double kSixth = 1.0 / 6.0, kThird = 1.0 / 3.0;
+    for (int cc = 0; cc < 3; ++cc) {
+        const double p0 = P[cc], p1 = P[3 + cc], p2 = P[6 + cc], p3 = P[9 + cc];
+        ctrl[cc] = ((p0 + 4.0 * p1) + p2) * kSixth;
+        ctrl[3 + cc] = (2.0 * p1 + p2) * kThird;
+        ctrl[6 + cc] = (p1 + 2.0 * p2) * kThird;
+        ctrl[9 + cc] = ((p1 + 4.0 * p2) + p3) * kSixth;
     }
 }
 

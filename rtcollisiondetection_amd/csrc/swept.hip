@@ -214,13 +214,68 @@ __device__ int quad_cert(float3 q0, float3 q1, float3 q2, float3 A, float3 B, fl
     return all_miss ? kMiss : kUnsure;
 }
 
+// Cubic Bezier (q0, q1, q2, q3) vs segment, the same adaptive scheme: a sub-piece with controls
+// (a, b, c, d) lies within (3/4) max(|b - (2a + d)/3|, |c - (a + 2d)/3|) of its chord (the curve minus
+// the chord, both in degree-3 Bernstein form, has coefficients (0, b - (2a+d)/3, c - (a+2d)/3, 0) and
+// B1 + B2 <= 3/4).  Sub-piece controls come from the blossom at dyadic parameters: a = B(t0, t0, t0),
+// b = B(t0, t0, t1), c = B(t0, t1, t1), d = B(t1, t1, t1).
+__device__ __forceinline__ float3 blossom3(float3 q0, float3 q1, float3 q2, float3 q3, float u, float v, float w) {
+    const float iu = 1.f - u, iv = 1.f - v, iw = 1.f - w;
+    const float w0 = iu * iv * iw, w1 = u * iv * iw + iu * v * iw + iu * iv * w;
+    const float w2 = u * v * iw + u * iv * w + iu * v * w, w3 = u * v * w;
+    return make_float3(w0 * q0.x + w1 * q1.x + w2 * q2.x + w3 * q3.x, w0 * q0.y + w1 * q1.y + w2 * q2.y + w3 * q3.y,
+                       w0 * q0.z + w1 * q1.z + w2 * q2.z + w3 * q3.z);
+}
+
+// Deviation bound of a cubic (sub-)piece from its chord, for controls within e2 per component.
+__device__ __forceinline__ float cubic_dev(float3 a, float3 b, float3 c, float3 d, float e2) {
+    constexpr float kThird = 1.f / 3.f;
+    const float3 d1 = b - kThird * (2.f * a + d), d2 = c - kThird * (a + 2.f * d);
+    return 0.75f * sqrtf(fmaxf(dot(d1, d1), dot(d2, d2))) * (1.f + 8.f * kEps32) + 4.f * 1.7321f * e2;
+}
+
+__device__ int cubic_cert(float3 q0, float3 q1, float3 q2, float3 q3, float3 A, float3 B, float r, float ein,
+                          uint32_t& n_cert) {
+    const float M = fmaxf(fmaxf(linf(q0), linf(q1)), fmaxf(linf(q2), linf(q3)));
+    // products of three dyadic parameters are exact, the sums of three of them round once or twice,
+    // and the weighted sum of four controls adds its own rounding: 12 eps M per component
+    const float e2 = ein + 12.f * kEps32 * M;
+    bool all_miss = true;
+    int d = 0;
+    uint32_t k = 0;
+    for (;;) {
+        const float t0 = (float) k / (float) (1u << d), t1 = (float) (k + 1) / (float) (1u << d);
+        const float3 pa = blossom3(q0, q1, q2, q3, t0, t0, t0), pb = blossom3(q0, q1, q2, q3, t0, t0, t1);
+        const float3 pc = blossom3(q0, q1, q2, q3, t0, t1, t1), pd = blossom3(q0, q1, q2, q3, t1, t1, t1);
+        const float h = cubic_dev(pa, pb, pc, pd, e2);
+        const int cls = cap_cert(pa, pd, A, B, r - h, r + h, e2);
+        ++n_cert;
+        if (cls == kHit) return kHit;
+        if (cls != kMiss) {
+            if (d < kQuadDepth) {
+                ++d;
+                k <<= 1;
+                continue;
+            }
+            all_miss = false;
+        }
+        while (d > 0 && (k & 1u)) {
+            --d;
+            k >>= 1;
+        }
+        if (d == 0) break;
+        ++k;
+    }
+    return all_miss ? kMiss : kUnsure;
+}
+
 // ---------------------------------------------------------------------------------- sweep items
 // Item it = (g * SUB + k) * S + s: sub-piece k of segment / quadratic piece g, sphere s.  Spheres run
 // fastest, so the lanes of a wave hold the spheres of one sub-piece and traverse coherently.
 struct Item {
     uint32_t g, traj;
     int s;
-    float3 q0, q1, q2;  // capsule axis (q0, q1) or Bezier sub-piece controls (q0, q1, q2)
+    float3 q0, q1, q2, q3;  // capsule axis (q0, q1) or Bezier sub-piece controls (q0 .. q_DEG)
     float r;            // sphere radius
     float ein;          // max abs. error of the FP32 controls per component
     float M;            // max |coordinate| of the controls
@@ -243,6 +298,28 @@ __device__ __forceinline__ Item make_item(const SweepArgs& a, uint32_t it, int S
         m.r = x0.w;
         m.M = fmaxf(linf(m.q0), linf(m.q1));
         m.ein = 1.01f * kEps32 * m.M;
+    } else if (DEG == 3) {
+        const int sub_k = (int) (gk - m.g * (uint32_t) SUB);
+        const int npc = a.n_ctrl - 3;
+        m.traj = m.g / (uint32_t) npc;
+        const int piece = (int) (m.g - m.traj * (uint32_t) npc);
+        const size_t pb = ((size_t) m.traj * S + m.s) * a.n_ctrl + piece;
+        const float4 p0 = a.P32[pb], p1 = a.P32[pb + 1], p2 = a.P32[pb + 2], p3 = a.P32[pb + 3];
+        const float3 P0 = f3(p0), P1 = f3(p1), P2 = f3(p2), P3 = f3(p3);
+        // Bezier controls of the uniform cubic span: within 6 eps Mp of those of the FP64 control points
+        constexpr float kSixth = 1.f / 6.f, kThird = 1.f / 3.f;
+        const float3 b0 = kSixth * ((P0 + 4.f * P1) + P2), b1 = kThird * (2.f * P1 + P2);
+        const float3 b2 = kThird * (P1 + 2.f * P2), b3 = kSixth * ((P1 + 4.f * P2) + P3);
+        const float Mp = fmaxf(fmaxf(linf(P0), linf(P1)), fmaxf(linf(P2), linf(P3)));
+        // this item's sub-piece from the blossom (exact weights at the item's dyadic parameters): + 6 eps
+        const float t0 = (float) sub_k / (float) SUB, t1 = (float) (sub_k + 1) / (float) SUB;
+        m.q0 = blossom3(b0, b1, b2, b3, t0, t0, t0);
+        m.q1 = blossom3(b0, b1, b2, b3, t0, t0, t1);
+        m.q2 = blossom3(b0, b1, b2, b3, t0, t1, t1);
+        m.q3 = blossom3(b0, b1, b2, b3, t1, t1, t1);
+        m.ein = 16.f * kEps32 * Mp;
+        m.M = fmaxf(fmaxf(linf(m.q0), linf(m.q1)), fmaxf(linf(m.q2), linf(m.q3)));
+        m.r = p0.w;
     } else {
         const int sub_k = (int) (gk - m.g * (uint32_t) SUB);
         const int npc = a.n_ctrl - 2;
@@ -279,19 +356,24 @@ __device__ __forceinline__ Cull make_cull(const Item& m, float scale) {
     const float pad = m.r + 4.f * m.ein + 1e-6f * (1.f + scale);
     c.lx = fminf(m.q0.x, m.q1.x); c.ly = fminf(m.q0.y, m.q1.y); c.lz = fminf(m.q0.z, m.q1.z);
     c.hx = fmaxf(m.q0.x, m.q1.x); c.hy = fmaxf(m.q0.y, m.q1.y); c.hz = fmaxf(m.q0.z, m.q1.z);
-    if (DEG == 2) {
+    if (DEG >= 2) {
         c.lx = fminf(c.lx, m.q2.x); c.ly = fminf(c.ly, m.q2.y); c.lz = fminf(c.lz, m.q2.z);
         c.hx = fmaxf(c.hx, m.q2.x); c.hy = fmaxf(c.hy, m.q2.y); c.hz = fmaxf(c.hz, m.q2.z);
     }
+    if (DEG == 3) {
+        c.lx = fminf(c.lx, m.q3.x); c.ly = fminf(c.ly, m.q3.y); c.lz = fminf(c.lz, m.q3.z);
+        c.hx = fmaxf(c.hx, m.q3.x); c.hy = fmaxf(c.hy, m.q3.y); c.hz = fmaxf(c.hz, m.q3.z);
+    }
     c.lx -= pad; c.ly -= pad; c.lz -= pad; c.hx += pad; c.hy += pad; c.hz += pad;
     c.cp = m.q0;
-    c.cinv = safe_inv((DEG == 1 ? m.q1 : m.q2) - m.q0);
+    c.cinv = safe_inv((DEG == 1 ? m.q1 : DEG == 2 ? m.q2 : m.q3) - m.q0);
     c.cpad = pad;
     if (DEG == 2) {
         // a quadratic sub-piece lies within |q1 - (q0 + q2) / 2| / 2 of its chord
         const float3 dev = m.q1 - 0.5f * (m.q0 + m.q2);
         c.cpad += 0.5f * sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 4.f * 1.7321f * (m.ein + 8.f * kEps32 * m.M);
     }
+    if (DEG == 3) c.cpad += cubic_dev(m.q0, m.q1, m.q2, m.q3, m.ein + 12.f * kEps32 * m.M);
     return c;
 }
 
@@ -394,8 +476,10 @@ __global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, int pass) {
                     if (DEG == 1) {
                         cls = cap_cert(m.q0, m.q1, A, B, m.r, m.r, m.ein);
                         nc = 1;
-                    } else {
+                    } else if (DEG == 2) {
                         cls = quad_cert(m.q0, m.q1, m.q2, A, B, m.r, m.ein, nc);
+                    } else {
+                        cls = cubic_cert(m.q0, m.q1, m.q2, m.q3, A, B, m.r, m.ein, nc);
                     }
                     work += nc;
                     if (COUNT) c_certs += nc;
@@ -473,7 +557,7 @@ template <int DEG>
 __global__ __launch_bounds__(kBS) void k_sweep_redecide(SweepArgs a) {
     const uint32_t total = min(*a.unsure_count, a.unsure_cap);
     const int S = a.robot->n_spheres;
-    const int npc = a.n_ctrl - 2;
+    const int npc = a.n_ctrl - DEG;
     for (uint32_t i = blockIdx.x * kBS + threadIdx.x; i < total; i += gridDim.x * kBS) {
         const uint4 u = a.unsure[i];
         const uint32_t g = u.x;
@@ -481,7 +565,7 @@ __global__ __launch_bounds__(kBS) void k_sweep_redecide(SweepArgs a) {
         const uint32_t traj = DEG == 1 ? a.seg_traj[g] : g / (uint32_t) npc;
         uint32_t* flag = a.out_seg ? a.out_seg + g : a.out + traj;
         if (load_flag(flag)) continue;
-        double ctrl[9];
+        double ctrl[12];
         float r;
         if (DEG == 1) {
             const uint32_t w = a.seg_wp[g];
@@ -489,6 +573,10 @@ __global__ __launch_bounds__(kBS) void k_sweep_redecide(SweepArgs a) {
                 ctrl[c] = a.centers64[((size_t) w * S + s) * 3 + c];
                 ctrl[3 + c] = a.centers64[((size_t) (w + 1) * S + s) * 3 + c];
             }
+            r = a.radius[s];
+        } else if (DEG == 3) {
+            const int piece = (int) (g - traj * npc);
+            x64::cubic_piece(a.ctrl64 + (((size_t) traj * S + s) * a.n_ctrl + piece) * 3, ctrl);
             r = a.radius[s];
         } else {
             const int piece = (int) (g - traj * npc);
@@ -616,7 +704,8 @@ void launch_sweep(const SweepArgs& a, int degree, int stack, bool count, hipStre
         else sweep_t<D, 64>(a, count, s);
     };
     if (degree == 1) go(std::integral_constant<int, 1>{});
-    else go(std::integral_constant<int, 2>{});
+    else if (degree == 2) go(std::integral_constant<int, 2>{});
+    else go(std::integral_constant<int, 3>{});
 }
 
 }  // namespace rtcd
@@ -776,12 +865,13 @@ void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_tra
     sweep_end(ws, events);
 }
 
-// The fit weights of a quadratic check in ws.W64: the caller's W, or the default uniform-knot fit.
-// Uploaded only when they differ from the last ones, so a captured check never copies them.
-static void prepare_weights(Workspace& ws, uint32_t nc, uint32_t nw, const double* W) {
+// The fit weights of a quadratic or cubic check in ws.W64: the caller's W, or the default
+// uniform-knot fit of that degree.  Uploaded only when they differ from the last ones, so a captured
+// check never copies them.
+static void prepare_weights(Workspace& ws, uint32_t nc, uint32_t nw, const double* W, int degree = 2) {
     std::vector<double> Wh((size_t) nc * nw);
     if (W) std::copy(W, W + Wh.size(), Wh.begin());
-    else quadratic_weights((int) nc, (int) nw, Wh.data());
+    else curve_weights(degree, false, (int) nc, (int) nw, Wh.data());
     if (ws.W_host != Wh) {
         ws.W_host = std::move(Wh);  // kept alive in the workspace for the asynchronous copy
         ws.W64.upload(ws.W_host, ws.stream);
@@ -789,12 +879,13 @@ static void prepare_weights(Workspace& ws, uint32_t nc, uint32_t nw, const doubl
     }
 }
 
-void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
-                       uint32_t nc, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget, bool events) {
+void enqueue_curve(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, uint32_t n_traj, uint32_t nw,
+                   uint32_t nc, int degree, const double* W, uint32_t* d_out, uint32_t* d_seg, size_t budget,
+                   bool events) {
     const hipStream_t s = ws.stream;
     const int S = c->n_spheres;
-    const int npc = (int) nc - 2;
-    prepare_weights(ws, nc, nw, W);
+    const int npc = (int) nc - degree;
+    prepare_weights(ws, nc, nw, W, degree);
     sweep_begin(ws, events);
     fill_u32(d_out, n_traj, 0u, s);
     if (d_seg) fill_u32(d_seg, (size_t) n_traj * npc, 0u, s);
@@ -817,7 +908,7 @@ void enqueue_quadratic(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_
     a.P32 = ws.P32.p;
     sweep_work(c, ws, a, sweep_items(c, (size_t) a.n_seg, true), budget);
     const int depth = sc.edge_bvh.depth;
-    if (a.n_seg > 0) launch_sweep(a, 2, stack_tier(depth), ws.counting, s);
+    if (a.n_seg > 0) launch_sweep(a, degree, stack_tier(depth), ws.counting, s);
     sweep_end(ws, events);
 }
 
@@ -1003,24 +1094,28 @@ rtcd_status rtcd_check_linear(rtcd_ctx* ctx, uint32_t scene_id, const float* tra
     });
 }
 
-rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_wp, uint32_t n_traj,
-                                        uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
-                                        uint32_t* d_out_seg, void* stream) {
+// The B-spline checks of degree 2 and 3 (rtcd_check_quadratic[_device], rtcd_check_cubic[_device]).
+static rtcd_status check_curve_device(rtcd_ctx* ctx, int degree, uint32_t scene_id, const float* d_wp, uint32_t n_traj,
+                                      uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
+                                      uint32_t* d_out_seg, void* stream) {
     std::shared_ptr<SceneRes> sc;
     if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
-    if (n_ctrl < 3 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)
-        return fail(RTCD_ERR_INVALID_ARG, "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
+    if (n_ctrl < (uint32_t) degree + 1 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)
+        return fail(RTCD_ERR_INVALID_ARG, degree == 2 ? "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096"
+                                                      : "need 4 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
     if (n_traj && (!d_wp || !d_out)) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     return guarded([&]() -> rtcd_status {
         RTCD_HIP(hipSetDevice(ctx->cfg.device));
         if (n_traj == 0) return RTCD_OK;
         on_main(ctx, stream, [&](Workspace& ws) {
-            prepare_weights(ws, n_ctrl, n_wp, W);
+            prepare_weights(ws, n_ctrl, n_wp, W, degree);
             sweep_main(ctx, ws,
-                       [&]() { return sweep_key(ctx, ws, *sc, 2, d_wp, nullptr, n_traj, n_wp, n_ctrl, d_out, d_out_seg); },
+                       [&]() {
+                           return sweep_key(ctx, ws, *sc, degree, d_wp, nullptr, n_traj, n_wp, n_ctrl, d_out, d_out_seg);
+                       },
                        [&](bool ev) {
-                           enqueue_quadratic(ctx, ws, *sc, d_wp, n_traj, n_wp, n_ctrl, W, d_out, d_out_seg, 0, ev);
+                           enqueue_curve(ctx, ws, *sc, d_wp, n_traj, n_wp, n_ctrl, degree, W, d_out, d_out_seg, 0, ev);
                        });
         });
         hipStream_t st = stream ? (hipStream_t) stream : ctx->main.stream;
@@ -1031,20 +1126,21 @@ rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const 
         ctx->pending.active = true;
         ctx->pending.stream = st;
         ctx->pending.scene = sc;
-        ctx->pending.sweep_rerun = [ctx, scp, d_wp, n_traj, n_wp, n_ctrl, Wc, d_out, d_out_seg]() {
-            enqueue_quadratic(ctx, ctx->main, *scp, d_wp, n_traj, n_wp, n_ctrl, Wc.empty() ? nullptr : Wc.data(), d_out,
-                              d_out_seg, 0);
+        ctx->pending.sweep_rerun = [ctx, scp, d_wp, n_traj, n_wp, n_ctrl, degree, Wc, d_out, d_out_seg]() {
+            enqueue_curve(ctx, ctx->main, *scp, d_wp, n_traj, n_wp, n_ctrl, degree, Wc.empty() ? nullptr : Wc.data(),
+                          d_out, d_out_seg, 0);
         };
         return RTCD_OK;
     });
 }
 
-rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* wp, uint32_t n_traj, uint32_t n_wp,
-                                 uint32_t n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
+static rtcd_status check_curve_host(rtcd_ctx* ctx, int degree, uint32_t scene_id, const float* wp, uint32_t n_traj,
+                                    uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
     std::shared_ptr<SceneRes> sc;
     if (rtcd_status st = sweep_precheck(ctx, scene_id, &sc)) return st;
-    if (n_ctrl < 3 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)
-        return fail(RTCD_ERR_INVALID_ARG, "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
+    if (n_ctrl < (uint32_t) degree + 1 || n_wp < n_ctrl || n_ctrl > 64 || n_wp > 4096)
+        return fail(RTCD_ERR_INVALID_ARG, degree == 2 ? "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096"
+                                                      : "need 4 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
     if (n_traj == 0) return RTCD_OK;
     if (!wp || !out) return fail(RTCD_ERR_INVALID_ARG, "null buffer");
     if (!finite_all(wp, (size_t) n_traj * n_wp * ctx->robot.dof)) return fail(RTCD_ERR_INVALID_ARG, "non-finite joint value");
@@ -1054,17 +1150,21 @@ rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* 
         Workspace& b = ctx->main;
         b.counting = ctx->counting;
         const hipStream_t s = b.stream;
-        const size_t npc = n_ctrl - 2;
+        const size_t npc = n_ctrl - (uint32_t) degree;
         b.h_poses.upload(wp, (size_t) n_traj * n_wp * ctx->robot.dof, s);
         b.h_out.reserve(n_traj);
         b.h_seg.reserve(n_traj * npc);
         uint32_t* seg = out_seg ? b.h_seg.p : nullptr;
-        auto run = [&]() { enqueue_quadratic(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, seg, 0); };
-        prepare_weights(b, n_ctrl, n_wp, W);
+        auto run = [&]() {
+            enqueue_curve(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, degree, W, b.h_out.p, seg, 0);
+        };
+        prepare_weights(b, n_ctrl, n_wp, W, degree);
         sweep_main(ctx, b,
-                   [&]() { return sweep_key(ctx, b, *sc, 2, b.h_poses.p, nullptr, n_traj, n_wp, n_ctrl, b.h_out.p, seg); },
+                   [&]() {
+                       return sweep_key(ctx, b, *sc, degree, b.h_poses.p, nullptr, n_traj, n_wp, n_ctrl, b.h_out.p, seg);
+                   },
                    [&](bool ev) {
-                       enqueue_quadratic(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, W, b.h_out.p, seg, 0, ev);
+                       enqueue_curve(ctx, b, *sc, b.h_poses.p, n_traj, n_wp, n_ctrl, degree, W, b.h_out.p, seg, 0, ev);
                    });
         finish_sweep(ctx, b, &ctx->last, run);
         RTCD_HIP(hipMemcpyAsync(out, b.h_out.p, sizeof(uint32_t) * n_traj, hipMemcpyDeviceToHost, s));
@@ -1072,6 +1172,28 @@ rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* 
         RTCD_HIP(hipStreamSynchronize(s));
         return RTCD_OK;
     });
+}
+
+rtcd_status rtcd_check_quadratic_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_wp, uint32_t n_traj,
+                                        uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
+                                        uint32_t* d_out_seg, void* stream) {
+    return check_curve_device(ctx, 2, scene_id, d_wp, n_traj, n_wp, n_ctrl, W, d_out, d_out_seg, stream);
+}
+
+rtcd_status rtcd_check_quadratic(rtcd_ctx* ctx, uint32_t scene_id, const float* wp, uint32_t n_traj, uint32_t n_wp,
+                                 uint32_t n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
+    return check_curve_host(ctx, 2, scene_id, wp, n_traj, n_wp, n_ctrl, W, out, out_seg);
+}
+
+rtcd_status rtcd_check_cubic_device(rtcd_ctx* ctx, uint32_t scene_id, const float* d_wp, uint32_t n_traj,
+                                    uint32_t n_wp, uint32_t n_ctrl, const double* W, uint32_t* d_out,
+                                    uint32_t* d_out_seg, void* stream) {
+    return check_curve_device(ctx, 3, scene_id, d_wp, n_traj, n_wp, n_ctrl, W, d_out, d_out_seg, stream);
+}
+
+rtcd_status rtcd_check_cubic(rtcd_ctx* ctx, uint32_t scene_id, const float* wp, uint32_t n_traj, uint32_t n_wp,
+                             uint32_t n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
+    return check_curve_host(ctx, 3, scene_id, wp, n_traj, n_wp, n_ctrl, W, out, out_seg);
 }
 
 }  // extern "C"

@@ -105,6 +105,16 @@ class Context:
         check(lib().rtcd_check_quadratic(self.h, sid, ptr(wp), n, nwp, n_ctrl, ptr(Wd), ptr(out), ptr(seg)))
         return (out, seg) if with_segments else out
 
+    def check_cubic(self, sid, waypoints, n_ctrl=8, W=None, with_segments=False):
+        """Cubic B-spline sweeps (rtcd_check_cubic): n_ctrl - 3 uniform cubic pieces per trajectory."""
+        wp = _f32(waypoints)
+        n, nwp = wp.shape[0], wp.shape[1]
+        Wd = np.ascontiguousarray(W, dtype=np.float64) if W is not None else None
+        out = np.zeros(n, dtype=np.uint32)
+        seg = np.zeros((n, n_ctrl - 3), dtype=np.uint32) if with_segments else None
+        check(lib().rtcd_check_cubic(self.h, sid, ptr(wp), n, nwp, n_ctrl, ptr(Wd), ptr(out), ptr(seg)))
+        return (out, seg) if with_segments else out
+
     # -------------------------------------------------------------- checks (device buffers, torch tensors)
     @staticmethod
     def _stream(stream):
