@@ -7,8 +7,8 @@ Headline (one JSON line on stdout): dense two-way discrete check, poses/s on one
   A step is one blocking check of the whole batch with poses already resident in HBM.
 
 Also reported in the same line: per-direction rates, the plain per-query launch (one pose per call),
-linear (4,096 random walks of 8-64 waypoints) and quadratic (4,096 B-splines, 8 control points)
-trajectories/s on the shelf and dense scenes, the serving stream, the roofline of the dominant kernel
+linear (4,096 random walks of 8-64 waypoints), quadratic and cubic (4,096 B-splines, 8 control
+points) trajectories/s on the shelf and dense scenes, the serving stream, the roofline of the dominant kernel
 (the obstacle->robot narrow pass) and the CPU oracle baseline timed on a bounded sample of the same
 workload.
 
@@ -341,6 +341,13 @@ def main():
             ctx.check_quadratic(sid2, wp, 8)
         extra['quadratic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
         extra['quadratic_ms_device'] = round(ctx.stats()['last_ms'], 3)
+        # the same B-splines as uniform cubic curves (RTCD's CUBIC_CURVE robot), cubic uniform-knot fit
+        ctx.check_cubic(sid2, wp, 8)
+        t = time.perf_counter()
+        for _ in range(3):
+            ctx.check_cubic(sid2, wp, 8)
+        extra['cubic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
+        extra['cubic_ms_device'] = round(ctx.stats()['last_ms'], 3)
         # the same trajectory sets on the dense scene
         ctx.check_linear(sid, traj, offs)
         t = time.perf_counter()
@@ -352,6 +359,11 @@ def main():
         for _ in range(3):
             ctx.check_quadratic(sid, wp, 8)
         extra['dense_quadratic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
+        ctx.check_cubic(sid, wp, 8)
+        t = time.perf_counter()
+        for _ in range(3):
+            ctx.check_cubic(sid, wp, 8)
+        extra['dense_cubic_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
         log(f'[bench] extra sweeps { {k: v for k, v in extra.items() if "traj" in k or "ms_device" in k} }')
         # the plain obstacle->robot case: 1,024 uniform poses (seed 0) against the 8 boxes (96 triangles),
         # one batched call versus one blocking call per pose

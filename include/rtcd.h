@@ -195,7 +195,7 @@ rtcd_status rtcd_curve_weights(uint32_t degree, uint32_t end_points, uint32_t n_
 
 /* ---- Serving: continuous batching over resident scenes (no RTCD counterpart; RTCD's closest
  * analogue is the NSTREAM ThreadPool of CollisionDetector::check, CollisionDetector.h:396-433). ---- */
-enum { RTCD_REQ_DISCRETE = 1, RTCD_REQ_LINEAR = 2, RTCD_REQ_QUADRATIC = 3 };
+enum { RTCD_REQ_DISCRETE = 1, RTCD_REQ_LINEAR = 2, RTCD_REQ_QUADRATIC = 3, RTCD_REQ_CUBIC = 4 };
 typedef struct {
     int32_t kind;            /* RTCD_REQ_* */
     int32_t direction;       /* discrete: RTCD_O2R / RTCD_R2O / RTCD_TWO_WAY */
@@ -203,7 +203,7 @@ typedef struct {
     uint32_t n;              /* poses or trajectories */
     const float* input;      /* host: poses [n][dof]; linear waypoints [offsets[n]][dof]; quadratic [n][n_wp][dof] */
     const uint32_t* offsets; /* linear: [n+1] */
-    uint32_t n_wp, n_ctrl;   /* quadratic (default fit weights) */
+    uint32_t n_wp, n_ctrl;   /* quadratic / cubic (default fit weights) */
     uint32_t* out;           /* host [n]; written before the request completes */
 } rtcd_request;
 typedef struct {

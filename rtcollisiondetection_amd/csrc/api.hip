@@ -896,7 +896,7 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
         ctx->tiny_batch = (uint32_t) value;
         return RTCD_OK;
     case RTCD_PARAM_SERVE_LANE_KINDS:
-        if (value < 1 || value > 7) return fail(RTCD_ERR_INVALID_ARG, "lane kinds must be a non-empty mask of bits 0..2");
+        if (value < 1 || value > 15) return fail(RTCD_ERR_INVALID_ARG, "lane kinds must be a non-empty mask of bits 0..3");
         ctx->serve_lane_kinds = (uint32_t) value;
         return RTCD_OK;
     case RTCD_PARAM_SERVE_CHUNK_DISCRETE:

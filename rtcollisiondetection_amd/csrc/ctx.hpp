@@ -258,7 +258,7 @@ struct rtcd_ctx {
     // the rest; requests are split into units of at most serve_chunk_discrete poses or
     // serve_chunk_sweep trajectories
     uint32_t serve_lane_max = 1024, serve_lane_cus = 0, serve_chunk_discrete = 4096, serve_chunk_sweep = 1024;
-    uint32_t serve_lane_kinds = 7;  // request kinds the lane takes: bit (kind - 1)
+    uint32_t serve_lane_kinds = 15;  // request kinds the lane takes: bit (kind - 1)
     uint32_t tiny_batch = 4;  // discrete batches of at most this many poses run 2 broad passes, 1 narrow pass
     // serving loop
     ServeState* serve = nullptr;

@@ -109,7 +109,7 @@ struct ServeState {
     size_t budget = 0, in_use = 0;
     uint32_t chunk_discrete = 4096, chunk_sweep = 1024;
     uint32_t lane_max = 0;   // slot 0 takes only requests of at most lane_max queries (0: any)
-    uint32_t lane_kinds = 7; // ... and of these kinds (bit kind - 1)
+    uint32_t lane_kinds = 15; // ... and of these kinds (bit kind - 1)
     uint32_t lane_cus = 0;   // CUs of slot 0's stream mask (0: no masks)
     std::vector<double> lat, qlat;
     uint64_t completed = 0, failed = 0;
@@ -168,7 +168,8 @@ void launch(ServeState* S, Slot& sl, const Batch& b) {
         RTCD_HIP(hipMemcpyAsync(ws.h_offsets.p, sl.offs.p, sizeof(uint32_t) * (b.n + 1), hipMemcpyHostToDevice, s));
         enqueue_linear(c, ws, sc, ws.h_poses.p, ws.h_offsets.p, b.n, b.nw, ws.h_out.p, nullptr, 0);
     } else {
-        enqueue_quadratic(c, ws, sc, ws.h_poses.p, b.n, r.n_wp, r.n_ctrl, nullptr, ws.h_out.p, nullptr, 0);
+        enqueue_curve(c, ws, sc, ws.h_poses.p, b.n, r.n_wp, r.n_ctrl, r.kind == RTCD_REQ_CUBIC ? 3 : 2, nullptr,
+                      ws.h_out.p, nullptr, 0);
     }
     sl.out.reserve(b.n);
     sl.sv.reserve(4);
@@ -199,8 +200,8 @@ void complete(ServeState* S, Slot& sl) {
                     enqueue_linear(c, sl.ws, sc, sl.ws.h_poses.p, sl.ws.h_offsets.p, b.n, b.nw, sl.ws.h_out.p, nullptr,
                                    0);
                 else
-                    enqueue_quadratic(c, sl.ws, sc, sl.ws.h_poses.p, b.n, r.n_wp, r.n_ctrl, nullptr, sl.ws.h_out.p,
-                                      nullptr, 0);
+                    enqueue_curve(c, sl.ws, sc, sl.ws.h_poses.p, b.n, r.n_wp, r.n_ctrl,
+                                  r.kind == RTCD_REQ_CUBIC ? 3 : 2, nullptr, sl.ws.h_out.p, nullptr, 0);
             };
             finish_sweep(c, sl.ws, nullptr, rerun);
             RTCD_HIP(hipMemcpyAsync(sl.out.p, sl.ws.h_out.p, sizeof(uint32_t) * b.n, hipMemcpyDeviceToHost,
@@ -212,12 +213,12 @@ void complete(ServeState* S, Slot& sl) {
         std::memcpy(b.units[k].q->r.out + b.units[k].i0, sl.out.p + b.first[k], sizeof(uint32_t) * b.units[k].n);
 }
 
-// Units that can share one launch: same kind, scene, direction and quadratic shape.
+// Units that can share one launch: same kind, scene, direction and B-spline shape.
 bool same_shape(const Unit& a, const Unit& b) {
     const rtcd_request &x = a.q->r, &y = b.q->r;
     return x.kind == y.kind && a.q->scene == b.q->scene &&
            (x.kind != RTCD_REQ_DISCRETE || x.direction == y.direction) &&
-           (x.kind != RTCD_REQ_QUADRATIC || (x.n_wp == y.n_wp && x.n_ctrl == y.n_ctrl));
+           ((x.kind != RTCD_REQ_QUADRATIC && x.kind != RTCD_REQ_CUBIC) || (x.n_wp == y.n_wp && x.n_ctrl == y.n_ctrl));
 }
 
 // Whether the small-request lane (slot 0) may take unit u.
@@ -440,12 +441,14 @@ rtcd_status rtcd_submit(rtcd_ctx* ctx, const rtcd_request* r, uint64_t* id) {
     if (!ctx || !r || !id) return fail(RTCD_ERR_INVALID_ARG, "null argument");
     ServeState* S = ctx->serve;
     if (!S) return fail(RTCD_ERR_NOT_READY, "serving loop not running");
-    if (r->kind < RTCD_REQ_DISCRETE || r->kind > RTCD_REQ_QUADRATIC) return fail(RTCD_ERR_INVALID_ARG, "bad kind");
+    if (r->kind < RTCD_REQ_DISCRETE || r->kind > RTCD_REQ_CUBIC) return fail(RTCD_ERR_INVALID_ARG, "bad kind");
     if (r->n == 0 || !r->input || !r->out) return fail(RTCD_ERR_INVALID_ARG, "empty request or null buffer");
     if (r->kind == RTCD_REQ_DISCRETE && (r->direction < 1 || r->direction > 3))
         return fail(RTCD_ERR_INVALID_ARG, "direction must be 1, 2 or 3");
     if (r->kind == RTCD_REQ_QUADRATIC && (r->n_ctrl < 3 || r->n_wp < r->n_ctrl || r->n_ctrl > 64 || r->n_wp > 4096))
         return fail(RTCD_ERR_INVALID_ARG, "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
+    if (r->kind == RTCD_REQ_CUBIC && (r->n_ctrl < 4 || r->n_wp < r->n_ctrl || r->n_ctrl > 64 || r->n_wp > 4096))
+        return fail(RTCD_ERR_INVALID_ARG, "need 4 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
     uint32_t nw = r->n;
     if (r->kind == RTCD_REQ_LINEAR) {
         if (!r->offsets || r->offsets[0] != 0) return fail(RTCD_ERR_INVALID_ARG, "offsets must start at 0");
@@ -453,7 +456,7 @@ rtcd_status rtcd_submit(rtcd_ctx* ctx, const rtcd_request* r, uint64_t* id) {
             if (r->offsets[i + 1] <= r->offsets[i])
                 return fail(RTCD_ERR_INVALID_ARG, "every trajectory needs at least one waypoint");
         nw = r->offsets[r->n];
-    } else if (r->kind == RTCD_REQ_QUADRATIC) {
+    } else if (r->kind == RTCD_REQ_QUADRATIC || r->kind == RTCD_REQ_CUBIC) {
         nw = r->n * r->n_wp;
     }
     if (!finite_all(r->input, (size_t) nw * ctx->robot.dof)) return fail(RTCD_ERR_INVALID_ARG, "non-finite joint value");

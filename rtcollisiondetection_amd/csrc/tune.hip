@@ -41,7 +41,8 @@ struct Runner {
             run();
             finish_sweep(c, ws, &st, run);
         } else {
-            auto run = [&]() { enqueue_quadratic(c, ws, *sc, in.p, r.n, r.n_wp, r.n_ctrl, nullptr, out.p, nullptr, 0); };
+            const int deg = r.kind == RTCD_REQ_CUBIC ? 3 : 2;
+            auto run = [&]() { enqueue_curve(c, ws, *sc, in.p, r.n, r.n_wp, r.n_ctrl, deg, nullptr, out.p, nullptr, 0); };
             run();
             finish_sweep(c, ws, &st, run);
         }
@@ -109,13 +110,15 @@ extern "C" {
 
 rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, rtcd_tune_report* rep) {
     if (!ctx || !r || !rep) return fail(RTCD_ERR_INVALID_ARG, "null argument");
-    if (r->kind < RTCD_REQ_DISCRETE || r->kind > RTCD_REQ_QUADRATIC) return fail(RTCD_ERR_INVALID_ARG, "bad kind");
+    if (r->kind < RTCD_REQ_DISCRETE || r->kind > RTCD_REQ_CUBIC) return fail(RTCD_ERR_INVALID_ARG, "bad kind");
     if (r->n == 0 || !r->input) return fail(RTCD_ERR_INVALID_ARG, "empty sample");
     if (r->kind == RTCD_REQ_DISCRETE && (r->direction < 1 || r->direction > 3))
         return fail(RTCD_ERR_INVALID_ARG, "direction must be 1, 2 or 3");
     if (r->kind == RTCD_REQ_LINEAR && !r->offsets) return fail(RTCD_ERR_INVALID_ARG, "linear sample needs offsets");
     if (r->kind == RTCD_REQ_QUADRATIC && (r->n_ctrl < 3 || r->n_wp < r->n_ctrl || r->n_ctrl > 64 || r->n_wp > 4096))
         return fail(RTCD_ERR_INVALID_ARG, "need 3 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
+    if (r->kind == RTCD_REQ_CUBIC && (r->n_ctrl < 4 || r->n_wp < r->n_ctrl || r->n_ctrl > 64 || r->n_wp > 4096))
+        return fail(RTCD_ERR_INVALID_ARG, "need 4 <= n_ctrl <= 64 and n_ctrl <= n_wp <= 4096");
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     if (!ctx->robot_loaded) return fail(RTCD_ERR_NOT_READY, "robot not loaded");
     if (ctx->serve) return fail(RTCD_ERR_NOT_READY, "stop the serving loop before tuning");
@@ -182,7 +185,7 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
             n_in = (size_t) R.nw * dof;
             R.offs.upload(o, r->n + 1, s);
         } else {
-            n_in = (size_t) r->n * (r->kind == RTCD_REQ_QUADRATIC ? r->n_wp : 1) * dof;
+            n_in = (size_t) r->n * (r->kind >= RTCD_REQ_QUADRATIC ? r->n_wp : 1) * dof;
         }
         if (!finite_all(r->input, n_in)) return fail(RTCD_ERR_INVALID_ARG, "non-finite joint value");
         R.in.upload(r->input, n_in, s);

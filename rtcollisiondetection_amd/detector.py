@@ -6,8 +6,8 @@ CollisionDetector.h:18-156) and of the robot wrapper's ``prepareUpdateData``
 (RTCD/CollisionDetector/CCRobotWrapper.h:39-46): poses are staged with ``prepare_update_data``,
 ``set_n_check`` sets the query count, ``check`` runs, ``download_result`` returns one uint32 per
 query (1 = collision).  The link type selects the mode as in RTCD (RTCD/Meshes/meshTypes.h:37-46):
-MESH = obstacle->robot, RAY_ONLINE = robot->obstacle, TWO_WAY = both, LINEAR_CURVE and
-QUADRATIC_CURVE = swept spheres.
+MESH = obstacle->robot, RAY_ONLINE = robot->obstacle, TWO_WAY = both, LINEAR_CURVE,
+QUADRATIC_CURVE and CUBIC_CURVE = swept spheres.
 """
 import ctypes as C
 
@@ -16,8 +16,8 @@ import numpy as np
 from . import _lib
 from ._lib import check, lib, ptr
 
-MESH, RAY_ONLINE, TWO_WAY, LINEAR_CURVE, QUADRATIC_CURVE = 'MESH', 'RAY_ONLINE', 'TWO_WAY', 'LINEAR_CURVE', \
-    'QUADRATIC_CURVE'
+MESH, RAY_ONLINE, TWO_WAY, LINEAR_CURVE, QUADRATIC_CURVE, CUBIC_CURVE = 'MESH', 'RAY_ONLINE', 'TWO_WAY', \
+    'LINEAR_CURVE', 'QUADRATIC_CURVE', 'CUBIC_CURVE'
 _DIR = {MESH: _lib.O2R, RAY_ONLINE: _lib.R2O, TWO_WAY: _lib.TWO_WAY}
 
 
@@ -209,7 +209,7 @@ class CollisionDetector:
 
     def prepare_update_data(self, poses, offsets=None):
         """Discrete: poses [n, dof].  Linear: waypoints [sum L, dof] with offsets [n+1].
-        Quadratic: waypoints [n, n_wp, dof]."""
+        Quadratic and cubic: waypoints [n, n_wp, dof]."""
         self._poses = _f32(poses)
         self._offsets = offsets
         self._n = len(offsets) - 1 if offsets is not None else len(self._poses)
@@ -227,6 +227,8 @@ class CollisionDetector:
             self._result = self.ctx.check_linear(self.scene, self._poses, offs)
         elif self.type == QUADRATIC_CURVE:
             self._result = self.ctx.check_quadratic(self.scene, self._poses[:self._n], self.n_ctrl)
+        elif self.type == CUBIC_CURVE:
+            self._result = self.ctx.check_cubic(self.scene, self._poses[:self._n], self.n_ctrl)
         else:
             raise ValueError(f'unknown link type {self.type}')
 

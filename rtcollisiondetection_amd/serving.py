@@ -20,7 +20,7 @@ from . import samplers
 
 @dataclass
 class Request:
-    kind: int                # _lib.REQ_DISCRETE / REQ_LINEAR / REQ_QUADRATIC
+    kind: int                # _lib.REQ_DISCRETE / REQ_LINEAR / REQ_QUADRATIC / REQ_CUBIC
     scene: int
     n: int
     data: np.ndarray         # poses [n, dof] / waypoints [sum L, dof] / [n, n_wp, dof]
@@ -129,4 +129,6 @@ def run_alone(ctx, r):
         return ctx.check_discrete(r.scene, r.data, r.direction)
     if r.kind == _lib.REQ_LINEAR:
         return ctx.check_linear(r.scene, r.data, r.offsets)
+    if r.kind == _lib.REQ_CUBIC:
+        return ctx.check_cubic(r.scene, r.data, r.n_ctrl)
     return ctx.check_quadratic(r.scene, r.data, r.n_ctrl)

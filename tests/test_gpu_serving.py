@@ -39,6 +39,27 @@ def test_mixed_stream_matches_alone(env, panda):
     print(f"served {len(reqs)} requests, p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms")
 
 
+def test_cubic_requests_in_the_mix(env, panda):
+    """Cubic B-spline requests (RTCD_REQ_CUBIC) served beside quadratic ones of the same shape."""
+    ctx, ids = env
+    from rtcollisiondetection_amd.serving import Request
+    from rtcollisiondetection_amd import samplers
+    reqs = []
+    for i, n in enumerate((3, 40, 7, 64, 1)):
+        wp, _ = samplers.bspline_trajectories(n, panda.lower, panda.upper, n_ctrl=8, n_wp=16, seed=40 + i)
+        kind = _lib.REQ_CUBIC if i % 2 == 0 else _lib.REQ_QUADRATIC
+        reqs.append(Request(kind, ids[i % 2], n, wp, n_wp=16, n_ctrl=8))
+    srv = Server(ctx, slots=2)
+    try:
+        run_stream(srv, reqs, realtime=False)
+        st = srv.stats()
+    finally:
+        srv.stop()
+    assert st['completed'] == len(reqs) and st['failed'] == 0
+    for r in reqs:
+        assert np.array_equal(r.out, run_alone(ctx, r)), (r.kind, r.n)
+
+
 def test_admission_rejects_oversized_and_bad_requests(env, panda):
     ctx, ids = env
     from rtcollisiondetection_amd.serving import Request
