@@ -30,9 +30,10 @@ namespace rtcd {
 // leaf and link instead of a chain of dependent node fetches); large ones near the root.
 constexpr size_t kFrontierMax = (size_t) 1 << 16;
 static void build_frontiers(const std::vector<BNode>& nodes, std::vector<FEntry>& all,
-                            std::vector<std::pair<int, int>>& levels) {
+                            std::vector<std::pair<int, int>>& levels, bool& leaves_only) {
     all.clear();
     levels.clear();
+    leaves_only = false;
     if (nodes.empty()) return;
     auto child = [&](const BNode& nd, int k) {
         FEntry e;
@@ -63,6 +64,7 @@ static void build_frontiers(const std::vector<BNode>& nodes, std::vector<FEntry>
                 if (c.c >= 0) nxt.push_back(c);
             }
         }
+        if (!inner) leaves_only = true;
         if (!inner || nxt.size() > kFrontierMax) break;
         cur.swap(nxt);
     }
@@ -172,6 +174,8 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     // robot->obstacle pass)
     const bool tiny = n <= c->tiny_batch;
     a.n_passes = tiny ? (int) std::min<uint32_t>(c->broad_passes, 2u) : (int) c->broad_passes;
+    // starting from the all-leaves level, pass 0 only tests boxes: nothing is handed off
+    if (sc.front_leaves && a.F > 0 && foff == sc.front_levels.back().first) a.n_passes = 1;
     a.nspill_cap = (uint32_t) std::min<size_t>(ws.nspill.n / 2, 0x7fffffffu);
     a.nspill[0] = ws.nspill.p;
     a.nspill[1] = ws.nspill.p + a.nspill_cap;
@@ -184,7 +188,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     mark(1);
     if (a.F > 0) launch_broad(a, broad_stack, ws.counting, s);
     mark(2);
-    launch_gather(a, s);
+    launch_gather(a, tiny, s);
     mark(3);
     // obstacle->robot first: it is cheaper and flags most colliding poses, which the robot->obstacle
     // pass then skips
@@ -607,7 +611,7 @@ static std::shared_ptr<SceneRes> build_scene(rtcd_ctx* ctx, const float* verts, 
     {
         std::vector<BNode> hn = sc->tri_bvh.nodes.download(sc->tri_bvh.n_nodes, s);
         std::vector<FEntry> fr;
-        build_frontiers(hn, fr, sc->front_levels);
+        build_frontiers(hn, fr, sc->front_levels, sc->front_leaves);
         sc->front.alloc(std::max<size_t>(fr.size(), 1));
         if (!fr.empty()) sc->front.upload(fr, s);
     }

@@ -56,7 +56,8 @@ struct SceneRes {
     std::vector<int32_t> h_tris;
     BvhOut tri_bvh, edge_bvh;
     DevBuf<FEntry> front;                         // frontier levels of the triangle BVH, concatenated
-    std::vector<std::pair<int, int>> front_levels; // (offset, size): sizes 2, 4, ... up to 1024
+    std::vector<std::pair<int, int>> front_levels; // (offset, size): sizes 2, 4, ... up to 65,536
+    bool front_leaves = false;  // the deepest level holds only leaves (a walk from it never descends)
     DevBuf<Tri> tris;
     DevBuf<Edge> edges;
     int n_tri = 0, n_edge = 0;

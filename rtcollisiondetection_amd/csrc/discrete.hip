@@ -59,11 +59,12 @@ __device__ __forceinline__ LinkTri link_tri(const Tri& t, const float4& r0, cons
 }
 
 // ------------------------------------------------------------------------------------------ FK
-// Row r of the world AABB of link box L under the link transform T (rounded outwards).
-__device__ __forceinline__ void link_box_row(const double* T, const LinkInfo& L, int r, float& lo, float& hi) {
-    const double c0 = L.c[0], c1 = L.c[1], c2 = L.c[2];
-    const double h0 = 0.5 * ((double) L.hi[0] - L.lo[0]), h1 = 0.5 * ((double) L.hi[1] - L.lo[1]),
-                 h2 = 0.5 * ((double) L.hi[2] - L.lo[2]);
+// Row r of the world AABB of a link box (b = centre[3], lo[3], hi[3] of LinkInfo) under the link
+// transform T (rounded outwards).
+__device__ __forceinline__ void link_box_row(const double* T, const float* b, int r, float& lo, float& hi) {
+    const double c0 = b[0], c1 = b[1], c2 = b[2];
+    const double h0 = 0.5 * ((double) b[6] - b[3]), h1 = 0.5 * ((double) b[7] - b[4]),
+                 h2 = 0.5 * ((double) b[8] - b[5]);
     const double wc = T[r * 4 + 0] * c0 + T[r * 4 + 1] * c1 + T[r * 4 + 2] * c2 + T[r * 4 + 3];
     const double wh = fabs(T[r * 4 + 0]) * h0 + fabs(T[r * 4 + 1]) * h1 + fabs(T[r * 4 + 2]) * h2;
     const double pad = 1e-6 * (1.0 + fabs(wc) + wh);
@@ -112,7 +113,9 @@ __global__ __launch_bounds__(kBS) void k_fk(const RobotDev* __restrict__ R, cons
             Ti[pl * 3 + j] = make_float4((float) T[0 * 4 + j], (float) T[1 * 4 + j], (float) T[2 * 4 + j],
                                          (float) T[j * 4 + 3]);
         float lo[3], hi[3];
-        for (int r = 0; r < 3; ++r) link_box_row(T, R->link[l], r, lo[r], hi[r]);
+        const LinkInfo& L = R->link[l];
+        const float b[9] = {L.c[0], L.c[1], L.c[2], L.lo[0], L.lo[1], L.lo[2], L.hi[0], L.hi[1], L.hi[2]};
+        for (int r = 0; r < 3; ++r) link_box_row(T, b, r, lo[r], hi[r]);
         wbox[pl * 2] = make_float4(lo[0], lo[1], lo[2], 0.f);
         wbox[pl * 2 + 1] = make_float4(hi[0], hi[1], hi[2], 0.f);
     }
@@ -126,8 +129,9 @@ __global__ __launch_bounds__(kBS) void k_fk(const RobotDev* __restrict__ R, cons
 __global__ __launch_bounds__(kBS) void k_fk_wave(const RobotDev* __restrict__ R, const float* __restrict__ poses,
                                                 int n, double* __restrict__ T64, float4* __restrict__ Ti,
                                                 float4* __restrict__ wbox, FkClear z) {
-    // the joint masks in LDS: the chain then waits on LDS, not on a global load per joint
-    __shared__ float cm[12 * kMaxDof], sm[12 * kMaxDof], om[12 * kMaxDof];
+    // the joint masks, the base and the link boxes in LDS: the chain then waits on LDS, not on a global
+    // load per joint and link
+    __shared__ float cm[12 * kMaxDof], sm[12 * kMaxDof], om[12 * kMaxDof], bs[12], lb[kMaxLinks][9];
     const int lane = threadIdx.x & 63;
     const int p = blockIdx.x * (kBS / 64) + (threadIdx.x >> 6);  // uniform per wave
     fk_clear(z, p, lane == 0);
@@ -137,13 +141,19 @@ __global__ __launch_bounds__(kBS) void k_fk_wave(const RobotDev* __restrict__ R,
         sm[i] = R->smask[i];
         om[i] = R->omask[i];
     }
+    if (threadIdx.x < 12) bs[threadIdx.x] = R->base[threadIdx.x];
+    for (int i = threadIdx.x; i < 9 * NL; i += kBS) {
+        const LinkInfo& L = R->link[i / 9];
+        const int k = i % 9;
+        lb[i / 9][k] = k < 3 ? L.c[k] : k < 6 ? L.lo[k - 3] : L.hi[k - 6];
+    }
     __syncthreads();
     if (p >= n) return;
     double sj = 0.0, cj = 0.0;
     if (lane < dof) x64::sincos((double) poses[(size_t) p * dof + lane], &sj, &cj);
     const int k = lane < 12 ? lane : 0;  // lanes 12.. shadow entry 0
     const int r = k >> 2, cc = k & 3;
-    double T = (double) R->base[k];
+    double T = (double) bs[k];
     for (int l = 0; l < NL; ++l) {
         if (l > 0) {
             const int i = l - 1;
@@ -170,7 +180,7 @@ __global__ __launch_bounds__(kBS) void k_fk_wave(const RobotDev* __restrict__ R,
         if (lane < 3) {
             Ti[pl * 3 + lane] = make_float4((float) Tf[0 * 4 + lane], (float) Tf[1 * 4 + lane], (float) Tf[2 * 4 + lane],
                                             (float) Tf[lane * 4 + 3]);
-            link_box_row(Tf, R->link[l], lane, lo, hi);
+            link_box_row(Tf, lb[l], lane, lo, hi);
         }
         const float lo1 = __shfl(lo, 1, 64), lo2 = __shfl(lo, 2, 64), hi1 = __shfl(hi, 1, 64), hi2 = __shfl(hi, 2, 64);
         if (lane == 0) {
@@ -366,6 +376,52 @@ __global__ __launch_bounds__(kBS) void k_seg_scan(const uint32_t* __restrict__ s
     for (int k = 0; k < per; ++k) {
         seg_pre[threadIdx.x * per + k] = run;
         run += loc[k];
+    }
+}
+
+// Small batches: both steps in one launch.  Every block builds the whole prefix in LDS (block 0 also
+// stores it), then the blocks copy the records grid-stride, each finding its segment by binary search.
+__global__ __launch_bounds__(kBS) void k_seg_scan_gather(const uint2* __restrict__ cand,
+                                                        const uint32_t* __restrict__ seg_count, uint32_t seg_cap,
+                                                        uint32_t* __restrict__ seg_pre, uint2* __restrict__ flat) {
+    constexpr int per = kSeg / kBS;
+    __shared__ uint32_t pre[kSeg + 1];
+    __shared__ uint32_t part[kBS / 64];
+    uint32_t loc[per], acc = 0;
+    for (int k = 0; k < per; ++k) {
+        loc[k] = min(seg_count[(size_t) (threadIdx.x * per + k) * kSegStride], seg_cap);
+        acc += loc[k];
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = acc;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    if (lane == 63) part[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < kBS / 64; ++w) {
+        before += w < wave ? part[w] : 0u;
+        all += part[w];
+    }
+    uint32_t run = before + inc - acc;
+    for (int k = 0; k < per; ++k) {
+        pre[threadIdx.x * per + k] = run;
+        run += loc[k];
+    }
+    if (threadIdx.x == 0) pre[kSeg] = all;
+    __syncthreads();
+    if (blockIdx.x == 0)
+        for (int b = threadIdx.x; b <= kSeg; b += kBS) seg_pre[b] = pre[b];
+    for (uint32_t i = blockIdx.x * kBS + threadIdx.x; i < all; i += gridDim.x * kBS) {
+        int lo = 0, hi = kSeg;  // the segment b with pre[b] <= i < pre[b + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (pre[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        flat[i] = cand[(size_t) lo * seg_cap + (i - pre[lo])];
     }
 }
 
@@ -1053,7 +1109,13 @@ static void narrow_dir(const DiscreteArgs& a, int stack, bool count, hipStream_t
     else throw std::runtime_error("BVH deeper than 64 levels");
 }
 
-void launch_gather(const DiscreteArgs& a, hipStream_t s) {
+void launch_gather(const DiscreteArgs& a, bool small, hipStream_t s) {
+    if (small) {
+        hipLaunchKernelGGL(k_seg_scan_gather, dim3(16), dim3(kBS), 0, s, a.cand, a.seg_count, a.seg_cap, a.seg_pre,
+                           a.flat);
+        RTCD_LAUNCH_CHECK();
+        return;
+    }
     hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kBS), 0, s, a.seg_count, a.seg_cap, a.seg_pre);
     RTCD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_seg_gather, dim3(kSeg), dim3(kBS), 0, s, a.cand, a.seg_count, a.seg_cap, a.seg_pre, a.flat);

@@ -92,8 +92,9 @@ void launch_fk(const RobotDev* robot, int dof, int n_links, const float* poses, 
                float4* wbox, const FkClear& z, hipStream_t s);
 // Broad phase: one pass appending leaf candidates (wave-aggregated atomics).
 void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
-// Gathers the candidate segments, in order, into one flat array (a.flat, a.seg_pre).
-void launch_gather(const DiscreteArgs& a, hipStream_t s);
+// Gathers the candidate segments, in order, into one flat array (a.flat, a.seg_pre).  `small`: one
+// launch that scans and copies (for batches with few records).
+void launch_gather(const DiscreteArgs& a, bool small, hipStream_t s);
 // Narrow phase, one grid-stride pass in direction `dir` (1: obstacle->robot, 2: robot->obstacle)
 // over the gathered candidates.  `stack` covers the robot BVH depth.
 void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStream_t s);
