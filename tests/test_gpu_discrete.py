@@ -183,3 +183,34 @@ def test_tiny_batches_fewer_passes(ctx, panda, orc_panda):
     finally:
         ctx.set_param('tiny_batch', 4)
         ctx.destroy_scene(sid)
+
+
+def test_pinned_small_batches_and_graph_replays(ctx, panda, orc_panda):
+    """Host batches of at most 1,024 poses go through pinned host memory (poses read by the pose kernel,
+    flags and status words written back by the last kernel); larger ones through device copies.  Sizes
+    around the switch, each called three times (eager, captured, replayed), equal the oracle and the
+    device entry point; batches alternating in size keep their flags apart."""
+    import torch
+    sc = scenes.scene_boxes8()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    m = O.scene_mesh(sc)
+    q = samplers.uniform_poses(1100, panda.lower, panda.upper, seed=91)
+    try:
+        for n in (1, 1023, 1024, 1025):
+            ref = O.check_discrete(orc_panda, m, q[:n], 3)
+            for k in range(3):
+                got = ctx.check_discrete(sid, q[:n], 3)
+                assert _mismatch(got, ref) == 0, (n, k)
+            assert ctx.stats()['graph_replayed'] == 1, n
+            dq = torch.from_numpy(np.ascontiguousarray(q[:n])).cuda()
+            dout = torch.zeros(n, dtype=torch.int32, device='cuda')
+            ctx.check_discrete_device(sid, dq, dout, 3)
+            ctx.sync()
+            assert np.array_equal(dout.cpu().numpy().astype(np.uint32), ref), n
+        a_ref = O.check_discrete(orc_panda, m, q[:700], 1)
+        b_ref = O.check_discrete(orc_panda, m, q[700:703], 1)
+        for _ in range(2):
+            assert _mismatch(ctx.check_discrete(sid, q[:700], 1), a_ref) == 0
+            assert _mismatch(ctx.check_discrete(sid, q[700:703], 1), b_ref) == 0
+    finally:
+        ctx.destroy_scene(sid)
