@@ -178,6 +178,9 @@ def main():
     ap.add_argument('--tune', action='store_true',
                     help='run the library autotuner on the workload first (untimed) and time the tuned shapes')
     ap.add_argument('--poses', type=int, default=65536)
+    ap.add_argument('--sweeps-only', action='store_true',
+                    help='only the shelf sweeps (4,096 trajectories, linear / quadratic / cubic, 3 calls each), '
+                         'for kernel profiles of the sweep kernels')
     ap.add_argument('--copy-calibration', action='store_true',
                     help='only run a plain 256 MiB device-to-device copy 5 times (reference for HBM counters)')
     args = ap.parse_args()
@@ -201,6 +204,23 @@ def main():
     rb = robots.load_panda()
     ctx = Context()
     ctx.load_robot(rb)
+    if args.sweeps_only:
+        shelf = scenes.scene_shelf()
+        sid2 = ctx.create_scene(shelf.verts, shelf.tris)
+        traj, offs = samplers.random_walk_trajectories(4096, rb.lower, rb.upper, lengths=(8, 64), seed=4)
+        wp, _ = samplers.bspline_trajectories(4096, rb.lower, rb.upper, n_ctrl=8, n_wp=16, seed=3)
+        rates = {}
+        for name, f in (('linear', lambda: ctx.check_linear(sid2, traj, offs)),
+                        ('quadratic', lambda: ctx.check_quadratic(sid2, wp, 8)),
+                        ('cubic', lambda: ctx.check_cubic(sid2, wp, 8))):
+            f()
+            t = time.perf_counter()
+            for _ in range(3):
+                f()
+            rates[f'{name}_traj_per_s'] = round(4096 * 3 / (time.perf_counter() - t), 1)
+            log(f'[bench] shelf {name}: {rates[f"{name}_traj_per_s"]} trajectories/s')
+        print(json.dumps({'workload': 'shelf sweeps, 4,096 trajectories', **rates}))
+        return
     dense = scenes.scene_dense()
     sid = ctx.create_scene(dense.verts, dense.tris)
     n = args.poses

@@ -543,6 +543,15 @@ static rtcd_status load_robot_impl(rtcd_ctx* ctx, const rtcd_robot_desc* d) {
     std::vector<int32_t> sl(d->sphere_link, d->sphere_link + d->n_spheres);
     ctx->r_sphere_link.alloc(std::max<size_t>(sl.size(), 1));
     ctx->r_sphere_link.upload(sl, s);
+    std::vector<int32_t> csr((size_t) NL + 1 + d->n_spheres, 0);
+    for (uint32_t i = 0; i < d->n_spheres; ++i) ++csr[(size_t) sl[i] + 1];
+    for (int l = 0; l < NL; ++l) csr[(size_t) l + 1] += csr[l];
+    {
+        std::vector<int32_t> at(csr.begin(), csr.begin() + NL);
+        for (uint32_t i = 0; i < d->n_spheres; ++i) csr[(size_t) NL + 1 + at[sl[i]]++] = (int32_t) i;
+    }
+    ctx->r_sph_csr.alloc(csr.size());
+    ctx->r_sph_csr.upload(csr, s);
     ctx->n_spheres = (int) d->n_spheres;
     std::vector<float> rad(std::max<size_t>(d->n_spheres, 1), 0.f);
     for (uint32_t i = 0; i < d->n_spheres; ++i) rad[i] = d->spheres[4 * i + 3];

@@ -204,6 +204,7 @@ struct rtcd_ctx {
     DevBuf<Edge> r_edges;
     DevBuf<float4> r_spheres;  // x y z r (link frame)
     DevBuf<int32_t> r_sphere_link;
+    DevBuf<int32_t> r_sph_csr;  // spheres by link: offsets [n_links + 1], then sphere indices [n_spheres]
     DevBuf<float> sphere_radius;
     // launch-shape parameters (rtcd_set_param)
     uint32_t sweep_budget = 32;                 // work units per sweep task before it hands off its subtrees

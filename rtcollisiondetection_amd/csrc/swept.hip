@@ -32,8 +32,9 @@ __device__ __forceinline__ uint32_t load_flag(const uint32_t* f) {
 }
 
 // ---------------------------------------------------------------------------------- centres / fit
+// One thread per waypoint; each link's spheres come from the by-link lists (csr: offsets, then indices).
 __global__ __launch_bounds__(kBS) void k_centers(const RobotDev* __restrict__ R, const float4* __restrict__ sph,
-                                                const int32_t* __restrict__ sl, int S, const float* __restrict__ poses,
+                                                const int32_t* __restrict__ csr, int S, const float* __restrict__ poses,
                                                 int nw, double* __restrict__ c64, float4* __restrict__ c32) {
     int w = blockIdx.x * kBS + threadIdx.x;
     if (w >= nw) return;
@@ -54,8 +55,9 @@ __global__ __launch_bounds__(kBS) void k_centers(const RobotDev* __restrict__ R,
             x64::affine_mul(T, L, Tn);
             for (int k = 0; k < 12; ++k) T[k] = Tn[k];
         }
-        for (int s = 0; s < S; ++s) {
-            if (sl[s] != l) continue;
+        const int32_t* order = csr + NL + 1;
+        for (int k = csr[l]; k < csr[l + 1]; ++k) {
+            const int s = order[k];
             const float4 sp = sph[s];
             double p[3] = {sp.x, sp.y, sp.z}, o[3];
             x64::xform(T, p, o);
@@ -743,7 +745,7 @@ static void enqueue_centers(rtcd_ctx* c, Workspace& ws, const float* d_poses, in
     ws.c32.reserve((size_t) nw * S);
     if (nw > 0 && S > 0) {
         hipLaunchKernelGGL(k_centers, dim3(blocks_for(nw, kBS)), dim3(kBS), 0, ws.stream, c->d_robot.p,
-                           c->r_spheres.p, c->r_sphere_link.p, S, d_poses, nw, ws.c64.p, ws.c32.p);
+                           c->r_spheres.p, c->r_sph_csr.p, S, d_poses, nw, ws.c64.p, ws.c32.p);
         RTCD_LAUNCH_CHECK();
     }
 }
@@ -919,7 +921,7 @@ static std::vector<uint64_t> sweep_key(const rtcd_ctx* c, const Workspace& ws, c
                                        uint32_t nc, const uint32_t* d_out, const uint32_t* d_seg) {
     auto P = [](const void* p) { return (uint64_t) (uintptr_t) p; };
     return {(uint64_t) degree, P(&sc), P(sc.edge_bvh.nodes.p), P(sc.edges.p), P(c->d_robot.p), P(c->r_spheres.p),
-            P(c->r_sphere_link.p), P(c->sphere_radius.p), P(ws.stream), P(d_in), P(d_offs), P(d_out), P(d_seg),
+            P(c->r_sphere_link.p), P(c->r_sph_csr.p), P(c->sphere_radius.p), P(ws.stream), P(d_in), P(d_offs), P(d_out), P(d_seg),
             n_traj, nw, nc, ws.W_version, P(ws.W64.p), P(ws.c64.p), ws.c64.n, P(ws.c32.p), ws.c32.n,
             P(ws.seg_traj.p), ws.seg_traj.n, P(ws.seg_wp.p), ws.seg_wp.n, P(ws.P64.p), ws.P64.n, P(ws.P32.p),
             ws.P32.n, P(ws.cand.p), ws.cand.n, P(ws.unsure.p), ws.unsure.n, P(ws.scalars.p), P(ws.stats.p),
