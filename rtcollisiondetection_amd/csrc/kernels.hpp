@@ -117,7 +117,9 @@ struct SweepArgs {
     const uint32_t* offsets;   // linear: [n_traj+1]; quadratic: nullptr (fixed n_wp)
     int n_wp, n_ctrl;          // quadratic
     const double* W;           // quadratic [n_ctrl][n_wp]
-    double* centers64;         // [n_wp_total][S][3]
+    double* centers64;         // [n_wp_total][S][3], or null (linear: recomputed for the queued tests)
+    const float4* spheres;     // [S] link-frame centre and radius
+    const int32_t* sphere_link;  // [S]
     double* ctrl64;            // quadratic: [n_traj][S][n_ctrl][3]
     const uint32_t* seg_traj;  // segment -> trajectory (linear) [n_seg]
     const uint32_t* seg_wp;    // segment -> global index of its first waypoint (linear) [n_seg]

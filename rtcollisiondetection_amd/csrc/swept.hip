@@ -32,7 +32,30 @@ __device__ __forceinline__ uint32_t load_flag(const uint32_t* f) {
 }
 
 // ---------------------------------------------------------------------------------- centres / fit
+// FP64 centre of a sphere (link l, link-frame centre sp) at the pose q: the FK chain up to link l and the
+// transform, the same operations in the same order as k_centers (so the same bits).
+__device__ __forceinline__ void sphere_center64(const RobotDev* __restrict__ R, const float* q, int l, float4 sp,
+                                                double* o) {
+    double T[12];
+    for (int k = 0; k < 12; ++k) T[k] = (double) R->base[k];
+    for (int j = 1; j <= l; ++j) {
+        const int i = j - 1;
+        double s, c, L[12], Tn[12];
+        x64::sincos((double) q[i], &s, &c);
+        {
+#pragma clang fp contract(off)
+            for (int k = 0; k < 12; ++k)
+                L[k] = (c * (double) R->cmask[i * 12 + k] + s * (double) R->smask[i * 12 + k]) + (double) R->omask[i * 12 + k];
+        }
+        x64::affine_mul(T, L, Tn);
+        for (int k = 0; k < 12; ++k) T[k] = Tn[k];
+    }
+    double p[3] = {sp.x, sp.y, sp.z};
+    x64::xform(T, p, o);
+}
+
 // One thread per waypoint; each link's spheres come from the by-link lists (csr: offsets, then indices).
+// c64 null: only the FP32 centres (the linear sweep recomputes the FP64 ones it needs).
 __global__ __launch_bounds__(kBS) void k_centers(const RobotDev* __restrict__ R, const float4* __restrict__ sph,
                                                 const int32_t* __restrict__ csr, int S, const float* __restrict__ poses,
                                                 int nw, double* __restrict__ c64, float4* __restrict__ c32) {
@@ -62,9 +85,11 @@ __global__ __launch_bounds__(kBS) void k_centers(const RobotDev* __restrict__ R,
             double p[3] = {sp.x, sp.y, sp.z}, o[3];
             x64::xform(T, p, o);
             const size_t idx = (size_t) w * S + s;
-            c64[idx * 3] = o[0];
-            c64[idx * 3 + 1] = o[1];
-            c64[idx * 3 + 2] = o[2];
+            if (c64) {
+                c64[idx * 3] = o[0];
+                c64[idx * 3 + 1] = o[1];
+                c64[idx * 3 + 2] = o[2];
+            }
             c32[idx] = make_float4((float) o[0], (float) o[1], (float) o[2], sp.w);
         }
     }
@@ -571,9 +596,15 @@ __global__ __launch_bounds__(kBS) void k_sweep_redecide(SweepArgs a) {
         float r;
         if (DEG == 1) {
             const uint32_t w = a.seg_wp[g];
-            for (int c = 0; c < 3; ++c) {
-                ctrl[c] = a.centers64[((size_t) w * S + s) * 3 + c];
-                ctrl[3 + c] = a.centers64[((size_t) (w + 1) * S + s) * 3 + c];
+            if (a.centers64) {
+                for (int c = 0; c < 3; ++c) {
+                    ctrl[c] = a.centers64[((size_t) w * S + s) * 3 + c];
+                    ctrl[3 + c] = a.centers64[((size_t) (w + 1) * S + s) * 3 + c];
+                }
+            } else {
+                const int dof = a.robot->dof;
+                sphere_center64(a.robot, a.poses + (size_t) w * dof, a.sphere_link[s], a.spheres[s], ctrl);
+                sphere_center64(a.robot, a.poses + (size_t) (w + 1) * dof, a.sphere_link[s], a.spheres[s], ctrl + 3);
             }
             r = a.radius[s];
         } else if (DEG == 3) {
@@ -739,13 +770,13 @@ static void budget_check_ws(const rtcd_ctx* c, const Workspace& ws, size_t budge
     if (&ws == &c->main) check_budget(c, 0);
 }
 
-static void enqueue_centers(rtcd_ctx* c, Workspace& ws, const float* d_poses, int nw) {
+static void enqueue_centers(rtcd_ctx* c, Workspace& ws, const float* d_poses, int nw, bool fp64) {
     const int S = c->n_spheres;
-    ws.c64.reserve((size_t) nw * S * 3);
+    if (fp64) ws.c64.reserve((size_t) nw * S * 3);
     ws.c32.reserve((size_t) nw * S);
     if (nw > 0 && S > 0) {
         hipLaunchKernelGGL(k_centers, dim3(blocks_for(nw, kBS)), dim3(kBS), 0, ws.stream, c->d_robot.p,
-                           c->r_spheres.p, c->r_sph_csr.p, S, d_poses, nw, ws.c64.p, ws.c32.p);
+                           c->r_spheres.p, c->r_sph_csr.p, S, d_poses, nw, fp64 ? ws.c64.p : nullptr, ws.c32.p);
         RTCD_LAUNCH_CHECK();
     }
 }
@@ -762,6 +793,8 @@ static SweepArgs base_args(rtcd_ctx* c, Workspace& ws, SceneRes& sc, uint32_t* d
     a.robot_n_spheres_host = c->n_spheres;
     a.sc = sc.dev();
     a.centers64 = ws.c64.p;
+    a.spheres = c->r_spheres.p;
+    a.sphere_link = c->r_sphere_link.p;
     a.out = d_out;
     a.out_seg = d_seg;
     a.stats = ws.stats.p;
@@ -847,7 +880,7 @@ void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_tra
     sweep_begin(ws, events);
     fill_u32(d_out, n_traj, 0u, s);
     if (d_seg && n_seg > 0) fill_u32(d_seg, (size_t) n_seg, 0u, s);
-    enqueue_centers(c, ws, d_traj, (int) nw);
+    enqueue_centers(c, ws, d_traj, (int) nw, false);
     ws.seg_traj.reserve(std::max(n_seg, 1));
     ws.seg_wp.reserve(std::max(n_seg, 1));
     budget_check_ws(c, ws, budget);
@@ -855,6 +888,8 @@ void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_tra
                        ws.seg_traj.p, ws.seg_wp.p);
     RTCD_LAUNCH_CHECK();
     SweepArgs a = base_args(c, ws, sc, d_out, d_seg);
+    a.centers64 = nullptr;  // the FP64 re-decision recomputes the centres of its tests from the poses
+    a.poses = d_traj;
     a.n_traj = (int) n_traj;
     a.n_wp_total = (int) nw;
     a.seg_traj = ws.seg_traj.p;
@@ -891,7 +926,7 @@ void enqueue_curve(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, 
     sweep_begin(ws, events);
     fill_u32(d_out, n_traj, 0u, s);
     if (d_seg) fill_u32(d_seg, (size_t) n_traj * npc, 0u, s);
-    enqueue_centers(c, ws, d_wp, (int) (n_traj * nw));
+    enqueue_centers(c, ws, d_wp, (int) (n_traj * nw), true);
     ws.P64.reserve((size_t) n_traj * S * nc * 3);
     ws.P32.reserve((size_t) n_traj * S * nc);
     budget_check_ws(c, ws, budget);
