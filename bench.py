@@ -121,7 +121,7 @@ REF_RTX4070 = {
 def reference_scene_bench(ctx, rb, reps=5):
     """RTCD's own benchmark inputs on its three cuRobo scenes: 4,096 of its 8,192 Halton poses per batch
     (each direction and both), single-pose batches, and 4,096 of its KNN trajectories (16 waypoints,
-    linear and quadratic with 8 control points).  Wall time of the blocking host-buffer entry points,
+    linear, and quadratic and cubic with 8 control points; RTCD publishes no cubic figures).  Wall time of the blocking host-buffer entry points,
     the way RTCD's Google-Benchmark loop times check() (median of `reps`)."""
     from rtcollisiondetection_amd import samplers, scenes
     q8k = samplers.benchmark_poses_discrete(8192, rb.lower, rb.upper)
@@ -131,6 +131,8 @@ def reference_scene_bench(ctx, rb, reps=5):
     tr = tr_all[np.random.default_rng(1).choice(len(tr_all), 4096, replace=False)]
     offs = np.arange(0, (len(tr) + 1) * 16, 16, dtype=np.uint32)
     W = samplers.quadratic_fit_weights(8, 16)
+    from rtcollisiondetection_amd.detector import curve_weights
+    W3 = curve_weights(3, 'normal', 8, 16)
 
     def med(f, r=reps):
         f()
@@ -162,6 +164,7 @@ def reference_scene_bench(ctx, rb, reps=5):
         r['collisions_two_way'] = int(ctx.check_discrete(sid, q, 3).astype(bool).sum())
         r['linear16_us_per_traj'] = round(med(lambda: ctx.check_linear(sid, tr.reshape(-1, 7), offs)) / len(tr) * 1e6, 3)
         r['quadratic8_us_per_traj'] = round(med(lambda: ctx.check_quadratic(sid, tr, 8, W)) / len(tr) * 1e6, 3)
+        r['cubic8_us_per_traj'] = round(med(lambda: ctx.check_cubic(sid, tr, 8, W3)) / len(tr) * 1e6, 3)
         r['rtx4070_published'] = REF_RTX4070[name]
         out[name] = r
         ctx.destroy_scene(sid)

@@ -47,4 +47,11 @@ def test_knn_trajectories_on_reference_scene(ctx, panda, orc_panda):
     refq = O.check_quadratic(orc_panda, m, tr, W)
     assert 0 < refq.sum() < len(refq)
     assert np.array_equal(ctx.check_quadratic(sid, tr, 8, W).astype(bool), refq.astype(bool))
+    # the same trajectories as cubic curves (RTCD's default CUBIC_CURVE robot), cubic uniform-knot fit
+    from rtcollisiondetection_amd.detector import curve_weights
+    W3 = curve_weights(3, 'normal', 8, 16)
+    refc, segc = O.check_cubic(orc_panda, m, tr, W3, with_segments=True)
+    assert 0 < refc.sum() < len(refc)
+    gotc, gsegc = ctx.check_cubic(sid, tr, 8, W3, with_segments=True)
+    assert np.array_equal(gotc.astype(bool), refc.astype(bool)) and np.array_equal(gsegc, segc)
     ctx.destroy_scene(sid)
