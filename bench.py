@@ -188,8 +188,8 @@ def main():
                     help='only run a plain 256 MiB device-to-device copy 5 times (reference for HBM counters)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build measures one GPU only; the path shards by query range '
-                                   '(see DESIGN.md), run with --gpus 1'}))
+        print(json.dumps({'error': 'one GPU only: a multi-GPU run would split the poses or trajectories '
+                                   'by range across devices, which is not built; run with --gpus 1'}))
         sys.exit(2)
 
     import torch
