@@ -80,10 +80,14 @@ __device__ __forceinline__ bool slab(float3 P, float3 inv, float lx, float ly, f
     return tmin <= tmax + fabsf(tmax) * 1e-6f + 1e-30f;
 }
 
+// Reciprocal direction for slab().  The hardware reciprocal is within 1 ulp: against a correctly rounded
+// one it moves each slab parameter t by at most one more ulp relative to t.  slab() compares
+// 0 <= tmin <= tmax with a relative slack of 1e-6 (about 16 ulp), which absorbs that; the absolute
+// errors of the box and point coordinates are covered by the pad, as before.
 __device__ __forceinline__ float3 safe_inv(float3 d) {
     auto f = [](float x) {
         float a = fabsf(x) < 1e-30f ? copysignf(1e-30f, x) : x;
-        return 1.f / a;
+        return __builtin_amdgcn_rcpf(a);
     };
     return make_float3(f(d.x), f(d.y), f(d.z));
 }
