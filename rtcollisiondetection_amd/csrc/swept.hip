@@ -162,15 +162,18 @@ __global__ __launch_bounds__(kBS) void k_fit(const double* __restrict__ c64, con
 __device__ int cap_cert(float3 C0, float3 C1, float3 A, float3 B, float r_hit, float r_miss, float ein) {
     const float3 E = C1 - C0, D = B - A, w0 = C0 - A;
     const float a = dot(E, E), b = dot(E, D), c = dot(D, D), d = dot(E, w0), e = dot(D, w0);
+    // the closest-point parameters only pick the point (s, t) of the unit square where both bounds are
+    // taken, and both bounds hold at any such point: hardware reciprocals are enough here
     float t = 0.f, s = 0.f;
     const float den = a * c - b * b;
-    if (den > 1e-12f * a * c && c > 0.f) t = fminf(fmaxf((b * e - c * d) / den, 0.f), 1.f);
+    const float ra = a > 0.f ? __builtin_amdgcn_rcpf(a) : 0.f;
+    if (den > 1e-12f * a * c && c > 0.f) t = fminf(fmaxf((b * e - c * d) * __builtin_amdgcn_rcpf(den), 0.f), 1.f);
     if (c > 0.f) {
-        s = (b * t + e) / c;
-        if (s < 0.f) { s = 0.f; t = a > 0.f ? fminf(fmaxf(-d / a, 0.f), 1.f) : 0.f; }
-        else if (s > 1.f) { s = 1.f; t = a > 0.f ? fminf(fmaxf((b - d) / a, 0.f), 1.f) : 0.f; }
+        s = (b * t + e) * __builtin_amdgcn_rcpf(c);
+        if (s < 0.f) { s = 0.f; t = a > 0.f ? fminf(fmaxf(-d * ra, 0.f), 1.f) : 0.f; }
+        else if (s > 1.f) { s = 1.f; t = a > 0.f ? fminf(fmaxf((b - d) * ra, 0.f), 1.f) : 0.f; }
     } else {
-        t = a > 0.f ? fminf(fmaxf(-d / a, 0.f), 1.f) : 0.f;
+        t = a > 0.f ? fminf(fmaxf(-d * ra, 0.f), 1.f) : 0.f;
     }
     const float3 v = w0 + t * E - s * D;
     const float f = dot(v, v);
