@@ -195,6 +195,8 @@ __device__ int cap_cert(float3 C0, float3 C1, float3 A, float3 B, float r_hit, f
     return kUnsure;
 }
 
+// Square roots below use the hardware instruction (within 1 ulp): the (1 + 8 eps) factor on each
+// deviation bound covers it with the rounding of the dot product and the products.
 // Quadratic Bezier (q0, q1, q2) vs segment, adaptive: the dyadic sub-pieces [k/2^d, (k+1)/2^d] are
 // visited depth first without a stack; a sub-piece with controls (a, m, b) lies within
 // h = |m - (a + b)/2| / 2 of its chord, so the chord capsule with radii r - h (hit) and r + h (miss)
@@ -218,10 +220,10 @@ __device__ int quad_cert(float3 q0, float3 q1, float3 q2, float3 A, float3 B, fl
     int d = 0;
     uint32_t k = 0;
     for (;;) {
-        const float t0 = (float) k / (float) (1u << d), t1 = (float) (k + 1) / (float) (1u << d);
+        const float t0 = ldexpf((float) k, -d), t1 = ldexpf((float) (k + 1), -d);  // exact: k < 2^d <= 2^8
         const float3 pa = blossom(q0, q1, q2, t0, t0), pm = blossom(q0, q1, q2, t0, t1), pb = blossom(q0, q1, q2, t1, t1);
         const float3 dev = pm - 0.5f * (pa + pb);
-        const float h = 0.5f * sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 2.f * 1.7321f * e2;
+        const float h = 0.5f * __builtin_amdgcn_sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 2.f * 1.7321f * e2;
         const int cls = cap_cert(pa, pb, A, B, r - h, r + h, e2);
         ++n_cert;
         if (cls == kHit) return kHit;
@@ -261,7 +263,7 @@ __device__ __forceinline__ float3 blossom3(float3 q0, float3 q1, float3 q2, floa
 __device__ __forceinline__ float cubic_dev(float3 a, float3 b, float3 c, float3 d, float e2) {
     constexpr float kThird = 1.f / 3.f;
     const float3 d1 = b - kThird * (2.f * a + d), d2 = c - kThird * (a + 2.f * d);
-    return 0.75f * sqrtf(fmaxf(dot(d1, d1), dot(d2, d2))) * (1.f + 8.f * kEps32) + 4.f * 1.7321f * e2;
+    return 0.75f * __builtin_amdgcn_sqrtf(fmaxf(dot(d1, d1), dot(d2, d2))) * (1.f + 8.f * kEps32) + 4.f * 1.7321f * e2;
 }
 
 __device__ int cubic_cert(float3 q0, float3 q1, float3 q2, float3 q3, float3 A, float3 B, float r, float ein,
@@ -274,7 +276,7 @@ __device__ int cubic_cert(float3 q0, float3 q1, float3 q2, float3 q3, float3 A, 
     int d = 0;
     uint32_t k = 0;
     for (;;) {
-        const float t0 = (float) k / (float) (1u << d), t1 = (float) (k + 1) / (float) (1u << d);
+        const float t0 = ldexpf((float) k, -d), t1 = ldexpf((float) (k + 1), -d);  // exact: k < 2^d <= 2^8
         const float3 pa = blossom3(q0, q1, q2, q3, t0, t0, t0), pb = blossom3(q0, q1, q2, q3, t0, t0, t1);
         const float3 pc = blossom3(q0, q1, q2, q3, t0, t1, t1), pd = blossom3(q0, q1, q2, q3, t1, t1, t1);
         const float h = cubic_dev(pa, pb, pc, pd, e2);
@@ -401,7 +403,7 @@ __device__ __forceinline__ Cull make_cull(const Item& m, float scale) {
     if (DEG == 2) {
         // a quadratic sub-piece lies within |q1 - (q0 + q2) / 2| / 2 of its chord
         const float3 dev = m.q1 - 0.5f * (m.q0 + m.q2);
-        c.cpad += 0.5f * sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 4.f * 1.7321f * (m.ein + 8.f * kEps32 * m.M);
+        c.cpad += 0.5f * __builtin_amdgcn_sqrtf(dot(dev, dev)) * (1.f + 8.f * kEps32) + 4.f * 1.7321f * (m.ein + 8.f * kEps32 * m.M);
     }
     if (DEG == 3) c.cpad += cubic_dev(m.q0, m.q1, m.q2, m.q3, m.ein + 12.f * kEps32 * m.M);
     return c;
