@@ -120,3 +120,13 @@ def test_create_reports_errors_without_crashing():
     L = _lib.lib()
     assert L.rtcd_check_discrete(None, 0, 3, None, 0, None) == 1
     assert L.rtcd_sync(None, None) == 1
+
+
+def test_cpp_facade_header_compiles():
+    """include/rtcd_detector.hpp and its check program compile against include/rtcd.h (no GPU needed)."""
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(['g++', '-std=c++17', '-fsyntax-only', '-Wall', '-Werror', '-I' + os.path.join(root, 'include'),
+                        os.path.join(root, 'tests', 'cpp', 'facade_check.cpp')], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
