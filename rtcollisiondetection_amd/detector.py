@@ -150,15 +150,15 @@ class Context:
         check(lib().rtcd_set_leaf_sizes(self.h, int(robot), int(scene_tri), int(scene_edge)))
 
     def autotune(self, kind, sid, data, offsets=None, direction=_lib.TWO_WAY, n_wp=16, n_ctrl=8, leaf_size=True):
-        """Tune the launch shapes on a sample (kind: _lib.REQ_DISCRETE / REQ_LINEAR / REQ_QUADRATIC) and
-        keep the fastest; returns the tuner's report as a dict."""
+        """Tune the launch shapes on a sample (kind: _lib.REQ_DISCRETE / REQ_LINEAR / REQ_QUADRATIC /
+        REQ_CUBIC) and keep the fastest; returns the tuner's report as a dict."""
         data = _f32(data)
         offs = np.ascontiguousarray(offsets, dtype=np.uint32) if offsets is not None else None
         if kind == _lib.REQ_LINEAR:
             n = len(offs) - 1
         else:
             n = data.shape[0]
-            if kind == _lib.REQ_QUADRATIC:
+            if kind in (_lib.REQ_QUADRATIC, _lib.REQ_CUBIC):
                 n_wp = data.shape[1]
         r = _lib.Request(kind, direction, sid, n, data.ctypes.data_as(C.c_void_p),
                          offs.ctypes.data_as(C.c_void_p) if offs is not None else None, n_wp, n_ctrl, None)

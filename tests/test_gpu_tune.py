@@ -63,6 +63,11 @@ def test_autotune_sweeps_then_parity(panda, orc_panda):
         from rtcollisiondetection_amd.detector import quadratic_weights
         W = quadratic_weights(8, 16)
         assert _mismatch(c.check_quadratic(sid, wp, 8), O.check_quadratic(orc_panda, m, wp, W)) == 0
+        rep = c.autotune(_lib.REQ_CUBIC, sid, wp, n_ctrl=8, leaf_size=False)
+        print('cubic tune report', rep)
+        from rtcollisiondetection_amd.detector import curve_weights
+        W3 = curve_weights(3, 'normal', 8, 16)
+        assert _mismatch(c.check_cubic(sid, wp, 8), O.check_cubic(orc_panda, m, wp, W3)) == 0
     finally:
         c.close()
 
