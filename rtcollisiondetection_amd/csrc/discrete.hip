@@ -255,11 +255,14 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
         uint32_t key;
         int f = 0, ni = 0;
         if (pass == 0) {
-            // item = (task, frontier entry); task = (link-major) (link, pose)
+            // item = (task, frontier entry); task = (link-major) (link, pose), distal links first: a
+            // serial arm's outer links sweep the most space, so their records flag the most poses
+            // early and the narrow passes skip the later records of those poses (dense 65,536-pose
+            // check: obstacle->robot 0.96 -> 0.94 ms)
             const int task = (int) (id / (size_t) a.F);
             f = (int) (id - (size_t) task * a.F);
             const int li = task / a.n_poses, p = task - li * a.n_poses;
-            key = (uint32_t) ((size_t) p * a.n_links + a.l0 + li);
+            key = (uint32_t) ((size_t) p * a.n_links + a.l0 + (a.n_active - 1 - li));
         } else {
             const uint2 r = in_q[id];
             if (r.x == 0xffffffffu) continue;  // slot of a hand-off that did not fit
