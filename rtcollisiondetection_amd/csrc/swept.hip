@@ -474,7 +474,9 @@ __global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, int pass) {
     };
     // one node visit; false when the walk has ended (hit, done, handed off or abandoned)
     auto step = [&]() -> bool {
-        if ((++iter & 7) == 0 && load_flag(flag)) return false;
+        // every 4th visit: against every 8th, quadratic and cubic sweeps run 4-15 % faster (walks of
+        // flagged trajectories stop sooner); checking every visit slows the linear sweeps
+        if ((++iter & 3) == 0 && load_flag(flag)) return false;
         if (spill_ok && work > a.budget) {
             const uint32_t want = (uint32_t) sp + 1u;
             const uint32_t o = atomicAdd(out_cnt, want);
