@@ -265,11 +265,12 @@ def main():
     log(f'[bench] two-way: {value:.4g} poses/s, {dt / args.steps * 1e3:.2f} ms/step, {hits} collisions, '
         f'stages (FK, broad, gather, O2R, R2O+FP64) ms {np.round(stage, 3).tolist()}')
 
-    # Roofline of the dominant kernel.  Each narrow pass is one launch, bracketed by events on the
-    # library's stream (stage 3: k_narrow<1>, stage 4: k_narrow<2> + the short FP64 kernel).  Their
-    # work comes from untimed counting runs: the obstacle->robot direction alone is exactly the first
-    # pass of the two-way check (same candidates, flags start at zero), and the robot->obstacle pass
-    # is the two-way count minus it.
+    # Roofline of the dominant kernel.  The library brackets every launch group with events on its
+    # stream and sums them per stage (stage 3: the k_narrow<1> launches, which a split check runs in
+    # parts between the broad passes; stage 4: k_narrow<2> + the short FP64 kernel).  Their work
+    # comes from untimed counting runs: the obstacle->robot direction alone runs exactly the
+    # obstacle->robot parts of the two-way check (same schedule, flags start at zero), and the
+    # robot->obstacle pass is the two-way count minus it.
     ctx.set_param('graphs', 1)
     ctx.set_counting(True)
     run(1)
@@ -296,7 +297,7 @@ def main():
     tp = os.path.join(ROOT, 'profiles', dom['traffic_file'])
     if os.path.exists(tp):
         try:
-            traffic = json.load(open(tp)).get('hbm_bytes_per_launch')
+            traffic = json.load(open(tp)).get('hbm_bytes_per_check')
         except Exception:
             traffic = None
     roofline = {'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',

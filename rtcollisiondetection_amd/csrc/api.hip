@@ -126,10 +126,16 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
 
     ws.seg_count.reserve((size_t) kCandSegments * kSegStride);
     ws.seg_pre.reserve(kCandSegments + 1);
+    ws.seg_base.reserve(kCandSegments + 1);
     ws.pin_status.reserve(kPublish);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, s);
+    // a segment of stage k starts here (k < 0: the check ends)
+    int n_seg = 0;
     auto mark = [&](int k) {
-        if (events) RTCD_HIP(hipEventRecord(ws.ev[k], s));
+        if (!events) return;
+        RTCD_HIP(hipEventRecord(ws.ev[n_seg], s));
+        if (k >= 0) ws.seg_stage[n_seg++] = (int8_t) k;
+        else ws.ev_end = n_seg;
     };
     mark(0);
     // the pose kernel also clears the flags, the status words and the segment counters
@@ -152,6 +158,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.seg_count = ws.seg_count.p;
     a.seg_pre = ws.seg_pre.p;
     a.flat = ws.flat.p;
+    a.seg_base = ws.seg_base.p;
     a.tri_candidates = reinterpret_cast<unsigned long long*>(ws.scalars.p + 4);
     a.overflow = ws.scalars.p + 1;
     a.flags = d_out;
@@ -185,14 +192,45 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.narrow_passes = (int) c->o2r_passes;
     a.narrow_bs = (int) c->narrow_bs;
     const int broad_stack = stack_for(sc.tri_bvh.depth);
+    // Split check: after each broad pass k with bit k of c->o2r_split set (below the last pass), an
+    // obstacle->robot pass runs over the records gathered since the previous one.  The hand-off passes
+    // mostly walk the poses deep in clutter, which collide: the early passes flag many of them and the
+    // later broad passes skip their walks.  The "broad" stage then holds the early gathers and
+    // obstacle->robot passes too.
+    const uint32_t split = (dir & 1) && !tiny && a.F > 0 && a.n_passes > 1
+                               ? c->o2r_split & ((1u << (a.n_passes - 1)) - 1u) : 0u;
+    // obstacle->robot part i counts its hand-offs in scalars [o2r_ctr(i) ..+16): fresh counters each
+    auto o2r_ctr = [](int i) { return i == 0 ? 24 : 40 + 16 * i; };
     mark(1);
-    if (a.F > 0) launch_broad(a, broad_stack, ws.counting, s);
-    mark(2);
-    launch_gather(a, tiny, s);
-    mark(3);
-    // obstacle->robot first: it is cheaper and flags most colliding poses, which the robot->obstacle
-    // pass then skips
-    if (dir & 1) launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
+    if (split) {
+        int p0 = 0, i = 0;
+        for (int p = 0; p < a.n_passes - 1; ++p) {
+            if (!(split >> p & 1u)) continue;
+            if (i > 0) mark(1);
+            launch_broad(a, broad_stack, ws.counting, s, p0, p + 1);
+            mark(2);
+            launch_gather(a, false, s, i == 0 ? 1 : 2);
+            mark(3);
+            a.nspill_count = ws.scalars.p + o2r_ctr(i++);
+            launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
+            p0 = p + 1;
+        }
+        mark(1);
+        launch_broad(a, broad_stack, ws.counting, s, p0, a.n_passes);
+        mark(2);
+        launch_gather(a, false, s, 2);
+        mark(3);
+        a.nspill_count = ws.scalars.p + o2r_ctr(i);
+        launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
+    } else {
+        if (a.F > 0) launch_broad(a, broad_stack, ws.counting, s);
+        mark(2);
+        launch_gather(a, tiny, s);
+        mark(3);
+        // obstacle->robot first: it is cheaper and flags most colliding poses, which the robot->obstacle
+        // pass then skips
+        if (dir & 1) launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
+    }
     mark(4);
     if (dir == 3 && !tiny) {
         // the broad records are no longer needed: the compacted list reuses their buffer (a tiny
@@ -207,7 +245,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.narrow_passes = tiny ? 1 : (int) c->narrow_passes;
     if (dir & 2) launch_narrow(a, 2, c->robot_depth + 1, ws.counting, s);
     launch_redecide(a, tiny, ws.scalars.p, ws.pin_status.d, ws.pub_flags, s);
-    mark(5);
+    mark(-1);
     ws.timing_pending = events;
     ws.stages_recorded = events;
 }
@@ -231,7 +269,7 @@ static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, c
             P(c->d_robot.p), P(c->r_tri_nodes.p), P(c->r_tris.p), P(c->r_edge_nodes.p), P(c->r_edges.p),
             P(ws.stream), P(d_poses), P(d_out), n, (uint64_t) dir,
             P(ws.T64.p), P(ws.Ti.p), P(ws.wbox.p), P(ws.cand.p), ws.cand.n, P(ws.flat.p), ws.flat.n,
-            P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.spill.p), ws.spill.n, P(ws.nspill.p), ws.nspill.n,
+            P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.seg_base.p), c->o2r_split, P(ws.spill.p), ws.spill.n, P(ws.nspill.p), ws.nspill.n,
             P(ws.unsure.p), ws.unsure.n, c->leaf_scene_tri, c->narrow_budget, c->narrow_budget_next, c->narrow_passes, c->o2r_budget,
             c->o2r_budget_next, c->o2r_passes,
             P(ws.scalars.p), P(ws.stats.p), P(ws.pin_status.d), P(ws.pub_flags), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
@@ -277,6 +315,7 @@ void enqueue_discrete_main(rtcd_ctx* c, SceneRes& sc, int dir, const float* d_po
         RTCD_HIP(hipEventRecord(ws.ev[0], s));
         RTCD_HIP(hipGraphLaunch(g.exec, s));
         RTCD_HIP(hipEventRecord(ws.ev[5], s));
+        ws.ev_end = 5;
         ws.timing_pending = true;
         ws.stages_recorded = false;
         g.last_replayed = true;
@@ -311,15 +350,16 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
             if (st) {
                 if (ws.timing_pending) {
                     float ms = 0.f;
-                    if (hipEventElapsedTime(&ms, ws.ev[0], ws.ev[5]) == hipSuccess) st->last_ms = ms;
-                    // stage k spans ev[k] .. ev[k+1]: FK, broad, gather, obstacle->robot narrow,
-                    // robot->obstacle narrow + FP64 re-decision
-                    hipEvent_t b[6] = {ws.ev[0], ws.ev[1], ws.ev[2], ws.ev[3], ws.ev[4], ws.ev[5]};
-                    for (int k = 0; k < 5; ++k) {
+                    if (hipEventElapsedTime(&ms, ws.ev[0], ws.ev[ws.ev_end]) == hipSuccess) st->last_ms = ms;
+                    // stages: FK, broad, gather, obstacle->robot narrow, robot->obstacle narrow + FP64
+                    // re-decision, each the sum of its segments (a split check interleaves them)
+                    for (int k = 0; k < 5; ++k) st->stage_ms[k] = ws.stages_recorded ? 0.0 : -1.0;
+                    for (int i = 0; ws.stages_recorded && i < ws.ev_end; ++i) {
                         float sm = 0.f;
-                        st->stage_ms[k] = ws.stages_recorded && hipEventElapsedTime(&sm, b[k], b[k + 1]) == hipSuccess
-                                              ? sm
-                                              : -1.0;
+                        if (hipEventElapsedTime(&sm, ws.ev[i], ws.ev[i + 1]) == hipSuccess)
+                            st->stage_ms[ws.seg_stage[i]] += sm;
+                        else
+                            st->stage_ms[ws.seg_stage[i]] = -1.0;
                     }
                 }
                 st->candidates = (uint64_t) sv[4] | ((uint64_t) sv[5] << 32);  // triangles in emitted leaves
@@ -908,6 +948,11 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
         if (value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "tiny batch must fit 32 bits");
         ctx->tiny_batch = (uint32_t) value;
         return RTCD_OK;
+    case RTCD_PARAM_O2R_SPLIT:
+        if (value >= (1ull << kMaxPasses) || __builtin_popcountll(value) > kMaxO2rSplits)
+            return fail(RTCD_ERR_INVALID_ARG, "split must be a mask of at most 3 of the bits 0..15");
+        ctx->o2r_split = (uint32_t) value;
+        return RTCD_OK;
     case RTCD_PARAM_SERVE_LANE_KINDS:
         if (value < 1 || value > 15) return fail(RTCD_ERR_INVALID_ARG, "lane kinds must be a non-empty mask of bits 0..3");
         ctx->serve_lane_kinds = (uint32_t) value;
@@ -954,6 +999,7 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_SERVE_CHUNK_SWEEP: *value = ctx->serve_chunk_sweep; return RTCD_OK;
     case RTCD_PARAM_SERVE_LANE_KINDS: *value = ctx->serve_lane_kinds; return RTCD_OK;
     case RTCD_PARAM_TINY_BATCH: *value = ctx->tiny_batch; return RTCD_OK;
+    case RTCD_PARAM_O2R_SPLIT: *value = ctx->o2r_split; return RTCD_OK;
     default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
 }

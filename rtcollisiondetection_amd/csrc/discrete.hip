@@ -268,6 +268,8 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
             if (r.x == 0xffffffffu) continue;  // slot of a hand-off that did not fit
             key = r.x;
             ni = (int) r.y;
+            // a split check's early obstacle->robot pass may have flagged the pose: its walks end
+            if (load_flag(a.flags + key / (uint32_t) a.n_links)) continue;
         }
         const int l = (int) (key % (uint32_t) a.n_links);
         const size_t pl = key;
@@ -351,13 +353,23 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
 
 // ------------------------------------------------------------------------------------------ gather
 // Exclusive prefix of the segment counts (each clamped to seg_cap): one block, seg_pre[kSeg] = total.
+// Split checks gather in parts: seg_base[b] holds the records of segment b gathered so far and
+// seg_base[kSeg] their total.  `gpart` 1 (first part) sets them; `gpart` 2 scans only the records
+// added since and places them after the earlier parts (seg_pre[0] = first new record), then updates
+// them.  Each word of seg_base is read and written by one thread only.
 __global__ __launch_bounds__(kBS) void k_seg_scan(const uint32_t* __restrict__ seg_count, uint32_t seg_cap,
-                                                 uint32_t* __restrict__ seg_pre) {
+                                                 uint32_t* __restrict__ seg_pre, uint32_t* __restrict__ seg_base,
+                                                 int gpart) {
     constexpr int per = kSeg / kBS;
     __shared__ uint32_t part[kBS / 64];
+    __shared__ uint32_t start_s;
+    if (threadIdx.x == 0) start_s = gpart == 2 ? seg_base[kSeg] : 0u;
     uint32_t loc[per], acc = 0;
     for (int k = 0; k < per; ++k) {
-        loc[k] = min(seg_count[(size_t) (threadIdx.x * per + k) * kSegStride], seg_cap);
+        const int b = threadIdx.x * per + k;
+        const uint32_t cnt = min(seg_count[(size_t) b * kSegStride], seg_cap);
+        loc[k] = gpart == 2 ? cnt - seg_base[b] : cnt;
+        if (gpart) seg_base[b] = cnt;
         acc += loc[k];
     }
     // inclusive scan of the per-thread sums within each wave, then across the waves
@@ -374,8 +386,12 @@ __global__ __launch_bounds__(kBS) void k_seg_scan(const uint32_t* __restrict__ s
         before += w < wave ? part[w] : 0u;
         all += part[w];
     }
-    if (threadIdx.x == 0) seg_pre[kSeg] = all;
-    uint32_t run = before + inc - acc;
+    const uint32_t start = start_s;
+    if (threadIdx.x == 0) {
+        seg_pre[kSeg] = start + all;
+        if (gpart) seg_base[kSeg] = start + all;
+    }
+    uint32_t run = start + before + inc - acc;
     for (int k = 0; k < per; ++k) {
         seg_pre[threadIdx.x * per + k] = run;
         run += loc[k];
@@ -428,13 +444,14 @@ __global__ __launch_bounds__(kBS) void k_seg_scan_gather(const uint2* __restrict
     }
 }
 
-// Block b copies segment b to flat[seg_pre[b] ..): the narrow phase then reads records by index.
+// Block b copies the newest seg_pre[b + 1] - seg_pre[b] records of segment b (all of them, unless a
+// split check gathered some before) to flat[seg_pre[b] ..): the narrow phase reads records by index.
 __global__ __launch_bounds__(kBS) void k_seg_gather(const uint2* __restrict__ cand, const uint32_t* __restrict__ seg_count,
                                                    uint32_t seg_cap, const uint32_t* __restrict__ seg_pre,
                                                    uint2* __restrict__ flat) {
     const uint32_t b = blockIdx.x;
-    const uint32_t n = min(seg_count[(size_t) b * kSegStride], seg_cap), base = seg_pre[b];
-    const uint2* src = cand + (size_t) b * seg_cap;
+    const uint32_t base = seg_pre[b], n = seg_pre[b + 1] - base;
+    const uint2* src = cand + (size_t) b * seg_cap + (min(seg_count[(size_t) b * kSegStride], seg_cap) - n);
     for (uint32_t j = threadIdx.x; j < n; j += kBS) flat[(size_t) base + j] = src[j];
 }
 
@@ -960,8 +977,10 @@ __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
     Ctr ctr;
     if (pass == 0) {
         __shared__ uint2 lq[BS / 64][128];
-        const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat;
-        const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg];
+        // obstacle->robot: the records of the latest gather (a split check gathers in parts)
+        const uint32_t first = DIR == 1 ? a.seg_pre[0] : 0u;
+        const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat + first;
+        const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg] - first;
         // a pass that may hand off and one that never does (the walks of the latter keep fewer registers)
         if (hs.may) narrow_lanes<DIR, STACK, BS, COUNT, true>(a, list, total, lq[threadIdx.x >> 6], st, ctr, hs);
         else narrow_lanes<DIR, STACK, BS, COUNT, false>(a, list, total, lq[threadIdx.x >> 6], st, ctr, hs);
@@ -1068,12 +1087,12 @@ void launch_fk(const RobotDev* robot, int, int, const float* poses, int n, doubl
 }
 
 template <int STACK>
-static void broad_t(const DiscreteArgs& a, bool count, hipStream_t s) {
+static void broad_t(const DiscreteArgs& a, bool count, hipStream_t s, int p0, int p1) {
     const unsigned nb0 = (unsigned) std::min<size_t>(blocks_for((size_t) a.n_active * a.n_poses * a.F, kBS), kSeg);
     if (!nb0) return;
     // later passes stride over the hand-off queue; they need at most kSeg blocks (one per segment)
     const unsigned nb1 = std::min<unsigned>((unsigned) device_cus() * 8u, kSeg);
-    for (int pass = 0; pass < a.n_passes; ++pass) {
+    for (int pass = p0; pass < (p1 < 0 ? a.n_passes : p1); ++pass) {
         const unsigned nb = pass == 0 ? nb0 : nb1;
         if (count) hipLaunchKernelGGL((k_broad<STACK, true>), dim3(nb), dim3(kBS), 0, s, a, pass);
         else hipLaunchKernelGGL((k_broad<STACK, false>), dim3(nb), dim3(kBS), 0, s, a, pass);
@@ -1081,12 +1100,12 @@ static void broad_t(const DiscreteArgs& a, bool count, hipStream_t s) {
     }
 }
 
-void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s) {
-    if (stack <= 20) broad_t<20>(a, count, s);
-    else if (stack <= 26) broad_t<26>(a, count, s);
-    else if (stack <= 32) broad_t<32>(a, count, s);
-    else if (stack <= 40) broad_t<40>(a, count, s);
-    else if (stack <= 64) broad_t<64>(a, count, s);
+void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s, int p0, int p1) {
+    if (stack <= 20) broad_t<20>(a, count, s, p0, p1);
+    else if (stack <= 26) broad_t<26>(a, count, s, p0, p1);
+    else if (stack <= 32) broad_t<32>(a, count, s, p0, p1);
+    else if (stack <= 40) broad_t<40>(a, count, s, p0, p1);
+    else if (stack <= 64) broad_t<64>(a, count, s, p0, p1);
     else throw std::runtime_error("BVH deeper than 64 levels");
 }
 
@@ -1117,14 +1136,14 @@ static void narrow_dir(const DiscreteArgs& a, int stack, bool count, hipStream_t
     else throw std::runtime_error("BVH deeper than 64 levels");
 }
 
-void launch_gather(const DiscreteArgs& a, bool small, hipStream_t s) {
+void launch_gather(const DiscreteArgs& a, bool small, hipStream_t s, int part) {
     if (small) {
         hipLaunchKernelGGL(k_seg_scan_gather, dim3(16), dim3(kBS), 0, s, a.cand, a.seg_count, a.seg_cap, a.seg_pre,
                            a.flat);
         RTCD_LAUNCH_CHECK();
         return;
     }
-    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kBS), 0, s, a.seg_count, a.seg_cap, a.seg_pre);
+    hipLaunchKernelGGL(k_seg_scan, dim3(1), dim3(kBS), 0, s, a.seg_count, a.seg_cap, a.seg_pre, a.seg_base, part);
     RTCD_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_seg_gather, dim3(kSeg), dim3(kBS), 0, s, a.cand, a.seg_count, a.seg_cap, a.seg_pre, a.flat);
     RTCD_LAUNCH_CHECK();

@@ -11,7 +11,10 @@ constexpr int kCandSegments = 2048;
 // same time then never contend for one line's atomics.
 constexpr int kSegStride = 32;
 // Words of the per-workspace scalar block (counters and flags of one launch).
-constexpr int kScalars = 64;
+// Early obstacle->robot passes of a split check (bits of RTCD_PARAM_O2R_SPLIT); with the final one
+// each counts its hand-offs in scalars of its own: [24..39], [56..71], [72..87], [88..103].
+constexpr int kMaxO2rSplits = 3;
+constexpr int kScalars = 104;
 
 struct RobotBvh {
     const BNode* tri_nodes;
@@ -49,6 +52,8 @@ struct DiscreteArgs {
                            // (may exceed seg_cap: overflow bit 1)
     uint32_t* seg_pre;     // [kCandSegments + 1] exclusive prefix of the clamped segment counts
     uint2* flat;           // the segments gathered in order: [seg_pre[kCandSegments]] records
+    uint32_t* seg_base;    // [kCandSegments + 1] split checks: records of each segment gathered so far,
+                           // then their total
     uint2* r2o_list;       // two-way checks: records of the poses obstacle->robot left unflagged, or null
     uint32_t* r2o_count;   // their number
     unsigned long long* tri_candidates;  // triangles in the emitted leaves
@@ -90,11 +95,14 @@ struct FkClear {
 // FK for n poses: T64, Ti, wbox for every link (exact FP64 sequence shared with the oracle).
 void launch_fk(const RobotDev* robot, int dof, int n_links, const float* poses, int n, double* T64, float4* Ti,
                float4* wbox, const FkClear& z, hipStream_t s);
-// Broad phase: one pass appending leaf candidates (wave-aggregated atomics).
-void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s);
+// Broad phase: passes p0 .. p1 - 1 (p1 < 0: to a.n_passes) appending leaf candidates (wave-aggregated
+// atomics).
+void launch_broad(const DiscreteArgs& a, int stack, bool count, hipStream_t s, int p0 = 0, int p1 = -1);
 // Gathers the candidate segments, in order, into one flat array (a.flat, a.seg_pre).  `small`: one
-// launch that scans and copies (for batches with few records).
-void launch_gather(const DiscreteArgs& a, bool small, hipStream_t s);
+// launch that scans and copies (for batches with few records).  `part` 1 gathers the records so far
+// and notes them in a.seg_base; part 2 appends the records emitted since (a.seg_pre[0] is then the
+// first of them, a.seg_pre[kCandSegments] the total).
+void launch_gather(const DiscreteArgs& a, bool small, hipStream_t s, int part = 0);
 // Narrow phase, one grid-stride pass in direction `dir` (1: obstacle->robot, 2: robot->obstacle)
 // over the gathered candidates.  `stack` covers the robot BVH depth.
 void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStream_t s);

@@ -214,3 +214,38 @@ def test_pinned_small_batches_and_graph_replays(ctx, panda, orc_panda):
             assert _mismatch(ctx.check_discrete(sid, q[700:703], 1), b_ref) == 0
     finally:
         ctx.destroy_scene(sid)
+
+
+def test_split_checks_match_oracle(ctx, panda, orc_panda):
+    """Large-batch schedule with early obstacle->robot passes between the broad passes (the later
+    broad passes skip flagged poses): every mask of split points gives the oracle's flags, also on
+    graph replays, and the counting launches agree."""
+    from rtcollisiondetection_amd._lib import RtcdError
+    sc = scenes.scene_dense()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    q = samplers.uniform_poses(1024, panda.lower, panda.upper, seed=21)
+    m = O.scene_mesh(sc)
+    refs = {d: O.check_discrete(orc_panda, m, q, d) for d in (1, 3)}
+    assert ctx.get_param('o2r_split') == 0b1010
+    old = ctx.get_param('frontier_max_tasks')
+    ctx.set_param('frontier_max_tasks', 0)  # one task per (pose, link) from the root's children
+    try:
+        for mask in (0, 1, 4, 0b101, 0b1010100, 1 << 10):
+            ctx.set_param('o2r_split', mask)
+            for d in (1, 3):
+                for _ in range(3):  # eager, capture, replay
+                    assert _mismatch(ctx.check_discrete(sid, q, d), refs[d]) == 0, (mask, d)
+        ctx.set_param('o2r_split', 0b101)
+        ctx.set_counting(True)
+        try:
+            assert _mismatch(ctx.check_discrete(sid, q, 3), refs[3]) == 0
+        finally:
+            ctx.set_counting(False)
+        with pytest.raises(RtcdError):
+            ctx.set_param('o2r_split', 0b1111)
+        with pytest.raises(RtcdError):
+            ctx.set_param('o2r_split', 1 << 16)
+    finally:
+        ctx.set_param('o2r_split', 0b1010)
+        ctx.set_param('frontier_max_tasks', old)
+    ctx.destroy_scene(sid)
