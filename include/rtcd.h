@@ -271,7 +271,8 @@ typedef enum {
     RTCD_PARAM_SERVE_LANE_KINDS = 24,  /* serving: request kinds slot 0 takes, bit (RTCD_REQ_* - 1) */
     RTCD_PARAM_TINY_BATCH = 25,        /* discrete batches of at most this many poses (default 4) run 2 broad
                                           passes and one robot->obstacle pass; 0 turns it off */
-    RTCD_PARAM_O2R_SPLIT = 26          /* discrete checks with obstacle->robot on large batches: bit k
+    RTCD_PARAM_O2R_SPLIT = 26          /* discrete checks with obstacle->robot on batches of at least
+                                          RTCD_PARAM_FRONTIER_MAX_TASKS (pose, link) tasks: bit k
                                           (at most 3 bits; default bits 1 and 3, 0 = off) runs an obstacle->robot
                                           pass over the records so far after broad pass k, and the later
                                           broad passes skip the poses it flagged */

@@ -197,7 +197,9 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     // mostly walk the poses deep in clutter, which collide: the early passes flag many of them and the
     // later broad passes skip their walks.  The "broad" stage then holds the early gathers and
     // obstacle->robot passes too.
-    const uint32_t split = (dir & 1) && !tiny && a.F > 0 && a.n_passes > 1
+    // Only batches of at least frontier_max_tasks tasks split: below, the extra launches cost more
+    // than the split saves (4,096 RTCD poses on its 71,746-triangle shelf: 0.237 -> 0.256 us per pose).
+    const uint32_t split = (dir & 1) && !tiny && a.F > 0 && a.n_passes > 1 && tasks >= c->frontier_max_tasks
                                ? c->o2r_split & ((1u << (a.n_passes - 1)) - 1u) : 0u;
     // obstacle->robot part i counts its hand-offs in scalars [o2r_ctr(i) ..+16): fresh counters each
     auto o2r_ctr = [](int i) { return i == 0 ? 24 : 40 + 16 * i; };
