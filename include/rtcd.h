@@ -271,11 +271,11 @@ typedef enum {
     RTCD_PARAM_SERVE_LANE_KINDS = 24,  /* serving: request kinds slot 0 takes, bit (RTCD_REQ_* - 1) */
     RTCD_PARAM_TINY_BATCH = 25,        /* discrete batches of at most this many poses (default 4) run 2 broad
                                           passes and one robot->obstacle pass; 0 turns it off */
-    RTCD_PARAM_O2R_SPLIT = 26          /* discrete checks with obstacle->robot on batches of at least
-                                          RTCD_PARAM_FRONTIER_MAX_TASKS (pose, link) tasks: bit k
-                                          (at most 3 bits; default bits 1 and 3, 0 = off) runs an obstacle->robot
-                                          pass over the records so far after broad pass k, and the later
-                                          broad passes skip the poses it flagged */
+    RTCD_PARAM_O2R_SPLIT = 26          /* discrete checks on batches of at least RTCD_PARAM_FRONTIER_MAX_TASKS
+                                          (pose, link) tasks: bit k (at most 3 bits; default bits 1 and 3,
+                                          0 = off) runs an obstacle->robot pass (robot->obstacle for
+                                          RTCD_R2O checks) over the records so far after broad pass k, and
+                                          the later broad passes skip the poses it flagged */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 /* Rebuilds the robot's and every scene's BVHs with other leaf sizes (1..16; 0 keeps one): the robot

@@ -199,8 +199,15 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     // obstacle->robot passes too.
     // Only batches of at least frontier_max_tasks tasks split: below, the extra launches cost more
     // than the split saves (4,096 RTCD poses on its 71,746-triangle shelf: 0.237 -> 0.256 us per pose).
-    const uint32_t split = (dir & 1) && !tiny && a.F > 0 && a.n_passes > 1 && tasks >= c->frontier_max_tasks
+    // A robot->obstacle-only check splits the same way, with robot->obstacle parts.
+    const uint32_t split = !tiny && a.F > 0 && a.n_passes > 1 && tasks >= c->frontier_max_tasks
                                ? c->o2r_split & ((1u << (a.n_passes - 1)) - 1u) : 0u;
+    const int part_dir = dir == 2 ? 2 : 1;
+    if (split && part_dir == 2) {
+        a.narrow_budget = c->narrow_budget;
+        a.narrow_budget_next = c->narrow_budget_next;
+        a.narrow_passes = (int) c->narrow_passes;
+    }
     // obstacle->robot part i counts its hand-offs in scalars [o2r_ctr(i) ..+16): fresh counters each
     auto o2r_ctr = [](int i) { return i == 0 ? 24 : 40 + 16 * i; };
     mark(1);
@@ -212,18 +219,18 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
             launch_broad(a, broad_stack, ws.counting, s, p0, p + 1);
             mark(2);
             launch_gather(a, false, s, i == 0 ? 1 : 2);
-            mark(3);
+            mark(part_dir + 2);
             a.nspill_count = ws.scalars.p + o2r_ctr(i++);
-            launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
+            launch_narrow(a, part_dir, c->robot_depth + 1, ws.counting, s);
             p0 = p + 1;
         }
         mark(1);
         launch_broad(a, broad_stack, ws.counting, s, p0, a.n_passes);
         mark(2);
         launch_gather(a, false, s, 2);
-        mark(3);
+        mark(part_dir + 2);
         a.nspill_count = ws.scalars.p + o2r_ctr(i);
-        launch_narrow(a, 1, c->robot_depth + 1, ws.counting, s);
+        launch_narrow(a, part_dir, c->robot_depth + 1, ws.counting, s);
     } else {
         if (a.F > 0) launch_broad(a, broad_stack, ws.counting, s);
         mark(2);
@@ -245,7 +252,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.narrow_budget = c->narrow_budget;
     a.narrow_budget_next = c->narrow_budget_next;
     a.narrow_passes = tiny ? 1 : (int) c->narrow_passes;
-    if (dir & 2) launch_narrow(a, 2, c->robot_depth + 1, ws.counting, s);
+    if ((dir & 2) && !(split && part_dir == 2)) launch_narrow(a, 2, c->robot_depth + 1, ws.counting, s);
     launch_redecide(a, tiny, ws.scalars.p, ws.pin_status.d, ws.pub_flags, s);
     mark(-1);
     ws.timing_pending = events;

@@ -977,8 +977,8 @@ __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
     Ctr ctr;
     if (pass == 0) {
         __shared__ uint2 lq[BS / 64][128];
-        // obstacle->robot: the records of the latest gather (a split check gathers in parts)
-        const uint32_t first = DIR == 1 ? a.seg_pre[0] : 0u;
+        // the gathered list: the records of the latest gather (a split check gathers in parts)
+        const uint32_t first = DIR == 1 || !a.r2o_list ? a.seg_pre[0] : 0u;
         const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat + first;
         const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg] - first;
         // a pass that may hand off and one that never does (the walks of the latter keep fewer registers)

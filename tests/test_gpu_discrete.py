@@ -225,14 +225,14 @@ def test_split_checks_match_oracle(ctx, panda, orc_panda):
     sid = ctx.create_scene(sc.verts, sc.tris)
     q = samplers.uniform_poses(1024, panda.lower, panda.upper, seed=21)
     m = O.scene_mesh(sc)
-    refs = {d: O.check_discrete(orc_panda, m, q, d) for d in (1, 3)}
+    refs = {d: O.check_discrete(orc_panda, m, q, d) for d in (1, 2, 3)}
     assert ctx.get_param('o2r_split') == 0b1010
     old = ctx.get_param('frontier_max_tasks')
     ctx.set_param('frontier_max_tasks', 0)  # one task per (pose, link) from the root's children
     try:
         for mask in (0, 1, 4, 0b101, 0b1010100, 1 << 10):
             ctx.set_param('o2r_split', mask)
-            for d in (1, 3):
+            for d in (1, 2, 3):
                 for _ in range(3):  # eager, capture, replay
                     assert _mismatch(ctx.check_discrete(sid, q, d), refs[d]) == 0, (mask, d)
         ctx.set_param('o2r_split', 0b101)
