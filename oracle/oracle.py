@@ -58,6 +58,12 @@ def lib():
         L.orc_check_cubic.argtypes = [C.c_void_p, C.c_void_p, _f32p, C.c_int, C.c_int, C.c_int, _f64p, _u32p,
                                       C.c_void_p, C.c_int]
         L.orc_cubic_piece.argtypes = [_f64p, _f64p]
+        L.orc_quad_piece.argtypes = [_f64p, _f64p]
+        L.orc_check_linear_brute.argtypes = [C.c_void_p, C.c_void_p, _f32p, _u32p, C.c_int, _u32p, C.c_void_p]
+        L.orc_check_quadratic_brute.argtypes = [C.c_void_p, C.c_void_p, _f32p, C.c_int, C.c_int, C.c_int, _f64p,
+                                                _u32p, C.c_void_p]
+        L.orc_check_cubic_brute.argtypes = [C.c_void_p, C.c_void_p, _f32p, C.c_int, C.c_int, C.c_int, _f64p,
+                                            _u32p, C.c_void_p]
         L.orc_fit_ctrl.argtypes = [_f64p, C.c_int, C.c_int, _f64p, _f64p]
         _lib = L
     return _lib
@@ -114,6 +120,14 @@ def cubic_piece(P):
     P = np.ascontiguousarray(np.asarray(P, dtype=np.float64).reshape(4, 3))
     out = np.zeros((4, 3))
     lib().orc_cubic_piece(P, out)
+    return out
+
+
+def quad_piece(P):
+    """Bezier controls [3, 3] of the uniform quadratic B-spline span with control points P [3, 3]."""
+    P = np.ascontiguousarray(np.asarray(P, dtype=np.float64).reshape(3, 3))
+    out = np.zeros((3, 3))
+    lib().orc_quad_piece(P, out)
     return out
 
 
@@ -219,3 +233,35 @@ def check_cubic(robot, mesh, wp, W, with_segments=False, nthreads=None):
     lib().orc_check_cubic(robot.h, mesh.h, wp, n, nwp, nc, W, out,
                           seg.ctypes.data_as(C.c_void_p) if seg is not None else None, nt)
     return (out, seg) if with_segments else out
+
+
+def check_linear_brute(robot, mesh, traj, offsets):
+    """(flags, per-segment flags) of the linear sweep with every edge against every capsule."""
+    traj = np.ascontiguousarray(traj, dtype=np.float32)
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+    n = len(offsets) - 1
+    out = np.zeros(n, dtype=np.uint32)
+    seg = np.zeros(max(int(offsets[-1]) - n, 1), dtype=np.uint32)
+    lib().orc_check_linear_brute(robot.h, mesh.h, traj, offsets, n, out, seg.ctypes.data_as(C.c_void_p))
+    return out, seg[:int(offsets[-1]) - n]
+
+
+def _curve_brute(fn, deg, robot, mesh, wp, W):
+    wp = np.ascontiguousarray(wp, dtype=np.float32)
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    n, nwp = wp.shape[0], wp.shape[1]
+    nc = W.shape[0]
+    out = np.zeros(n, dtype=np.uint32)
+    seg = np.zeros((n, nc - deg), dtype=np.uint32)
+    fn(robot.h, mesh.h, wp, n, nwp, nc, W, out, seg.ctypes.data_as(C.c_void_p))
+    return out, seg
+
+
+def check_quadratic_brute(robot, mesh, wp, W):
+    """(flags, per-piece flags) of the quadratic sweep with every edge against every tube."""
+    return _curve_brute(lib().orc_check_quadratic_brute, 2, robot, mesh, wp, W)
+
+
+def check_cubic_brute(robot, mesh, wp, W):
+    """(flags, per-piece flags) of the cubic sweep with every edge against every tube."""
+    return _curve_brute(lib().orc_check_cubic_brute, 3, robot, mesh, wp, W)

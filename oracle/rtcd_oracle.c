@@ -731,6 +731,17 @@ void orc_check_linear(const orc_robot* rb, const orc_mesh* scene, const float* t
     par_for(n_traj, nthreads, lin_one, &c);
 }
 
+/* Bezier controls of span p of the uniform quadratic B-spline with control points P (the span
+ * OPTIX_PRIMITIVE_TYPE_ROUND_QUADRATIC_BSPLINE draws, RTCD/Robot/batchCurveRobot.h:379-402):
+ * (P0 + P1) / 2, P1, (P1 + P2) / 2. */
+void orc_quad_piece(const double* P, double* ctrl) {
+    for (int cc = 0; cc < 3; ++cc) {
+        ctrl[cc] = (P[cc] + P[3 + cc]) * 0.5;
+        ctrl[3 + cc] = P[3 + cc];
+        ctrl[6 + cc] = (P[3 + cc] + P[6 + cc]) * 0.5;
+    }
+}
+
 static void quad_one(void* c, int i) {
     sweep_ctx* x = (sweep_ctx*) c;
     const orc_robot* rb = x->rb;
@@ -747,11 +758,7 @@ static void quad_one(void* c, int i) {
         for (int p = 0; p < npc; ++p) {
             if (seg[p]) continue;
             double ctrl[9];
-            for (int cc = 0; cc < 3; ++cc) {
-                ctrl[cc] = (P[3 * p + cc] + P[3 * (p + 1) + cc]) * 0.5;
-                ctrl[3 + cc] = P[3 * (p + 1) + cc];
-                ctrl[6 + cc] = (P[3 * (p + 1) + cc] + P[3 * (p + 2) + cc]) * 0.5;
-            }
+            orc_quad_piece(P + 3 * p, ctrl);
             seg[p] = (uint32_t) tube_vs_scene(x->sc, 2, ctrl, rb->sph[4 * s + 3]);
         }
     }
@@ -818,4 +825,79 @@ void orc_check_cubic(const orc_robot* rb, const orc_mesh* scene, const float* wp
     int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg, int nthreads) {
     sweep_ctx c = {rb, scene, wp, NULL, out, out_seg, n_wp, n_ctrl, W};
     par_for(n_traj, nthreads, cubic_one, &c);
+}
+
+/* ------------------------------------------------------------------------------------------------
+ * Brute-force sweeps: every scene edge against every tube, no BVH and no culling boxes.  They share
+ * only the pinned building blocks (orc_sphere_centers, orc_fit_ctrl, orc_quad_piece,
+ * orc_cubic_piece, orc_tube_decide) with the BVH versions above; the loops, the per-segment
+ * placement and the any-hit bookkeeping are restated here so that the BVH versions can be checked
+ * against them on small inputs.
+ * ---------------------------------------------------------------------------------------------- */
+static int tube_vs_all_edges(const orc_mesh* sc, int deg, const double* ctrl, double r) {
+    for (int e = 0; e < sc->ne; ++e)
+        if (orc_tube_decide(deg, ctrl, r, sc->v + 3 * sc->e[2 * e], sc->v + 3 * sc->e[2 * e + 1])) return 1;
+    return 0;
+}
+
+void orc_check_linear_brute(const orc_robot* rb, const orc_mesh* sc, const float* traj,
+    const uint32_t* offsets, int n_traj, uint32_t* out, uint32_t* out_seg) {
+    /* segments are numbered across trajectories in order: trajectory i's segments follow the
+     * (offsets[i] - i) segments of trajectories 0..i-1 */
+    size_t gseg = 0;
+    double* c0 = (double*) malloc(sizeof(double) * 3 * (rb->ns > 0 ? rb->ns : 1));
+    double* c1 = (double*) malloc(sizeof(double) * 3 * (rb->ns > 0 ? rb->ns : 1));
+    for (int i = 0; i < n_traj; ++i) {
+        out[i] = 0;
+        for (uint32_t w = offsets[i]; w + 1 < offsets[i + 1]; ++w, ++gseg) {
+            orc_sphere_centers(rb, traj + (size_t) w * rb->dof, c0);
+            orc_sphere_centers(rb, traj + (size_t) (w + 1) * rb->dof, c1);
+            uint32_t hit = 0;
+            for (int s = 0; s < rb->ns; ++s) {
+                double ctrl[6] = {c0[3 * s], c0[3 * s + 1], c0[3 * s + 2], c1[3 * s], c1[3 * s + 1], c1[3 * s + 2]};
+                if (tube_vs_all_edges(sc, 1, ctrl, rb->sph[4 * s + 3])) hit = 1;
+            }
+            if (out_seg) out_seg[gseg] = hit;
+            if (hit) out[i] = 1;
+        }
+    }
+    free(c0); free(c1);
+}
+
+static void curve_brute(const orc_robot* rb, const orc_mesh* sc, int deg, const float* wp, int n_traj,
+    int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
+    const int npc = n_ctrl - deg;
+    const int ns = rb->ns > 0 ? rb->ns : 1;
+    double* cen = (double*) malloc(sizeof(double) * 3 * ns * n_wp);   /* [wp][sphere][3] */
+    double* X = (double*) malloc(sizeof(double) * 3 * n_wp);
+    double* P = (double*) malloc(sizeof(double) * 3 * n_ctrl);
+    for (int i = 0; i < n_traj; ++i) {
+        for (int k = 0; k < n_wp; ++k) orc_sphere_centers(rb, wp + ((size_t) i * n_wp + k) * rb->dof, cen + (size_t) 3 * ns * k);
+        out[i] = 0;
+        for (int p = 0; p < npc; ++p) {
+            uint32_t hit = 0;
+            for (int s = 0; s < rb->ns; ++s) {
+                for (int k = 0; k < n_wp; ++k)
+                    for (int cc = 0; cc < 3; ++cc) X[3 * k + cc] = cen[(size_t) 3 * ns * k + 3 * s + cc];
+                orc_fit_ctrl(W, n_ctrl, n_wp, X, P);
+                double ctrl[12];
+                if (deg == 2) orc_quad_piece(P + 3 * p, ctrl);
+                else orc_cubic_piece(P + 3 * p, ctrl);
+                if (tube_vs_all_edges(sc, deg, ctrl, rb->sph[4 * s + 3])) hit = 1;
+            }
+            if (out_seg) out_seg[(size_t) i * npc + p] = hit;
+            if (hit) out[i] = 1;
+        }
+    }
+    free(cen); free(X); free(P);
+}
+
+void orc_check_quadratic_brute(const orc_robot* rb, const orc_mesh* sc, const float* wp, int n_traj,
+    int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
+    curve_brute(rb, sc, 2, wp, n_traj, n_wp, n_ctrl, W, out, out_seg);
+}
+
+void orc_check_cubic_brute(const orc_robot* rb, const orc_mesh* sc, const float* wp, int n_traj,
+    int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
+    curve_brute(rb, sc, 3, wp, n_traj, n_wp, n_ctrl, W, out, out_seg);
 }

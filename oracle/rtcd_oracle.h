@@ -98,6 +98,17 @@ void orc_check_cubic(const orc_robot* rb, const orc_mesh* scene, const float* wp
     int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg, int nthreads);
 /* Bezier controls ctrl[12] of the uniform cubic B-spline span with control points P[4][3]. */
 void orc_cubic_piece(const double* P, double* ctrl);
+/* Bezier controls ctrl[9] of the uniform quadratic B-spline span with control points P[3][3]. */
+void orc_quad_piece(const double* P, double* ctrl);
+
+/* The three sweeps with every scene edge tested against every tube (no BVH, no culling boxes) and
+ * the same outputs: they pin the BVH versions on small inputs. */
+void orc_check_linear_brute(const orc_robot* rb, const orc_mesh* scene, const float* traj,
+    const uint32_t* offsets, int n_traj, uint32_t* out, uint32_t* out_seg);
+void orc_check_quadratic_brute(const orc_robot* rb, const orc_mesh* scene, const float* wp, int n_traj,
+    int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg);
+void orc_check_cubic_brute(const orc_robot* rb, const orc_mesh* scene, const float* wp, int n_traj,
+    int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg);
 
 /* Control points of one sphere path: X[n_wp][3] -> P[n_ctrl][3] (sweptVolume.cu:22-83). */
 void orc_fit_ctrl(const double* W, int n_ctrl, int n_wp, const double* X, double* P);

@@ -368,3 +368,119 @@ def test_skip_base_flag(panda, orc_panda):
     m = O.Mesh(v, t)
     q = samplers.uniform_poses(8, panda.lower, panda.upper, seed=4)
     assert O.check_discrete_brute(orc_panda, m, q, 3, skip_base=False).all()
+
+
+# ---------------------------------------------------------------- sweeps: spans, sphere centres, BVH vs brute force
+def test_quad_piece_is_the_uniform_bspline_span():
+    # span of the uniform quadratic B-spline: ((1-t)^2 / 2) P0 + ((-2t^2 + 2t + 1) / 2) P1 + (t^2 / 2) P2,
+    # against the Bernstein form of the Bezier controls orc_quad_piece gives
+    rng = np.random.default_rng(24)
+    t = np.linspace(0, 1, 33)[:, None]
+    for _ in range(50):
+        P = rng.uniform(-1, 1, (3, 3))
+        span = (1 - t) ** 2 / 2 * P[0] + (-2 * t ** 2 + 2 * t + 1) / 2 * P[1] + t ** 2 / 2 * P[2]
+        b = O.quad_piece(P)
+        bez = (1 - t) ** 2 * b[0] + 2 * t * (1 - t) * b[1] + t ** 2 * b[2]
+        assert np.abs(span - bez).max() < 1e-14
+
+
+def _sphere_link_violations(desc, centres_of, poses):
+    """Spheres whose centre is not on its own link: outside the link's world box grown by r, or
+    farther than r from every vertex of the link (each of the robot's spheres touches its link)."""
+    bad = 0
+    for q in poses:
+        T = O.fk(desc, q)
+        C = centres_of(q)
+        for s in range(len(desc.spheres)):
+            l = int(desc.sphere_link[s])
+            r = float(desc.spheres[s, 3])
+            Tl = T[l].reshape(3, 4)
+            W = desc.link_verts[l].astype(np.float64) @ Tl[:, :3].T + Tl[:, 3]
+            outside = np.any(C[s] < W.min(0) - r) or np.any(C[s] > W.max(0) + r)
+            far = np.sqrt(((W - C[s]) ** 2).sum(1)).min() > r * (1 + 1e-9)
+            bad += int(outside or far)
+    return bad
+
+
+def test_sphere_centres_sit_on_their_links(panda, orc_panda):
+    q = samplers.uniform_poses(40, panda.lower, panda.upper, seed=25)
+    assert _sphere_link_violations(panda, lambda p: O.sphere_centers(orc_panda, p), q) == 0
+    # planted mistake: centres moved by a neighbouring link's transform are caught
+    import copy
+    for shift in (1, -1):
+        wrong = copy.deepcopy(panda)
+        wrong.sphere_link = np.ascontiguousarray(np.clip(panda.sphere_link + shift, 0, panda.n_links - 1), np.int32)
+        orc_wrong = O.Robot(wrong)
+        assert _sphere_link_violations(panda, lambda p: O.sphere_centers(orc_wrong, p), q[:8]) > 100
+
+
+def _near_boxes():
+    return O.scene_mesh(scenes.random_boxes(6, seed=7, side=(0.04, 0.12), center_lo=(0.25, -0.4, 0.2),
+                                            center_hi=(0.6, 0.4, 0.8)))
+
+
+def test_linear_bvh_equals_brute_force(panda, orc_panda):
+    sc = _near_boxes()
+    traj, offs = samplers.random_walk_trajectories(24, panda.lower, panda.upper, lengths=(1, 7), max_step=0.3,
+                                                   seed=26)
+    a, sa = O.check_linear(orc_panda, sc, traj, offs, with_segments=True)
+    b, sb = O.check_linear_brute(orc_panda, sc, traj, offs)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    assert 0 < a.sum() < len(a) and 0 < sa.sum() < len(sa)
+
+
+def test_linear_segments_are_their_own_two_waypoint_sweeps(panda, orc_panda):
+    # segment k of trajectory i (index offsets[i] - i + k) is the capsule sweep of waypoints k, k+1 alone
+    sc = _near_boxes()
+    traj, offs = samplers.random_walk_trajectories(16, panda.lower, panda.upper, lengths=(2, 6), max_step=0.3,
+                                                   seed=27)
+    _, seg = O.check_linear(orc_panda, sc, traj, offs, with_segments=True)
+    pairs, k = [], 0
+    for i in range(len(offs) - 1):
+        for w in range(int(offs[i]), int(offs[i + 1]) - 1):
+            assert k == int(offs[i]) - i + (w - int(offs[i]))
+            pairs.append(traj[w:w + 2])
+            k += 1
+    alone = O.check_linear(orc_panda, sc, np.concatenate(pairs), np.arange(0, 2 * len(pairs) + 1, 2, dtype=np.uint32))
+    assert np.array_equal(seg, alone)
+    assert 0 < seg.sum() < len(seg)
+
+
+def _cubic_fit_weights(n_ctrl, n_wp):
+    # pinv of the uniform cubic B-spline basis sampled at u = linspace(0, n_ctrl - 3, n_wp)
+    u = np.linspace(0.0, n_ctrl - 3, n_wp)
+    B = np.zeros((n_wp, n_ctrl))
+    for i, uu in enumerate(u):
+        s = min(int(np.floor(uu)), n_ctrl - 4)
+        t = uu - s
+        B[i, s:s + 4] = [(1 - t) ** 3 / 6, (3 * t ** 3 - 6 * t ** 2 + 4) / 6, (-3 * t ** 3 + 3 * t ** 2 + 3 * t + 1) / 6,
+                         t ** 3 / 6]
+    return np.ascontiguousarray(np.linalg.pinv(B))
+
+
+@pytest.mark.parametrize('deg', [2, 3])
+def test_curve_bvh_equals_brute_force(panda, orc_panda, deg):
+    sc = _near_boxes()
+    wp, _ = samplers.bspline_trajectories(12, panda.lower, panda.upper, n_ctrl=6, n_wp=12, seed=28)
+    W = samplers.quadratic_fit_weights(6, 12) if deg == 2 else _cubic_fit_weights(6, 12)
+    fast = O.check_quadratic if deg == 2 else O.check_cubic
+    brute = O.check_quadratic_brute if deg == 2 else O.check_cubic_brute
+    a, sa = fast(orc_panda, sc, wp, W, with_segments=True)
+    b, sb = brute(orc_panda, sc, wp, W)
+    assert np.array_equal(a, b) and np.array_equal(sa, sb)
+    assert 0 < a.sum() < len(a) and 0 < sa.sum() < sa.size
+
+
+@pytest.mark.parametrize('deg', [2, 3])
+def test_curve_pieces_are_their_own_sweeps(panda, orc_panda, deg):
+    # with W = I the control points are the sphere centres at the waypoints, so piece p of a trajectory is
+    # the one-piece sweep of waypoints p .. p + deg alone
+    sc = _near_boxes()
+    n_ctrl = 7
+    wp, _ = samplers.bspline_trajectories(10, panda.lower, panda.upper, n_ctrl=5, n_wp=n_ctrl, seed=29)
+    fast = O.check_quadratic if deg == 2 else O.check_cubic
+    _, seg = fast(orc_panda, sc, wp, np.eye(n_ctrl), with_segments=True)
+    parts = np.stack([wp[i, p:p + deg + 1] for i in range(len(wp)) for p in range(n_ctrl - deg)])
+    alone = fast(orc_panda, sc, parts, np.eye(deg + 1))
+    assert np.array_equal(seg.reshape(-1), alone)
+    assert 0 < seg.sum() < seg.size
