@@ -84,6 +84,9 @@ typedef struct {
                                     are then -1: the graph records no inner events) */
     uint64_t r2o_records;        /* two-way: leaf records of the poses the obstacle->robot pass left
                                     unflagged, the only ones the robot->obstacle pass reads */
+    uint32_t reruns;             /* launches of the last check repeated because a candidate buffer or
+                                    FP64 queue overflowed (grown and re-run, or split in halves by the
+                                    allocation-free entry point) */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */
@@ -271,11 +274,20 @@ typedef enum {
     RTCD_PARAM_SERVE_LANE_KINDS = 24,  /* serving: request kinds slot 0 takes, bit (RTCD_REQ_* - 1) */
     RTCD_PARAM_TINY_BATCH = 25,        /* discrete batches of at most this many poses (default 4) run 2 broad
                                           passes and one robot->obstacle pass; 0 turns it off */
-    RTCD_PARAM_O2R_SPLIT = 26          /* discrete checks on batches of at least RTCD_PARAM_FRONTIER_MAX_TASKS
+    RTCD_PARAM_O2R_SPLIT = 26,         /* discrete checks on batches of at least RTCD_PARAM_FRONTIER_MAX_TASKS
                                           (pose, link) tasks: bit k (at most 3 bits; default bits 1 and 3,
                                           0 = off) runs an obstacle->robot pass (robot->obstacle for
                                           RTCD_R2O checks) over the records so far after broad pass k, and
                                           the later broad passes skip the poses it flagged */
+    /* Starting capacities of work buffers, in records (0 = automatic, the default).  They exist to
+     * exercise the overflow paths: a smaller buffer overflows, and the check grows it and re-runs
+     * (candidates, FP64 queues) or ends hand-offs early (hand-off queues, which then grow for later
+     * calls); results never change.  They apply when a buffer is first sized, so set them on a fresh
+     * context.  The allocation-free entry point caps its candidate buffer by RTCD_PARAM_TEST_CAND_CAP. */
+    RTCD_PARAM_TEST_CAND_CAP = 27,     /* discrete leaf-candidate records */
+    RTCD_PARAM_TEST_UNSURE_CAP = 28,   /* FP64 re-decision queue entries (discrete and swept) */
+    RTCD_PARAM_TEST_SPILL_CAP = 29,    /* broad-phase and sweep hand-off records (both halves) */
+    RTCD_PARAM_TEST_NSPILL_CAP = 30    /* narrow-phase hand-off records (both halves) */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 /* Rebuilds the robot's and every scene's BVHs with other leaf sizes (1..16; 0 keeps one): the robot

@@ -98,7 +98,8 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     }
     if (foff < 0) { F = 1; foff = 0; }
     const size_t items = tasks * (size_t) F;
-    const size_t need_cap = std::max<size_t>((size_t) 1 << 20, tasks * 64);
+    const size_t need_cap = c->test_cand_cap ? std::max<size_t>(c->test_cand_cap, kCandSegments)
+                                             : std::max<size_t>((size_t) 1 << 20, tasks * 64);
     if (ws.cand.n < need_cap) {
         budget_check(c, ws, budget, (need_cap - ws.cand.n) * sizeof(uint2));
         ws.cand.reserve(need_cap);
@@ -108,7 +109,8 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
         ws.flat.alloc(ws.cand.n);
     }
     ws.cand_cap = (uint32_t) std::min<size_t>(ws.cand.n, 0xffffffffu);
-    const size_t need_unsure = std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
+    const size_t need_unsure = c->test_unsure_cap ? (size_t) c->test_unsure_cap
+                                                  : std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
     if (ws.unsure.n < need_unsure) {
         budget_check(c, ws, budget, (need_unsure - ws.unsure.n) * sizeof(uint4));
         ws.unsure.reserve(need_unsure);
@@ -121,8 +123,10 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     (void) items;
     ws.scalars.reserve(kScalars);
     ws.stats.reserve(1);
-    if (ws.spill.n < ((size_t) 2 << 20)) ws.spill.reserve((size_t) 2 << 20);
-    if (ws.nspill.n < ((size_t) 1 << 20)) ws.nspill.reserve((size_t) 1 << 20);
+    const size_t need_spill = c->test_spill_cap ? std::max<size_t>(c->test_spill_cap, 2) : (size_t) 2 << 20;
+    const size_t need_nspill = c->test_nspill_cap ? std::max<size_t>(c->test_nspill_cap, 2) : (size_t) 1 << 20;
+    if (ws.spill.n < need_spill) ws.spill.reserve(need_spill);
+    if (ws.nspill.n < need_nspill) ws.nspill.reserve(need_nspill);
 
     ws.seg_count.reserve((size_t) kCandSegments * kSegStride);
     ws.seg_pre.reserve(kCandSegments + 1);
@@ -387,6 +391,7 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                 }
                 st->cert_calls = 0;
                 st->graph_replayed = &ws == &c->main && c->dgraph.last_replayed ? 1u : 0u;
+                st->reruns = (uint32_t) attempt;
             }
             ws.timing_pending = false;
             // a full hand-off queue costs time, not results: grow it (within the budget) for later calls
@@ -962,6 +967,10 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
             return fail(RTCD_ERR_INVALID_ARG, "split must be a mask of at most 3 of the bits 0..15");
         ctx->o2r_split = (uint32_t) value;
         return RTCD_OK;
+    case RTCD_PARAM_TEST_CAND_CAP: ctx->test_cand_cap = value; return RTCD_OK;
+    case RTCD_PARAM_TEST_UNSURE_CAP: ctx->test_unsure_cap = value; return RTCD_OK;
+    case RTCD_PARAM_TEST_SPILL_CAP: ctx->test_spill_cap = value; return RTCD_OK;
+    case RTCD_PARAM_TEST_NSPILL_CAP: ctx->test_nspill_cap = value; return RTCD_OK;
     case RTCD_PARAM_SERVE_LANE_KINDS:
         if (value < 1 || value > 15) return fail(RTCD_ERR_INVALID_ARG, "lane kinds must be a non-empty mask of bits 0..3");
         ctx->serve_lane_kinds = (uint32_t) value;
@@ -1009,6 +1018,10 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_SERVE_LANE_KINDS: *value = ctx->serve_lane_kinds; return RTCD_OK;
     case RTCD_PARAM_TINY_BATCH: *value = ctx->tiny_batch; return RTCD_OK;
     case RTCD_PARAM_O2R_SPLIT: *value = ctx->o2r_split; return RTCD_OK;
+    case RTCD_PARAM_TEST_CAND_CAP: *value = ctx->test_cand_cap; return RTCD_OK;
+    case RTCD_PARAM_TEST_UNSURE_CAP: *value = ctx->test_unsure_cap; return RTCD_OK;
+    case RTCD_PARAM_TEST_SPILL_CAP: *value = ctx->test_spill_cap; return RTCD_OK;
+    case RTCD_PARAM_TEST_NSPILL_CAP: *value = ctx->test_nspill_cap; return RTCD_OK;
     default: return fail(RTCD_ERR_INVALID_ARG, "unknown parameter");
     }
 }

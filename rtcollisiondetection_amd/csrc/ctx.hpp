@@ -274,6 +274,9 @@ struct rtcd_ctx {
     // two-way check 2.62 ms unsplit, 1.99 after pass 2 only, 1.95 after 1 and 3, 1.97 after 1, 3 and 5.
     uint32_t o2r_split = (1u << 1) | (1u << 3);
     uint32_t tiny_batch = 4;  // discrete batches of at most this many poses run 2 broad passes, 1 narrow pass
+    // starting capacities (records) of the candidate buffer, FP64 queues and hand-off queues; 0 = the
+    // automatic sizes.  They exist to exercise the overflow paths (RTCD_PARAM_TEST_*_CAP).
+    uint64_t test_cand_cap = 0, test_unsure_cap = 0, test_spill_cap = 0, test_nspill_cap = 0;
     // serving loop
     ServeState* serve = nullptr;
     size_t serve_bytes = 0;  // device bytes held by serving slots

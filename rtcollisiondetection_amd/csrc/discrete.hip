@@ -225,8 +225,12 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
     // Block b owns candidate segment b.  In pass 0 it walks one contiguous chunk of the (task,
     // frontier entry) items, so the segments in order keep the link-major task order (poses flagged
     // through an earlier link skip the later ones in the narrow phase); later passes stride.
-    uint32_t* seg_count = a.seg_count + (size_t) blockIdx.x * kSegStride;
-    uint2* seg_cand = a.cand + (size_t) blockIdx.x * a.seg_cap;
+    // Later passes rotate the block -> segment map by 167 segments per pass: a short hand-off queue runs
+    // on the first few blocks, and without the rotation their segments collected the records of every
+    // pass (a 4,096-pose dense check needed 8,192 records in one segment for 765,710 in all).
+    const size_t seg = pass == 0 ? blockIdx.x : (blockIdx.x + (size_t) pass * 167u) % kSeg;
+    uint32_t* seg_count = a.seg_count + seg * kSegStride;
+    uint2* seg_cand = a.cand + seg * a.seg_cap;
     const uint2* in_q = pass > 0 ? ((pass - 1) & 1 ? a.spill[1] : a.spill[0]) : nullptr;
     uint2* out_q = (pass & 1 ? a.spill[1] : a.spill[0]);
     uint32_t* out_cnt = a.spill_count + pass;

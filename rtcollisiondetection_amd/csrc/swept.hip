@@ -485,8 +485,10 @@ __global__ __launch_bounds__(kBS) void k_sweep(SweepArgs a, int pass) {
                 for (int j = 0; j < sp; ++j) out_q[o + 1 + j] = make_uint2(it, (uint32_t) st[j * kBS]);
                 return false;
             }
-            // queue full: mark the reserved slots empty and finish this subtree here
+            // queue full: mark the reserved slots empty and finish this subtree here (bit 16 lets the
+            // host grow the queue for later calls)
             for (uint64_t j = o; j < a.spill_cap && j < (uint64_t) o + want; ++j) out_q[j] = make_uint2(0xffffffffu, 0u);
+            atomicOr(a.overflow, 16u);
             spill_ok = false;
         }
         const BNode nd = a.sc.edge_nodes[ni];
@@ -808,10 +810,11 @@ static SweepArgs base_args(rtcd_ctx* c, Workspace& ws, SceneRes& sc, uint32_t* d
     return a;
 }
 
-static void sweep_begin(Workspace& ws, bool events) {
+static void sweep_begin(const rtcd_ctx* c, Workspace& ws, bool events) {
     ws.stats.reserve(1);
     ws.scalars.reserve(kScalars);
-    if (ws.unsure.n < ((size_t) 1 << 16)) ws.unsure.reserve((size_t) 1 << 16);
+    const size_t need_unsure = c->test_unsure_cap ? (size_t) c->test_unsure_cap : (size_t) 1 << 16;
+    if (ws.unsure.n < need_unsure) ws.unsure.reserve(need_unsure);
     fill_u32(ws.scalars.p, kScalars, 0u, ws.stream);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, ws.stream);
     if (events) RTCD_HIP(hipEventRecord(ws.ev[0], ws.stream));
@@ -822,7 +825,7 @@ static void sweep_begin(Workspace& ws, bool events) {
 static void sweep_work(rtcd_ctx* c, Workspace& ws, SweepArgs& a, size_t items, size_t budget) {
     if (items > 0xffffffffull) throw std::invalid_argument("too many (segment, sphere) items in one call");
     a.n_items = (uint32_t) items;
-    const size_t need = (size_t) 2 << 20;
+    const size_t need = c->test_spill_cap ? std::max<size_t>(c->test_spill_cap, 2) : (size_t) 2 << 20;
     if (ws.cand.n < need) {
         if (budget && ws.bytes() + (need - ws.cand.n) * sizeof(uint2) > budget)
             throw HipError(hipErrorOutOfMemory, "request needs more than its memory budget");
@@ -844,15 +847,20 @@ static void sweep_end(Workspace& ws, bool events) {
 void finish_sweep(rtcd_ctx* c, Workspace& ws, rtcd_stats* st, const std::function<void()>& rerun) {
     RTCD_HIP(hipStreamSynchronize(ws.stream));
     std::vector<uint32_t> sv;
-    for (int attempt = 0;; ++attempt) {
+    uint32_t reruns = 0;
+    for (;; ++reruns) {
         sv = ws.scalars.download(3, ws.stream);
         if (sv[1] & 2u) throw std::runtime_error("traversal stack overflow (BVH deeper than the stack)");
         if (!(sv[1] & 4u)) break;
-        if (attempt == 3 || !rerun) throw std::runtime_error("FP64 re-decision queue overflow persisted");
+        if (reruns == 3 || !rerun) throw std::runtime_error("FP64 re-decision queue overflow persisted");
         ws.unsure.alloc((size_t) sv[2] + sv[2] / 4 + 1024);
         rerun();
         RTCD_HIP(hipStreamSynchronize(ws.stream));
     }
+    if (st) st->reruns = reruns;
+    // a full hand-off queue costs time, not results: grow it for later calls (it is the candidate
+    // buffer's memory, which discrete checks on this workspace also use)
+    if ((sv[1] & 16u) && ws.cand.n < ((size_t) 1 << 27)) ws.cand.reserve(std::min<size_t>(ws.cand.n * 2, (size_t) 1 << 27));
     if (st) {
         if (ws.timing_pending) {
             float ms = 0.f;
@@ -884,7 +892,7 @@ void enqueue_linear(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_tra
                     uint32_t n_traj, uint32_t nw, uint32_t* d_out, uint32_t* d_seg, size_t budget, bool events) {
     const hipStream_t s = ws.stream;
     const int n_seg = (int) (nw - n_traj);
-    sweep_begin(ws, events);
+    sweep_begin(c, ws, events);
     fill_u32(d_out, n_traj, 0u, s);
     if (d_seg && n_seg > 0) fill_u32(d_seg, (size_t) n_seg, 0u, s);
     enqueue_centers(c, ws, d_traj, (int) nw, false);
@@ -930,7 +938,7 @@ void enqueue_curve(rtcd_ctx* c, Workspace& ws, SceneRes& sc, const float* d_wp, 
     const int S = c->n_spheres;
     const int npc = (int) nc - degree;
     prepare_weights(ws, nc, nw, W, degree);
-    sweep_begin(ws, events);
+    sweep_begin(c, ws, events);
     fill_u32(d_out, n_traj, 0u, s);
     if (d_seg) fill_u32(d_seg, (size_t) n_traj * npc, 0u, s);
     enqueue_centers(c, ws, d_wp, (int) (n_traj * nw), true);
