@@ -120,6 +120,24 @@ rtcd_status rtcd_check_discrete_device(rtcd_ctx* ctx, uint32_t scene_id, int32_t
                                        uint32_t n, uint32_t* d_out, void* stream);
 rtcd_status rtcd_sync(rtcd_ctx* ctx, void* stream);
 
+/* Allocation-free variant of rtcd_check_discrete_device.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory while checking (tight serving loops, harnesses that
+ * forbid hipMalloc).  The call makes no hipMalloc / hipHostMalloc / hipFree: every work buffer lives
+ * in one static device array of RTCD_NOALLOC_SCRATCH_BYTES per GPU, held from the first context on
+ * that GPU until the process ends.  The robot and the scene must be loaded beforehand.
+ * Same kernels and results as rtcd_check_discrete_device; it waits for `stream` (NULL: the context's
+ * stream) and returns when d_out holds the flags.  n <= RTCD_NOALLOC_MAX_POSES (else
+ * RTCD_ERR_INVALID_ARG).  A batch whose candidates or FP64 tests overflow the scratch is re-run in
+ * smaller parts (rtcd_stats.reruns counts them): candidates by sub-batches that start at the scene
+ * BVH's all-leaves level (bounded records per segment), FP64 tests by halves; RTCD_ERR_OOM only if
+ * one pose does not fit.  Every call on
+ * one GPU shares that GPU's array and it keeps the last call's contents: the library serialises the
+ * calls of one process per GPU, and calls that overlapped on one GPU would race. */
+#define RTCD_NOALLOC_MAX_POSES 4096u
+#define RTCD_NOALLOC_SCRATCH_BYTES ((size_t) 96 << 20)
+rtcd_status rtcd_check_discrete_device_noalloc(rtcd_ctx* ctx, uint32_t scene_id, int32_t direction,
+                                               const float* d_poses, uint32_t n, uint32_t* d_out, void* stream);
+
 /* Piecewise-linear sweep (RTCD LINEAR_CURVE, RTCD/Robot/batchCurveRobot.h:346-378): trajectory i
  * has waypoints offsets[i]..offsets[i+1]-1 of traj [*][dof]; one capsule per sphere per waypoint
  * segment.  out [n_traj]; out_seg (nullable) [offsets[n_traj] - n_traj], segment k of trajectory

@@ -22,7 +22,7 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
            'rtcd_check_cubic', 'rtcd_check_cubic_device', 'rtcd_quadratic_weights', 'rtcd_curve_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
            'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_wait_timed', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param',
            'rtcd_set_leaf_size', 'rtcd_set_leaf_sizes', 'rtcd_autotune', 'rtcd_obj_read', 'rtcd_mesh_edges',
-           'rtcd_urdf_read', 'rtcd_spheres_read', 'rtcd_robot_load_urdf']
+           'rtcd_urdf_read', 'rtcd_spheres_read', 'rtcd_robot_load_urdf', 'rtcd_check_discrete_device_noalloc']
 TUNE_LEAF_SIZE = 1
 PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad_budget': 4,
           'broad_budget_small': 5, 'broad_budget_next': 6, 'broad_passes': 7, 'sweep_passes': 8, 'graphs': 9,
@@ -31,6 +31,8 @@ PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad
           'leaf_scene_edge': 19, 'serve_lane_max': 20, 'serve_lane_cus': 21, 'serve_chunk_discrete': 22,
           'serve_chunk_sweep': 23, 'serve_lane_kinds': 24, 'tiny_batch': 25,
           'o2r_split': 26, 'test_cand_cap': 27, 'test_unsure_cap': 28, 'test_spill_cap': 29, 'test_nspill_cap': 30}
+NOALLOC_MAX_POSES = 4096
+NOALLOC_SCRATCH_BYTES = 96 << 20
 REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC, REQ_CUBIC = 1, 2, 3, 4
 
 
@@ -120,6 +122,7 @@ def lib():
         L.rtcd_check_discrete.argtypes = [vp, u32, i32, vp, u32, vp]
         L.rtcd_check_discrete_device.argtypes = [vp, u32, i32, vp, u32, vp, vp]
         L.rtcd_sync.argtypes = [vp, vp]
+        L.rtcd_check_discrete_device_noalloc.argtypes = [vp, u32, i32, vp, u32, vp, vp]
         L.rtcd_check_linear.argtypes = [vp, u32, vp, vp, u32, vp, vp]
         L.rtcd_check_linear_device.argtypes = [vp, u32, vp, vp, u32, u32, vp, vp, vp]
         L.rtcd_check_quadratic.argtypes = [vp, u32, vp, u32, u32, u32, vp, vp, vp]

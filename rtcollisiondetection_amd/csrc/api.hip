@@ -89,49 +89,62 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     // split each (pose, link) task over F frontier entries so that small batches still fill the GPU
     // (large batches already fill the device: one thread per task walks the whole tree)
     int F = 1, foff = -1;
-    if (tasks < c->frontier_max_tasks) {
+    if (ws.deepest_frontier && sc.front_leaves && !sc.front_levels.empty()) {
+        F = sc.front_levels.back().second;
+        foff = sc.front_levels.back().first;
+    } else if (tasks < c->frontier_max_tasks) {
+        const uint64_t work = std::max<uint64_t>(1, c->frontier_work / std::max<uint64_t>(1, ws.frontier_div));
         for (const auto& lv : sc.front_levels)
-            if (tasks * (size_t) lv.second <= c->frontier_work || foff < 0) { F = lv.second; foff = lv.first; }
+            if (tasks * (size_t) lv.second <= work || foff < 0) { F = lv.second; foff = lv.first; }
     } else if (!sc.front_levels.empty()) {
         F = sc.front_levels[0].second;  // the root's children
         foff = sc.front_levels[0].first;
     }
     if (foff < 0) { F = 1; foff = 0; }
-    const size_t items = tasks * (size_t) F;
-    const size_t need_cap = c->test_cand_cap ? std::max<size_t>(c->test_cand_cap, kCandSegments)
-                                             : std::max<size_t>((size_t) 1 << 20, tasks * 64);
-    if (ws.cand.n < need_cap) {
-        budget_check(c, ws, budget, (need_cap - ws.cand.n) * sizeof(uint2));
-        ws.cand.reserve(need_cap);
-    }
-    if (ws.flat.n < ws.cand.n) {
-        budget_check(c, ws, budget, (ws.cand.n - ws.flat.n) * sizeof(uint2));
-        ws.flat.alloc(ws.cand.n);
+    ws.last_frontier = sc.front_levels.empty() ? 0 : F;
+    const size_t npl = (size_t) n * NL;
+    if (ws.fixed) {
+        // fixed memory: every buffer must already hold this batch (sizes of the per-pose buffers are
+        // exact; the candidate, FP64 and hand-off buffers may be of any size, overflows are reported)
+        if (ws.T64.n < npl * 12 || ws.Ti.n < npl * 3 || ws.wbox.n < npl * 2 || ws.cand.n < kCandSegments ||
+            ws.flat.n < ws.cand.n || ws.unsure.n < 1 || ws.scalars.n < (size_t) kScalars || ws.stats.n < 1 ||
+            ws.spill.n < 2 || ws.nspill.n < 2 || ws.seg_count.n < (size_t) kCandSegments * kSegStride ||
+            ws.seg_pre.n < (size_t) kCandSegments + 1 || ws.seg_base.n < (size_t) kCandSegments + 1 ||
+            ws.pin_status.n < (size_t) kPublish)
+            throw std::runtime_error("fixed workspace too small for this batch");
+    } else {
+        const size_t need_cap = c->test_cand_cap ? std::max<size_t>(c->test_cand_cap, kCandSegments)
+                                                 : std::max<size_t>((size_t) 1 << 20, tasks * 64);
+        if (ws.cand.n < need_cap) {
+            budget_check(c, ws, budget, (need_cap - ws.cand.n) * sizeof(uint2));
+            ws.cand.reserve(need_cap);
+        }
+        if (ws.flat.n < ws.cand.n) {
+            budget_check(c, ws, budget, (ws.cand.n - ws.flat.n) * sizeof(uint2));
+            ws.flat.alloc(ws.cand.n);
+        }
+        const size_t need_unsure = c->test_unsure_cap ? (size_t) c->test_unsure_cap
+                                                      : std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
+        if (ws.unsure.n < need_unsure) {
+            budget_check(c, ws, budget, (need_unsure - ws.unsure.n) * sizeof(uint4));
+            ws.unsure.reserve(need_unsure);
+        }
+        budget_check(c, ws, budget, 0);
+        ws.T64.reserve(npl * 12);
+        ws.Ti.reserve(npl * 3);
+        ws.wbox.reserve(npl * 2);
+        ws.scalars.reserve(kScalars);
+        ws.stats.reserve(1);
+        const size_t need_spill = c->test_spill_cap ? std::max<size_t>(c->test_spill_cap, 2) : (size_t) 2 << 20;
+        const size_t need_nspill = c->test_nspill_cap ? std::max<size_t>(c->test_nspill_cap, 2) : (size_t) 1 << 20;
+        if (ws.spill.n < need_spill) ws.spill.reserve(need_spill);
+        if (ws.nspill.n < need_nspill) ws.nspill.reserve(need_nspill);
+        ws.seg_count.reserve((size_t) kCandSegments * kSegStride);
+        ws.seg_pre.reserve(kCandSegments + 1);
+        ws.seg_base.reserve(kCandSegments + 1);
+        ws.pin_status.reserve(kPublish);
     }
     ws.cand_cap = (uint32_t) std::min<size_t>(ws.cand.n, 0xffffffffu);
-    const size_t need_unsure = c->test_unsure_cap ? (size_t) c->test_unsure_cap
-                                                  : std::max<size_t>((size_t) 1 << 16, (size_t) n * 16);
-    if (ws.unsure.n < need_unsure) {
-        budget_check(c, ws, budget, (need_unsure - ws.unsure.n) * sizeof(uint4));
-        ws.unsure.reserve(need_unsure);
-    }
-    const size_t npl = (size_t) n * NL;
-    budget_check(c, ws, budget, 0);
-    ws.T64.reserve(npl * 12);
-    ws.Ti.reserve(npl * 3);
-    ws.wbox.reserve(npl * 2);
-    (void) items;
-    ws.scalars.reserve(kScalars);
-    ws.stats.reserve(1);
-    const size_t need_spill = c->test_spill_cap ? std::max<size_t>(c->test_spill_cap, 2) : (size_t) 2 << 20;
-    const size_t need_nspill = c->test_nspill_cap ? std::max<size_t>(c->test_nspill_cap, 2) : (size_t) 1 << 20;
-    if (ws.spill.n < need_spill) ws.spill.reserve(need_spill);
-    if (ws.nspill.n < need_nspill) ws.nspill.reserve(need_nspill);
-
-    ws.seg_count.reserve((size_t) kCandSegments * kSegStride);
-    ws.seg_pre.reserve(kCandSegments + 1);
-    ws.seg_base.reserve(kCandSegments + 1);
-    ws.pin_status.reserve(kPublish);
     if (ws.counting) fill_u32(reinterpret_cast<uint32_t*>(ws.stats.p), sizeof(Stats) / 4, 0u, s);
     // a segment of stage k starts here (k < 0: the check ends)
     int n_seg = 0;
@@ -444,6 +457,8 @@ rtcd_status rtcd_create(const rtcd_config* cfg, rtcd_ctx** out) {
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(RTCD_ERR_NO_DEVICE, "no HIP device");
         if (c.device < 0 || c.device >= ndev) return fail(RTCD_ERR_NO_DEVICE, "device index out of range");
         RTCD_HIP(hipSetDevice(c.device));
+        // the device's copy of the allocation-free entry point's scratch array (once per device)
+        scratch_init(c.device);
         auto ctx = std::make_unique<rtcd_ctx>();
         ctx->cfg = c;
         if (c.leaf_size) {

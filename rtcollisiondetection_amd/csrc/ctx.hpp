@@ -90,6 +90,16 @@ constexpr uint32_t kPinnedBatch = 1024;
 struct Workspace {
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    // buffers bound to fixed memory (the allocation-free entry point): a check never grows them, and a
+    // candidate or FP64-queue overflow is left to the caller (status word 1)
+    bool fixed = false;
+    // start every broad walk at the scene BVH's all-leaves frontier level (when it has one): pass 0 then
+    // only tests leaf boxes and a block emits at most one record per item it owns
+    bool deepest_frontier = false;
+    // small batches pick the deepest frontier level with tasks * entries <= frontier_work / frontier_div
+    // (a sub-batch of 1/k of a batch with divisor k keeps the batch's level)
+    uint64_t frontier_div = 1;
+    int last_frontier = 0;  // frontier entries per task of the last enqueued discrete check (0: none)
     // discrete
     DevBuf<double> T64;
     DevBuf<float4> Ti, wbox;
@@ -330,5 +340,8 @@ size_t discrete_bytes(const rtcd_ctx* c, uint32_t n);
 size_t sweep_bytes(const rtcd_ctx* c, uint32_t n_traj, uint32_t n_wp_total, uint32_t n_ctrl);
 // Stops the serving loop (drains in-flight requests) and frees its slots.
 void serve_shutdown(rtcd_ctx* c);
+// Takes the address of the allocation-free entry point's scratch array on `device` (which the caller
+// made current), once per device; throws HipError if the runtime has none.
+void scratch_init(int device);
 
 }  // namespace rtcd

@@ -11,21 +11,30 @@ template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    bool view = false;  // bound to memory it does not own (bind): never allocates or frees
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), view(o.view) { o.p = nullptr; o.n = 0; o.view = false; }
     DevBuf& operator=(DevBuf&& o) noexcept {
         if (this != &o) {
             release();
-            p = o.p; n = o.n;
-            o.p = nullptr; o.n = 0;
+            p = o.p; n = o.n; view = o.view;
+            o.p = nullptr; o.n = 0; o.view = false;
         }
         return *this;
     }
     ~DevBuf() { release(); }
+    // Makes this buffer a view of count elements at ptr (memory owned elsewhere).
+    void bind(T* ptr, size_t count) {
+        release();
+        p = ptr;
+        n = count;
+        view = true;
+    }
     void alloc(size_t count) {
+        if (view) throw std::runtime_error("a buffer bound to fixed memory cannot grow");
         release();
         if (count) RTCD_HIP(hipMalloc(&p, count * sizeof(T)));
         n = count;
@@ -35,9 +44,10 @@ struct DevBuf {
         if (count > n) alloc(count + count / 4);
     }
     void release() {
-        if (p) (void) hipFree(p);
+        if (p && !view) (void) hipFree(p);
         p = nullptr;
         n = 0;
+        view = false;
     }
     void upload(const T* h, size_t count, hipStream_t s) {
         reserve(count);
@@ -66,8 +76,16 @@ struct HostMapped {
     HostMapped(const HostMapped&) = delete;
     HostMapped& operator=(const HostMapped&) = delete;
     ~HostMapped() { release(); }
+    // Device memory standing in for the pinned buffer (h stays null): kernels write there and the
+    // host copies what it needs.  Such a buffer never allocates.
+    void bind_device(T* ptr, size_t count) {
+        release();
+        d = ptr;
+        n = count;
+    }
     void reserve(size_t count) {
         if (count <= n) return;
+        if (d && !h) throw std::runtime_error("a buffer bound to fixed memory cannot grow");
         release();
         const size_t c = count + count / 4 + 16;
         RTCD_HIP(hipHostMalloc((void**) &h, c * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent));

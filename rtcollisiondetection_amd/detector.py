@@ -129,6 +129,13 @@ class Context:
         check(lib().rtcd_check_discrete_device(self.h, sid, direction, C.c_void_p(d_poses.data_ptr()), n,
                                                C.c_void_p(d_out.data_ptr()), self._stream(stream)))
 
+    def check_discrete_device_noalloc(self, sid, d_poses, d_out, direction=_lib.TWO_WAY, stream=None):
+        """Allocation-free variant (rtcd_check_discrete_device_noalloc, at most NOALLOC_MAX_POSES poses):
+        work buffers in the library's fixed device scratch; returns when d_out holds the flags."""
+        n = d_poses.shape[0]
+        check(lib().rtcd_check_discrete_device_noalloc(self.h, sid, direction, C.c_void_p(d_poses.data_ptr()), n,
+                                                       C.c_void_p(d_out.data_ptr()), self._stream(stream)))
+
     def sync(self, stream=None):
         check(lib().rtcd_sync(self.h, self._stream(stream)))
 
