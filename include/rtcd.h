@@ -335,6 +335,21 @@ typedef struct {
 } rtcd_tune_report;
 rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* sample, uint32_t flags, rtcd_tune_report* report);
 
+/* ---- Test entry points: two building blocks of the discrete check alone, on device buffers,
+ * blocking, for the checks the reference runs on its own counterparts. ---- */
+/* Exclusive prefix sum of n uint32 (d_out[i] = d_in[0] + ... + d_in[i-1]) and the total in *d_total.
+ * Replaces RTCD's exclusiveScan (RTCD/Utils/cuda/cudaAlgorithms.cu:206-220), checked the way
+ * RTCD/Utils/Test/src/testExclusiveScan.cpp:27-83 checks it. */
+rtcd_status rtcd_test_exclusive_scan(const uint32_t* d_in, uint32_t* d_out, uint32_t n, uint32_t* d_total);
+/* Candidate compaction: records d_records [n][2] (pose * n_links + link, payload) whose pose has a
+ * zero flag in d_flags (keep_flagged == 0), or a non-zero one (keep_flagged != 0), copied to d_out,
+ * their number in *d_count.  Records keep their order within each work-group's contiguous chunk of the
+ * input; the chunks land in the order their work-groups reserve space (the check reads them as a set).  Replaces RTCD's streamCompact<Flag> (RTCD/Utils/cuda/
+ * streamCompact.inl:24-30 with the scan above), checked as RTCD/Utils/Test/src/testStreamCompact.cpp:27-104
+ * does. */
+rtcd_status rtcd_test_compact(const uint32_t* d_records, uint32_t n, const uint32_t* d_flags, uint32_t n_links,
+                              int32_t keep_flagged, uint32_t* d_out, uint32_t* d_count);
+
 /* Counting mode: when enabled, check calls also count node fetches / primitive tests (slower). */
 rtcd_status rtcd_set_counting(rtcd_ctx* ctx, int32_t enable);
 rtcd_status rtcd_get_stats(rtcd_ctx* ctx, rtcd_stats* stats);

@@ -22,7 +22,8 @@ EXPORTS = ['rtcd_create', 'rtcd_destroy', 'rtcd_last_error', 'rtcd_robot_load', 
            'rtcd_check_cubic', 'rtcd_check_cubic_device', 'rtcd_quadratic_weights', 'rtcd_curve_weights', 'rtcd_set_counting', 'rtcd_get_stats', 'rtcd_serve_start', 'rtcd_serve_stop',
            'rtcd_submit', 'rtcd_poll', 'rtcd_wait', 'rtcd_wait_timed', 'rtcd_serve_get_stats', 'rtcd_set_param', 'rtcd_get_param',
            'rtcd_set_leaf_size', 'rtcd_set_leaf_sizes', 'rtcd_autotune', 'rtcd_obj_read', 'rtcd_mesh_edges',
-           'rtcd_urdf_read', 'rtcd_spheres_read', 'rtcd_robot_load_urdf', 'rtcd_check_discrete_device_noalloc']
+           'rtcd_urdf_read', 'rtcd_spheres_read', 'rtcd_robot_load_urdf', 'rtcd_check_discrete_device_noalloc',
+           'rtcd_test_exclusive_scan', 'rtcd_test_compact']
 TUNE_LEAF_SIZE = 1
 PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad_budget': 4,
           'broad_budget_small': 5, 'broad_budget_next': 6, 'broad_passes': 7, 'sweep_passes': 8, 'graphs': 9,
@@ -123,6 +124,8 @@ def lib():
         L.rtcd_check_discrete_device.argtypes = [vp, u32, i32, vp, u32, vp, vp]
         L.rtcd_sync.argtypes = [vp, vp]
         L.rtcd_check_discrete_device_noalloc.argtypes = [vp, u32, i32, vp, u32, vp, vp]
+        L.rtcd_test_exclusive_scan.argtypes = [vp, vp, u32, vp]
+        L.rtcd_test_compact.argtypes = [vp, u32, vp, u32, i32, vp, vp]
         L.rtcd_check_linear.argtypes = [vp, u32, vp, vp, u32, vp, vp]
         L.rtcd_check_linear_device.argtypes = [vp, u32, vp, vp, u32, u32, vp, vp, vp]
         L.rtcd_check_quadratic.argtypes = [vp, u32, vp, u32, u32, u32, vp, vp, vp]

@@ -459,10 +459,14 @@ __global__ __launch_bounds__(kBS) void k_seg_gather(const uint2* __restrict__ ca
     for (uint32_t j = threadIdx.x; j < n; j += kBS) flat[(size_t) base + j] = src[j];
 }
 
-// After the obstacle->robot pass: the records of poses it left unflagged, in order, into a.r2o_list
-// (block b compacts the b-th contiguous chunk of the gathered list: one sweep counts, one global
-// atomic per block reserves, a second sweep writes with block-wide ranks).
-__global__ __launch_bounds__(kBS) void k_compact_unflagged(DiscreteArgs a) {
+// After the obstacle->robot pass: the records of poses it left unflagged into a.r2o_list (block b
+// compacts the b-th contiguous chunk of the gathered list: one sweep counts, one global atomic per
+// block reserves, a second sweep writes with block-wide ranks).  Each block's records keep their order;
+// the blocks' ranges follow the order of their reservations, which the robot->obstacle pass does not
+// depend on (it reads the list as a set).  KEEP_FLAGGED keeps the records of flagged poses instead
+// (the other polarity, used by the test entry point only).
+template <bool KEEP_FLAGGED>
+__global__ __launch_bounds__(kBS) void k_compact(DiscreteArgs a) {
     __shared__ uint32_t wsum[kBS / 64];
     __shared__ uint32_t base;
     const uint32_t total = a.seg_pre[kSeg];
@@ -470,7 +474,7 @@ __global__ __launch_bounds__(kBS) void k_compact_unflagged(DiscreteArgs a) {
     const uint32_t beg = min(total, blockIdx.x * chunk), end = min(total, beg + chunk);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     auto keep = [&](uint32_t i) {
-        return i < end && !load_flag(a.flags + a.flat[i].x / (uint32_t) a.n_links);
+        return i < end && (load_flag(a.flags + a.flat[i].x / (uint32_t) a.n_links) != 0u) == KEEP_FLAGGED;
     };
     uint32_t mine = 0;
     for (uint32_t i = beg + threadIdx.x; i < end; i += kBS) mine += keep(i) ? 1u : 0u;
@@ -1161,8 +1165,9 @@ void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStr
     else narrow_dir<2>(a, stack, count, s);
 }
 
-void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_compact_unflagged, dim3(kSeg), dim3(kBS), 0, s, a);
+void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s, bool keep_flagged) {
+    if (keep_flagged) hipLaunchKernelGGL(k_compact<true>, dim3(kSeg), dim3(kBS), 0, s, a);
+    else hipLaunchKernelGGL(k_compact<false>, dim3(kSeg), dim3(kBS), 0, s, a);
     RTCD_LAUNCH_CHECK();
 }
 

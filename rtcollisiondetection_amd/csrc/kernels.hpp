@@ -106,8 +106,10 @@ void launch_gather(const DiscreteArgs& a, bool small, hipStream_t s, int part = 
 // Narrow phase, one grid-stride pass in direction `dir` (1: obstacle->robot, 2: robot->obstacle)
 // over the gathered candidates.  `stack` covers the robot BVH depth.
 void launch_narrow(const DiscreteArgs& a, int dir, int stack, bool count, hipStream_t s);
-// Compacts the gathered records of still-unflagged poses into a.r2o_list (after obstacle->robot).
-void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s);
+// Compacts the gathered records (a.flat[0 .. a.seg_pre[kCandSegments])) of still-unflagged poses into
+// a.r2o_list, counting them in *a.r2o_count (after obstacle->robot); in order within each block's chunk,
+// the chunks in the order their blocks reserve space.  keep_flagged keeps the records of flagged poses.
+void launch_compact_unflagged(const DiscreteArgs& a, hipStream_t s, bool keep_flagged = false);
 // FP64 re-decision of the queued tests, then a copy of the first kPublish status words (and of the
 // flags when flags_out is set) into pinned host memory (device addresses of HostMapped buffers).
 // one_block: the whole queue in one work-group that also publishes (small batches: one launch).
