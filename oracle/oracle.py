@@ -59,6 +59,8 @@ def lib():
                                       C.c_void_p, C.c_int]
         L.orc_cubic_piece.argtypes = [_f64p, _f64p]
         L.orc_quad_piece.argtypes = [_f64p, _f64p]
+        L.orc_seg_tri_batch.argtypes = [C.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, _i32p]
+        L.orc_tube_decide_batch.argtypes = [C.c_int, C.c_int, _f64p, _f64p, _f64p, _f64p, _i32p]
         L.orc_check_linear_brute.argtypes = [C.c_void_p, C.c_void_p, _f32p, _u32p, C.c_int, _u32p, C.c_void_p]
         L.orc_check_quadratic_brute.argtypes = [C.c_void_p, C.c_void_p, _f32p, C.c_int, C.c_int, C.c_int, _f64p,
                                                 _u32p, C.c_void_p]
@@ -113,6 +115,27 @@ def tube_decide(ctrl, r, A, B):
     ctrl = np.ascontiguousarray(np.asarray(ctrl, dtype=np.float64).reshape(-1))
     deg = len(ctrl) // 3 - 1
     return lib().orc_tube_decide(deg, ctrl, float(r), _d3(A), _d3(B))
+
+
+def _d(x, shape):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(shape))
+
+
+def seg_tri_batch(P, Q, A, B, Cc):
+    """orc_seg_tri of n cases given as [n, 3] arrays."""
+    n = len(P)
+    out = np.zeros(n, dtype=np.int32)
+    lib().orc_seg_tri_batch(n, _d(P, (n, 3)), _d(Q, (n, 3)), _d(A, (n, 3)), _d(B, (n, 3)), _d(Cc, (n, 3)), out)
+    return out
+
+
+def tube_decide_batch(ctrl, r, A, B):
+    """orc_tube_decide of n cases: ctrl [n, deg + 1, 3], r [n], A, B [n, 3]."""
+    ctrl = np.asarray(ctrl, dtype=np.float64)
+    n, k = ctrl.shape[0], ctrl.shape[1]
+    out = np.zeros(n, dtype=np.int32)
+    lib().orc_tube_decide_batch(n, k - 1, _d(ctrl, (n, k, 3)), _d(r, (n,)), _d(A, (n, 3)), _d(B, (n, 3)), out)
+    return out
 
 
 def cubic_piece(P):

@@ -901,3 +901,18 @@ void orc_check_cubic_brute(const orc_robot* rb, const orc_mesh* sc, const float*
     int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg) {
     curve_brute(rb, sc, 3, wp, n_traj, n_wp, n_ctrl, W, out, out_seg);
 }
+
+/* ------------------------------------------------------------------------------------------------
+ * Batches of the two decisions, for property tests that judge many cases (plain loops).
+ * ---------------------------------------------------------------------------------------------- */
+void orc_seg_tri_batch(int n, const double* P, const double* Q, const double* A, const double* B, const double* C,
+    int32_t* out) {
+    for (int i = 0; i < n; ++i)
+        out[i] = orc_seg_tri(P + 3 * i, Q + 3 * i, A + 3 * i, B + 3 * i, C + 3 * i);
+}
+
+void orc_tube_decide_batch(int n, int deg, const double* ctrl, const double* r, const double* A, const double* B,
+    int32_t* out) {
+    for (int i = 0; i < n; ++i)
+        out[i] = orc_tube_decide(deg, ctrl + (size_t) 3 * (deg + 1) * i, r[i], A + 3 * i, B + 3 * i);
+}

@@ -110,6 +110,13 @@ void orc_check_quadratic_brute(const orc_robot* rb, const orc_mesh* scene, const
 void orc_check_cubic_brute(const orc_robot* rb, const orc_mesh* scene, const float* wp, int n_traj,
     int n_wp, int n_ctrl, const double* W, uint32_t* out, uint32_t* out_seg);
 
+/* n independent decisions: out[i] = orc_seg_tri of the i-th points ([n][3] each), and
+ * out[i] = orc_tube_decide(deg, ctrl + i (deg+1) 3, r[i], A + 3 i, B + 3 i). */
+void orc_seg_tri_batch(int n, const double* P, const double* Q, const double* A, const double* B, const double* C,
+    int32_t* out);
+void orc_tube_decide_batch(int n, int deg, const double* ctrl, const double* r, const double* A, const double* B,
+    int32_t* out);
+
 /* Control points of one sphere path: X[n_wp][3] -> P[n_ctrl][3] (sweptVolume.cu:22-83). */
 void orc_fit_ctrl(const double* W, int n_ctrl, int n_wp, const double* X, double* P);
 
