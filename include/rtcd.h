@@ -115,7 +115,9 @@ rtcd_status rtcd_check_discrete(rtcd_ctx* ctx, uint32_t scene_id, int32_t direct
 /* Same with device pointers, enqueued on `stream` (hipStream_t); returns before the work completes.
  * NULL selects the context's own stream, which is ordered with the device's null stream (work the
  * caller put on the default stream runs first).  rtcd_sync(ctx, stream) waits and reports deferred
- * errors. */
+ * errors.  A context holds one set of work buffers: call rtcd_sync before enqueuing the next check on
+ * another stream, or give each concurrent stream a context of its own (RTCD's NSTREAM streams each
+ * have their own launch data, CollisionDetector.h:112-131). */
 rtcd_status rtcd_check_discrete_device(rtcd_ctx* ctx, uint32_t scene_id, int32_t direction, const float* d_poses,
                                        uint32_t n, uint32_t* d_out, void* stream);
 rtcd_status rtcd_sync(rtcd_ctx* ctx, void* stream);

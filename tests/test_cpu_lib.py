@@ -3,6 +3,7 @@ declares, host-only entry points work, and error paths report status codes witho
 import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -130,3 +131,24 @@ def test_cpp_facade_header_compiles():
     r = subprocess.run(['g++', '-std=c++17', '-fsyntax-only', '-Wall', '-Werror', '-I' + os.path.join(root, 'include'),
                         os.path.join(root, 'tests', 'cpp', 'facade_check.cpp')], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_reference_bridge_compiles_against_rtcd_standins():
+    """include/rtcd_bridge.hpp, the code a maintainer adds to RTCD, compiles against stand-ins that declare
+    only the members of RTCD's types it cites (tests/cpp/rtcd_standins.hpp): robotConfig<DOF>,
+    robot<DOF>::getOrderedLinks, meshModel::mesh / sphereOrigins / sphereRadii, obstacle, and the robot
+    wrapper's getPoseBufferPtr / getBatchSize / getDOF.  Its check program (which runs it on the GPU)
+    instantiates every template of the bridge."""
+    root = ROOT
+    r = subprocess.run(['g++', '-std=c++17', '-fsyntax-only', '-Wall', '-Werror', '-D__HIP_PLATFORM_AMD__',
+                        '-I/opt/rocm/include', '-I' + os.path.join(root, 'include'),
+                        '-I' + os.path.join(root, 'tests', 'cpp'), os.path.join(root, 'tests', 'cpp', 'bridge_check.cpp')],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    # the stand-ins declare nothing beyond what the bridge reads: a member the bridge does not use would
+    # make the compile above prove less
+    src = open(os.path.join(root, 'tests', 'cpp', 'rtcd_standins.hpp')).read()
+    for name in ('baseTransform', 'cosMask', 'sinMask', 'oneMask', 'getDof', 'getOrderedLinks', 'sphereOrigins',
+                 'sphereRadii', 'vertices', 'indices', 'getPoseBufferPtr', 'getBatchSize', 'getDOF'):
+        assert name in open(os.path.join(root, 'include', 'rtcd_bridge.hpp')).read(), name
+        assert name in src, name

@@ -1,5 +1,7 @@
-"""The C++ facade (include/rtcd_detector.hpp): a C++ program drives CollisionDetector<NSTREAM, TYPE> for
-every link type through the C ABI, and its flags equal the Python binding's and the oracle's."""
+"""The C++ facade (include/rtcd_detector.hpp): a C++ program drives BatchRobot, CCWrappedRobot /
+CCRayRobot and CollisionDetector<NSTREAM, TYPE> for every link type the way RTCD's benchmarks drive
+RTCD's detector (prepareUpdateData with a pose count, warmup, setNCheck, check, detect per batch), and
+its flags equal the Python binding's and the oracle's."""
 import os
 import subprocess
 
@@ -36,10 +38,11 @@ def test_cpp_facade_all_link_types(tmp_path, panda, orc_panda):
            sc.tris.astype(np.int32))
     q = samplers.uniform_poses(300, rb.lower, rb.upper, seed=17)
     _write(tmp_path / 'poses.bin', np.array([len(q), rb.dof], u), q)
-    traj, offs = samplers.random_walk_trajectories(40, rb.lower, rb.upper, lengths=(4, 20), seed=18)
-    _write(tmp_path / 'lin.bin', np.array([40, len(traj)], u), offs.astype(u), traj)
-    wp, _ = samplers.bspline_trajectories(48, rb.lower, rb.upper, n_ctrl=8, n_wp=16, seed=19)
-    _write(tmp_path / 'curves.bin', np.array([48, 16, 8], u), wp)
+    # RTCD's linear curve robots take trajectories of one length (nTrajPts = 16, BM_LINEAR<16, 16>)
+    traj, offs = samplers.random_walk_trajectories(300, rb.lower, rb.upper, lengths=16, seed=18)
+    _write(tmp_path / 'lin.bin', np.array([300, 16], u), traj)
+    wp, _ = samplers.bspline_trajectories(200, rb.lower, rb.upper, n_ctrl=8, n_wp=16, seed=19)
+    _write(tmp_path / 'curves.bin', np.array([200, 16, 8], u), wp)
     print('running', BIN, flush=True)
     r = subprocess.run([BIN, str(tmp_path)], capture_output=True, text=True, timeout=180)
     assert r.returncode == 0, r.stderr
@@ -49,6 +52,7 @@ def test_cpp_facade_all_link_types(tmp_path, panda, orc_panda):
         n = int(raw[k])
         res.append(raw[k + 1:k + 1 + n])
         k += 1 + n
+    from rtcollisiondetection_amd.detector import curve_weights
     ctx = Context()
     try:
         ctx.load_robot(rb)
@@ -58,9 +62,13 @@ def test_cpp_facade_all_link_types(tmp_path, panda, orc_panda):
             ref = O.check_discrete(orc_panda, m, q, d)
             assert np.array_equal(res[i], ref), d
             assert np.array_equal(res[i], ctx.check_discrete(sid, q, d)), d
+        assert np.array_equal(res[3], O.check_linear(orc_panda, m, traj, offs))
         assert np.array_equal(res[3], ctx.check_linear(sid, traj, offs))
-        assert np.array_equal(res[4], ctx.check_quadratic(sid, wp, 8))
-        assert np.array_equal(res[5], ctx.check_cubic(sid, wp, 8))
-        assert 0 < res[2].sum() < len(q) and 0 < res[5].sum()
+        W2, W3 = curve_weights(2, 'normal', 8, 16), curve_weights(3, 'normal', 8, 16)
+        assert np.array_equal(res[4], O.check_quadratic(orc_panda, m, wp, W2))
+        assert np.array_equal(res[5], O.check_cubic(orc_panda, m, wp, W3))
+        assert np.array_equal(res[5], ctx.check_cubic(sid, wp, 8, W3))
+        for k in range(6):
+            assert 0 < res[k].sum() < len(res[k]), k
     finally:
         ctx.close()
