@@ -307,7 +307,9 @@ typedef enum {
     RTCD_PARAM_TEST_CAND_CAP = 27,     /* discrete leaf-candidate records */
     RTCD_PARAM_TEST_UNSURE_CAP = 28,   /* FP64 re-decision queue entries (discrete and swept) */
     RTCD_PARAM_TEST_SPILL_CAP = 29,    /* broad-phase and sweep hand-off records (both halves) */
-    RTCD_PARAM_TEST_NSPILL_CAP = 30    /* narrow-phase hand-off records (both halves) */
+    RTCD_PARAM_TEST_NSPILL_CAP = 30,   /* narrow-phase hand-off records (both halves) */
+    RTCD_PARAM_NARROW_SHARE = 31       /* discrete narrow passes: 1 = once a wave's candidates are used up, its
+                                          idle lanes take pending subtrees of its busy lanes' walks */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 /* Rebuilds the robot's and every scene's BVHs with other leaf sizes (1..16; 0 keeps one): the robot

@@ -208,6 +208,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.narrow_budget_next = c->o2r_budget_next;
     a.narrow_passes = (int) c->o2r_passes;
     a.narrow_bs = (int) c->narrow_bs;
+    a.narrow_share = (int) c->narrow_share;
     const int broad_stack = stack_for(sc.tri_bvh.depth);
     // Split check: after each broad pass k with bit k of c->o2r_split set (below the last pass), an
     // obstacle->robot pass runs over the records gathered since the previous one.  The hand-off passes
@@ -297,7 +298,7 @@ static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, c
             P(ws.T64.p), P(ws.Ti.p), P(ws.wbox.p), P(ws.cand.p), ws.cand.n, P(ws.flat.p), ws.flat.n,
             P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.seg_base.p), c->o2r_split, P(ws.spill.p), ws.spill.n, P(ws.nspill.p), ws.nspill.n,
             P(ws.unsure.p), ws.unsure.n, c->leaf_scene_tri, c->narrow_budget, c->narrow_budget_next, c->narrow_passes, c->o2r_budget,
-            c->o2r_budget_next, c->o2r_passes,
+            c->o2r_budget_next, c->o2r_passes, c->narrow_share,
             P(ws.scalars.p), P(ws.stats.p), P(ws.pin_status.d), P(ws.pub_flags), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
             c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth, c->narrow_bs,
             c->tiny_batch};
@@ -982,6 +983,10 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
             return fail(RTCD_ERR_INVALID_ARG, "split must be a mask of at most 3 of the bits 0..15");
         ctx->o2r_split = (uint32_t) value;
         return RTCD_OK;
+    case RTCD_PARAM_NARROW_SHARE:
+        if (value > 1) return fail(RTCD_ERR_INVALID_ARG, "narrow share must be 0 or 1");
+        ctx->narrow_share = (uint32_t) value;
+        return RTCD_OK;
     case RTCD_PARAM_TEST_CAND_CAP: ctx->test_cand_cap = value; return RTCD_OK;
     case RTCD_PARAM_TEST_UNSURE_CAP: ctx->test_unsure_cap = value; return RTCD_OK;
     case RTCD_PARAM_TEST_SPILL_CAP: ctx->test_spill_cap = value; return RTCD_OK;
@@ -1033,6 +1038,7 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_SERVE_LANE_KINDS: *value = ctx->serve_lane_kinds; return RTCD_OK;
     case RTCD_PARAM_TINY_BATCH: *value = ctx->tiny_batch; return RTCD_OK;
     case RTCD_PARAM_O2R_SPLIT: *value = ctx->o2r_split; return RTCD_OK;
+    case RTCD_PARAM_NARROW_SHARE: *value = ctx->narrow_share; return RTCD_OK;
     case RTCD_PARAM_TEST_CAND_CAP: *value = ctx->test_cand_cap; return RTCD_OK;
     case RTCD_PARAM_TEST_UNSURE_CAP: *value = ctx->test_unsure_cap; return RTCD_OK;
     case RTCD_PARAM_TEST_SPILL_CAP: *value = ctx->test_spill_cap; return RTCD_OK;

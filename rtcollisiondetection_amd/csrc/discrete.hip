@@ -830,9 +830,16 @@ __device__ __forceinline__ int resume_candidate(const DiscreteArgs& a, uint32_t 
 // together would keep every lane waiting for the longest walk: walks are short on average (a few
 // link-BVH nodes) but vary widely.  Obstacle->robot traces the owned edges of the triangle one after
 // the other; robot->obstacle walks the triangle once.
+//
+// Once the wave's candidates are used up, idle lanes share the walks still running in the wave
+// (a.narrow_share): every busy lane with a pending subtree on its stack gives the top one to an idle
+// lane through the wave's LDS slots `sh`, as the record a hand-off would write (pose-link, triangle,
+// node, edge), and the idle lane walks that subtree with its own stack.  Each subtree is still walked
+// once, so the flags and FP64 tests are those of a single walk; a long walk (a large scene triangle
+// whose plane cuts a link) then spreads over the wave instead of holding one lane while 63 wait.
 template <int DIR, int STACK, int BS, bool COUNT, bool SPILL>
 __device__ __forceinline__ void narrow_lanes(const DiscreteArgs& a, const uint2* list, uint32_t total, uint2* q,
-                                             int* st, Ctr& ctr, const NarrowSpill& hs) {
+                                             uint4* sh, int* st, Ctr& ctr, const NarrowSpill& hs) {
     constexpr int kRefill = RTCD_NARROW_REFILL;  // idle lanes that make the wave refill
     constexpr int kSteps = RTCD_NARROW_STEPS;    // node visits between refill checks
     const int lane = (int) (threadIdx.x & 63);
@@ -901,6 +908,29 @@ __device__ __forceinline__ void narrow_lanes(const DiscreteArgs& a, const uint2*
             return true;
         }
     };
+    // a subtree shared by another lane of the wave: the same walk set-up (a pure function of the
+    // candidate and the edge), an empty stack, no further owned edges
+    auto start_shared = [&](uint4 e) -> bool {
+        key = e.x;
+        tri = e.y;
+        flag = a.flags + key / NL;
+        const LinkInfo& L = a.robot->link[key % NL];
+        const float4 r0 = a.Ti[(size_t) key * 3], r1 = a.Ti[(size_t) key * 3 + 1], r2 = a.Ti[(size_t) key * 3 + 2];
+        const Tri t = a.sc.tri_prims[tri];
+        const LinkTri m = link_tri(t, r0, r1, r2);
+        bool ok;
+        if constexpr (DIR == 1) {
+            own = 0u;
+            const float3 P = e.w == 0 ? m.A : e.w == 1 ? add(m.A, m.e1) : add(m.A, m.e2);
+            const float3 d = e.w == 0 ? m.e1 : e.w == 1 ? rot(r0, r1, r2, sub(f3(t.c), f3(t.b))) : neg(m.e2);
+            ok = o2r_begin(L, P, d, m.delta, e.w, w);
+        } else {
+            ok = r2o_begin(L, m.A, m.e1, m.e2, m.delta, w);
+        }
+        begin_walk();
+        ni = (int) e.z;
+        return ok;
+    };
     // the lane's walk has ended without a hit: obstacle->robot goes on with the next owned edge
     auto walk_ended = [&]() -> bool {
         if constexpr (DIR == 1) return next_edge(a.robot->link[key % NL]);
@@ -937,6 +967,27 @@ __device__ __forceinline__ void narrow_lanes(const DiscreteArgs& a, const uint2*
             qn -= take;
             __builtin_amdgcn_wave_barrier();
             if (mine) act = start(c);
+            if (a.narrow_share && qn == 0 && base >= total) {
+                // no candidates left: the idle lanes take the top pending subtree of busy lanes
+                const uint64_t idle2 = __ballot(!act), donors = __ballot(act && sp > 0);
+                const int n_give = min(__popcll(idle2), __popcll(donors));
+                if (n_give > 0) {
+                    const int dr = __popcll(donors & below);
+                    if (act && sp > 0 && dr < n_give) {
+                        --sp;
+                        uint32_t j = 3u;
+                        if constexpr (DIR == 1) j = w.e;
+                        sh[dr] = make_uint4(key, tri, (uint32_t) st[sp * BS], j);
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    const int ir = __popcll(idle2 & below);
+                    uint4 e = make_uint4(0u, 0u, 0u, 0u);
+                    const bool taker = !act && ir < n_give;
+                    if (taker) e = sh[ir];
+                    __builtin_amdgcn_wave_barrier();
+                    if (taker) act = start_shared(e);
+                }
+            }
             if (qn == 0 && base >= total && __ballot(act) == 0) break;
         }
         for (int s = 0; s < kSteps && act; ++s) {
@@ -985,13 +1036,16 @@ __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
     Ctr ctr;
     if (pass == 0) {
         __shared__ uint2 lq[BS / 64][128];
+        __shared__ uint4 sh[BS / 64][64];
         // the gathered list: the records of the latest gather (a split check gathers in parts)
         const uint32_t first = DIR == 1 || !a.r2o_list ? a.seg_pre[0] : 0u;
         const uint2* list = DIR == 2 && a.r2o_list ? a.r2o_list : a.flat + first;
         const uint32_t total = DIR == 2 && a.r2o_list ? *a.r2o_count : a.seg_pre[kSeg] - first;
         // a pass that may hand off and one that never does (the walks of the latter keep fewer registers)
-        if (hs.may) narrow_lanes<DIR, STACK, BS, COUNT, true>(a, list, total, lq[threadIdx.x >> 6], st, ctr, hs);
-        else narrow_lanes<DIR, STACK, BS, COUNT, false>(a, list, total, lq[threadIdx.x >> 6], st, ctr, hs);
+        if (hs.may)
+            narrow_lanes<DIR, STACK, BS, COUNT, true>(a, list, total, lq[threadIdx.x >> 6], sh[threadIdx.x >> 6], st, ctr, hs);
+        else
+            narrow_lanes<DIR, STACK, BS, COUNT, false>(a, list, total, lq[threadIdx.x >> 6], sh[threadIdx.x >> 6], st, ctr, hs);
     } else {
         const uint4* in = ((pass - 1) & 1 ? a.nspill[1] : a.nspill[0]);
         const uint32_t total = min(a.nspill_count[pass - 1], a.nspill_cap);

@@ -79,6 +79,8 @@ struct DiscreteArgs {
     uint32_t* nspill_count;  // [narrow_passes - 1] per direction: hand-offs of each pass
     uint32_t narrow_budget, narrow_budget_next;  // link-BVH nodes a walk visits before handing off
     int narrow_passes;     // passes per direction (<= kMaxPasses); the last never hands off
+    int narrow_share;      // 1: once a wave's candidates are used up, its idle lanes take pending
+                           // subtrees of its busy lanes' walks
 };
 // Most hand-off passes of one broad phase or sweep: the first walks every task, each later one the
 // subtrees handed off by the previous pass.

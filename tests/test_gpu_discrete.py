@@ -249,3 +249,31 @@ def test_split_checks_match_oracle(ctx, panda, orc_panda):
         ctx.set_param('o2r_split', 0b1010)
         ctx.set_param('frontier_max_tasks', old)
     ctx.destroy_scene(sid)
+
+
+def test_narrow_share_matches_unshared_schedules(ctx, panda, orc_panda):
+    """Idle lanes sharing the wave's running walks (RTCD_PARAM_NARROW_SHARE, one robot->obstacle pass)
+    give the flags of the unshared schedule with six hand-off passes, in every direction, on batches
+    that use the frontier split (1,024 poses) and the split check (20,000 poses); the oracle agrees on
+    a strided sample."""
+    sc = scenes.scene_dense()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    m = O.scene_mesh(sc)
+    q = samplers.uniform_poses(20000, panda.lower, panda.upper, seed=71)
+    try:
+        assert ctx.get_param('narrow_share') == 1 and ctx.get_param('narrow_passes') == 1
+        for n in (1024, 20000):
+            for d in (1, 2, 3):
+                shared = ctx.check_discrete(sid, q[:n], d)
+                ctx.set_param('narrow_share', 0)
+                ctx.set_param('narrow_passes', 6)
+                try:
+                    unshared = ctx.check_discrete(sid, q[:n], d)
+                finally:
+                    ctx.set_param('narrow_share', 1)
+                    ctx.set_param('narrow_passes', 1)
+                assert np.array_equal(shared, unshared), (n, d)
+                idx = np.arange(0, n, max(1, n // 1000))
+                assert _mismatch(shared[idx], O.check_discrete(orc_panda, m, q[:n][idx], d)) == 0, (n, d)
+    finally:
+        ctx.destroy_scene(sid)
