@@ -6,11 +6,12 @@ Headline (one JSON line on stdout): dense two-way discrete check, poses/s on one
             edges), both directions (obstacle->robot and robot->obstacle), one flag per pose.
   A step is one blocking check of the whole batch with poses already resident in HBM.
 
-Also reported in the same line: per-direction rates, the plain per-query launch (one pose per call),
-linear (4,096 random walks of 8-64 waypoints), quadratic and cubic (4,096 B-splines, 8 control
-points) trajectories/s on the shelf and dense scenes, the serving stream, the roofline of the dominant kernel
-(the obstacle->robot narrow pass) and the CPU oracle baseline timed on a bounded sample of the same
-workload.
+Also reported in the same line: per-direction rates, a batch-size sweep with graph replay, the plain
+per-query launch (one pose per call), linear (4,096 random walks of 8-64 waypoints), quadratic and cubic
+(4,096 B-splines, 8 control points) trajectories/s on the shelf and dense scenes, RTCD's own benchmark
+inputs on its cuRobo scenes, the serving stream, the roofline of the broad phase and of both narrow
+kernels (the slowest one heads the roofline object), and the CPU baseline: the FP32 CPU port of the
+same method on the whole workload, each direction, on the host cores this process may use.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
 """
@@ -47,6 +48,31 @@ S_REC, S_EVAL, S_NODE, S_TRI, S_EDGE = 8, 96, 64, 48, 32
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def _cpu_cores():
+    """Cores this process may use: its affinity set, capped by the cgroup's CPU quota."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    try:
+        quota, period = open('/sys/fs/cgroup/cpu.max').read().split()[:2]
+        if quota != 'max':
+            n = min(n, max(1, int(int(quota) / int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def _cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
 
 
 def _boxes8():
@@ -114,7 +140,8 @@ REF_RTX4070 = {
               'linear16_us_per_traj': 11.4, 'quadratic8_us_per_traj': 27.0},
     'dense_shelf': {'o2r_us_per_pose': 0.42, 'r2o_us_per_pose': 0.48, 'two_way_us_per_pose': 0.90,
                     'single_pose_batch_us': {'o2r': 94, 'r2o': 51, 'two_way': 147},
-                    'linear16_us_per_traj': 21.0, 'quadratic8_us_per_traj': 54.5},
+                    'linear16_us_per_traj': 21.0, 'quadratic8_us_per_traj': 54.5,
+                    'discrete16_o2r_plus_r2o_us_per_traj': 13.0, 'discrete32_o2r_plus_r2o_us_per_traj': 27.0},
 }
 
 
@@ -129,6 +156,8 @@ def reference_scene_bench(ctx, rb, reps=5):
     pts = samplers.halton_reference(8192, rb.lower, rb.upper)
     tr_all = samplers.knn_trajectories(pts, 4, 16)
     tr = tr_all[np.random.default_rng(1).choice(len(tr_all), 4096, replace=False)]
+    tr32_all = samplers.knn_trajectories(pts, 4, 32)
+    tr32 = tr32_all[np.random.default_rng(2).choice(len(tr32_all), 4096, replace=False)]
     offs = np.arange(0, (len(tr) + 1) * 16, 16, dtype=np.uint32)
     W = samplers.quadratic_fit_weights(8, 16)
     from rtcollisiondetection_amd.detector import curve_weights
@@ -162,6 +191,16 @@ def reference_scene_bench(ctx, rb, reps=5):
             r[f'{key}_single_pose_us'] = round(float(np.median(ts)) * 1e6, 1)
             r[f'{key}_single_pose_p90_us'] = round(float(np.percentile(ts, 90)) * 1e6, 1)
         r['collisions_two_way'] = int(ctx.check_discrete(sid, q, 3).astype(bool).sum())
+        # RTCD's curve benchmark also checks the waypoints of its 4,096 KNN trajectories as discrete poses,
+        # 16 and 32 per trajectory (BM_MESH / BM_RAY, RTCD/Benchmark/Curve/benchmarkUtils.h:152-195), and
+        # plots obstacle->robot + robot->obstacle per trajectory
+        for npt, wp_set in ((16, tr), (32, tr32)):
+            flat = np.ascontiguousarray(wp_set.reshape(-1, rb.dof))
+            for key, d in (('o2r', 1), ('r2o', 2), ('two_way', 3)):
+                r[f'discrete{npt}_{key}_us_per_traj'] = round(
+                    med(lambda: ctx.check_discrete(sid, flat, d)) / len(wp_set) * 1e6, 3)
+            r[f'discrete{npt}_o2r_plus_r2o_us_per_traj'] = round(
+                r[f'discrete{npt}_o2r_us_per_traj'] + r[f'discrete{npt}_r2o_us_per_traj'], 3)
         r['linear16_us_per_traj'] = round(med(lambda: ctx.check_linear(sid, tr.reshape(-1, 7), offs)) / len(tr) * 1e6, 3)
         r['quadratic8_us_per_traj'] = round(med(lambda: ctx.check_quadratic(sid, tr, 8, W)) / len(tr) * 1e6, 3)
         r['cubic8_us_per_traj'] = round(med(lambda: ctx.check_cubic(sid, tr, 8, W3)) / len(tr) * 1e6, 3)
@@ -265,56 +304,79 @@ def main():
     log(f'[bench] two-way: {value:.4g} poses/s, {dt / args.steps * 1e3:.2f} ms/step, {hits} collisions, '
         f'stages (FK, broad, gather, O2R, R2O+FP64) ms {np.round(stage, 3).tolist()}')
 
-    # Roofline of the dominant kernel.  The library brackets every launch group with events on its
-    # stream and sums them per stage (stage 3: the k_narrow<1> launches, which a split check runs in
-    # parts between the broad passes; stage 4: k_narrow<2> + the short FP64 kernel).  Their work
-    # comes from untimed counting runs: the obstacle->robot direction alone runs exactly the
-    # obstacle->robot parts of the two-way check (same schedule, flags start at zero), and the
-    # robot->obstacle pass is the two-way count minus it.
+    # Roofline.  The library brackets every launch group with events on its stream and sums them per
+    # stage (1: the k_broad passes; 3: the k_narrow<1> launches, which a split check runs in parts between
+    # the broad passes; 4: k_narrow<2> + the short FP64 kernel).  Their work comes from counting launches
+    # of the same two-way check (the narrow kernels count per direction); a counting launch runs the
+    # counting variant of each kernel, whose timing, and with it the race of the pose flags, differs
+    # from the timed one, so the counts of three counting runs are reported with their spread.
     ctx.set_param('graphs', 1)
     ctx.set_counting(True)
-    run(1)
-    c1 = ctx.stats()
-    run(3)
-    c3 = ctx.stats()
+    counts = []
+    for _ in range(3):
+        t = time.perf_counter()
+        run(3)
+        cs = ctx.stats()
+        cs['wall_ms'] = (time.perf_counter() - t) * 1e3
+        counts.append(cs)
     ctx.set_counting(False)
     lanes = ctx.get_param('leaf_scene_tri')  # threads per leaf record = the scene-triangle leaf size
-    rec_lanes = lanes * c1['leaf_candidates']
-    k1 = {'kernel': 'k_narrow<1> (obstacle->robot narrow pass)', 'ms': float(stage[3]),
-          'bytes': (S_REC * rec_lanes + S_EVAL * c1['narrow_candidates'] + S_NODE * c1['node_fetches'] +
-                    S_TRI * c1['prim_tests']),
-          'node_fetches': c1['node_fetches'], 'primitive_tests': c1['prim_tests'],
-          'candidates_evaluated': c1['narrow_candidates'], 'traffic_file': 'traffic_narrow_o2r_dense.json'}
-    k2 = {'kernel': 'k_narrow<2> (robot->obstacle narrow pass)', 'ms': float(stage[4]),
-          'bytes': (S_REC * lanes * c3['r2o_records'] + S_EVAL * (c3['narrow_candidates'] - c1['narrow_candidates']) +
-                    S_NODE * (c3['node_fetches'] - c1['node_fetches']) + S_EDGE * c3['edge_tests']),
-          'node_fetches': c3['node_fetches'] - c1['node_fetches'], 'primitive_tests': c3['edge_tests'],
-          'candidates_evaluated': c3['narrow_candidates'] - c1['narrow_candidates'],
-          'traffic_file': 'traffic_narrow_r2o_dense.json'}
-    dom, other = (k1, k2) if k1['ms'] >= k2['ms'] else (k2, k1)
-    achieved = dom['bytes'] / (dom['ms'] * 1e-3) / 1e9
-    traffic = None
-    tp = os.path.join(ROOT, 'profiles', dom['traffic_file'])
-    if os.path.exists(tp):
-        try:
-            traffic = json.load(open(tp)).get('hbm_bytes_per_check')
-        except Exception:
-            traffic = None
-    roofline = {'bound': 'hbm', 'achieved': round(achieved, 2), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': traffic,
-                'kernel': dom['kernel'], 'kernel_ms': round(dom['ms'], 4), 'algorithmic_bytes': int(dom['bytes']),
-                'node_fetches': int(dom['node_fetches']), 'primitive_tests': int(dom['primitive_tests']),
-                'leaf_records': int(c1['leaf_candidates']), 'candidates_evaluated': int(dom['candidates_evaluated']),
-                'other_narrow': {'kernel': other['kernel'], 'kernel_ms': round(other['ms'], 4),
-                                 'achieved_gbs': round(other['bytes'] / (other['ms'] * 1e-3) / 1e9, 2)},
+
+    def kernel_bytes(c):
+        """Algorithmic bytes per check of the broad phase and the two narrow kernels."""
+        return {
+            # broad: scene BVH nodes read + leaf records written (a lower bound: the per-item task reads
+            # of 80 B, link rows and world box, are left out)
+            'broad': S_NODE * c['broad_node_fetches'] + S_REC * c['leaf_candidates'],
+            'o2r': (S_REC * lanes * c['leaf_candidates'] + S_EVAL * c['dir_candidates'][0] +
+                    S_NODE * c['dir_node_fetches'][0] + S_TRI * c['dir_prim_tests'][0]),
+            'r2o': (S_REC * lanes * c['r2o_records'] + S_EVAL * c['dir_candidates'][1] +
+                    S_NODE * c['dir_node_fetches'][1] + S_EDGE * c['dir_prim_tests'][1]),
+        }
+    kb = [kernel_bytes(c) for c in counts]
+    stage_of = {'broad': 1, 'o2r': 3, 'r2o': 4}
+    names = {'broad': 'k_broad (broad phase, all passes)', 'o2r': 'k_narrow<1> (obstacle->robot narrow pass)',
+             'r2o': 'k_narrow<2> (robot->obstacle narrow pass, + FP64 re-decision)'}
+    traffic_files = {'o2r': 'traffic_narrow_o2r_dense.json', 'r2o': 'traffic_narrow_r2o_dense.json',
+                     'broad': 'traffic_broad_dense.json'}
+    kernels = {}
+    for k in ('broad', 'o2r', 'r2o'):
+        ms = float(stage[stage_of[k]])
+        b = [x[k] for x in kb]
+        ach = float(np.median(b)) / (ms * 1e-3) / 1e9
+        traffic = None
+        tp = os.path.join(ROOT, 'profiles', traffic_files[k])
+        if os.path.exists(tp):
+            try:
+                traffic = json.load(open(tp)).get('hbm_bytes_per_check')
+            except Exception:
+                traffic = None
+        kernels[k] = {'kernel': names[k], 'ms': round(ms, 4), 'algorithmic_bytes': int(np.median(b)),
+                      'bytes_spread_over_3_counting_runs': [int(min(b)), int(max(b))],
+                      'achieved_gbs': round(ach, 2), 'frac': round(ach / HBM_PEAK_GBS, 4), 'traffic': traffic}
+    c0 = counts[0]
+    kernels['broad']['node_fetches'] = int(c0['broad_node_fetches'])
+    kernels['broad']['leaf_records'] = int(c0['leaf_candidates'])
+    for k, d in (('o2r', 0), ('r2o', 1)):
+        kernels[k]['node_fetches'] = int(c0['dir_node_fetches'][d])
+        kernels[k]['primitive_tests'] = int(c0['dir_prim_tests'][d])
+        kernels[k]['candidates_evaluated'] = int(c0['dir_candidates'][d])
+    dom_key = max(kernels, key=lambda k: kernels[k]['ms'])
+    dom = kernels[dom_key]
+    roofline = {'bound': 'hbm', 'achieved': dom['achieved_gbs'], 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': dom['frac'], 'traffic': dom['traffic'],
+                'kernel': dom['kernel'], 'kernel_ms': dom['ms'], 'algorithmic_bytes': dom['algorithmic_bytes'],
+                'kernels': kernels,
+                'counting_runs_wall_ms': [round(c['wall_ms'], 3) for c in counts],
                 'stage_ms': {'fk': round(float(stage[0]), 4), 'broad': round(float(stage[1]), 4),
                              'gather': round(float(stage[2]), 4), 'narrow_o2r': round(float(stage[3]), 4),
                              'narrow_r2o_fp64': round(float(stage[4]), 4)},
-                'two_way_counts': {'broad_node_fetches': int(c3['broad_node_fetches']),
-                                   'candidate_tris': int(c3['candidates']),
-                                   'narrow_node_fetches': int(c3['node_fetches']),
-                                   'prim_tests': int(c3['prim_tests']), 'edge_tests': int(c3['edge_tests']),
-                                   'fp64_redecisions': int(c3['fp64_redecisions'])}}
+                'two_way_counts': {'broad_node_fetches': int(c0['broad_node_fetches']),
+                                   'leaf_records': int(c0['leaf_candidates']),
+                                   'candidate_tris': int(c0['candidates']),
+                                   'narrow_node_fetches': int(c0['node_fetches']),
+                                   'prim_tests': int(c0['prim_tests']), 'edge_tests': int(c0['edge_tests']),
+                                   'fp64_redecisions': int(c0['fp64_redecisions'])}}
     log(f'[bench] roofline {roofline}')
 
     extra = {}
@@ -336,6 +398,22 @@ def main():
         ctx.set_param('graphs', 1)
         extra['plain_per_query_poses_per_s'] = round(plain, 1)
         extra['batched_over_plain'] = round(value / plain, 1)
+        # batch sizes of the dense two-way check, repeated calls replaying a captured hipGraph (device
+        # API, submit -> flags ready); 8,192 poses is one GPU's share of the batch split over 8 GPUs
+        sweep = {}
+        for nn in (1, 64, 1024, 4096, 8192, 65536):
+            dqs, dos = dq[:nn].contiguous(), torch.zeros(nn, dtype=torch.int32, device='cuda')
+            lat = []
+            for _ in range(13):
+                t = time.perf_counter()
+                ctx.check_discrete_device(sid, dqs, dos, 3)
+                ctx.sync()
+                lat.append(time.perf_counter() - t)
+            ms = float(np.median(lat[3:])) * 1e3
+            sweep[str(nn)] = {'ms': round(ms, 4), 'poses_per_s': round(nn / ms * 1e3, 1),
+                              'graph_replayed': ctx.stats()['graph_replayed']}
+        extra['batch_sweep_graph_replay'] = sweep
+        log(f'[bench] batch sweep {sweep}')
         # repeated small checks replay a captured hipGraph: device-API latency (submit -> flags ready)
         for nn in (1, 1024):
             dqs, dos = dq[:nn].contiguous(), torch.zeros(nn, dtype=torch.int32, device='cuda')
@@ -419,27 +497,32 @@ def main():
 
     cpu = None
     if not args.no_cpu:
+        # The CPU baseline: the same ray-edge method on the host (oracle/cpu_port.cpp: FP64 FK, FP32 BVH
+        # traversal with the device's error bands, the oracle's FP64 predicate inside them), std::thread
+        # workers over the cores this process may use, on the whole 65,536-pose batch per direction.
         from oracle import oracle as O
-        cores = min(os.cpu_count() or 1, 16)
-        m = O.scene_mesh(dense)
-        orb = O.Robot(rb)
-        sample = 2048
-        qs = q[:sample]
-        t = time.perf_counter()
-        ref = O.check_discrete(orb, m, qs, 3, nthreads=cores)
-        cdt = time.perf_counter() - t
-        run(3)
-        got = dout.cpu().numpy()[:sample].astype(np.uint32)
-        mism = int(np.sum(ref.astype(bool) != got.astype(bool)))
-        cpu = {'value': round(sample / cdt, 1), 'unit': 'poses/s', 'cores': cores, 'kind': 'port',
-               'sample': f'first {sample} of the 65,536 poses, dense scene, two-way, FP64 oracle with BVHs',
-               'mismatches_vs_gpu': mism}
+        cores = _cpu_cores()
+        port = O.CpuPort(rb, dense)
+        per_dir = {}
+        for d, name in ((1, 'o2r'), (2, 'r2o'), (3, 'two_way')):
+            t = time.perf_counter()
+            ref = port.check_discrete(q, d, nthreads=cores)
+            per_dir[name] = {'poses_per_s': round(n / (time.perf_counter() - t), 1)}
+            run(d)
+            got = dout.cpu().numpy().astype(np.uint32)
+            per_dir[name]['mismatches_vs_gpu'] = int(np.sum(ref.astype(bool) != got.astype(bool)))
+        cpu = {'value': per_dir['two_way']['poses_per_s'], 'unit': 'poses/s', 'cores': cores, 'kind': 'port',
+               'cpu_model': _cpu_model(),
+               'sample': f'all {n} poses of the workload, dense scene, each direction; FP32 CPU port with the '
+                         'device error bands (FP64 oracle predicate inside them), one std::thread per core',
+               'per_direction': per_dir}
         if 'c1_boxes8_o2r' in extra:
-            # the plain obstacle->robot case on the CPU too (the whole 1,024 poses, one thread per core)
+            # the plain obstacle->robot case on the CPU too (the whole 1,024 poses)
             c1s = scenes.scene_boxes8()
             c1q = samplers.uniform_poses(1024, rb.lower, rb.upper, seed=0)
+            c1_port = O.CpuPort(rb, c1s)
             t = time.perf_counter()
-            c1_ref = O.check_discrete(orb, O.scene_mesh(c1s), c1q, 1, nthreads=cores)
+            c1_ref = c1_port.check_discrete(c1q, 1, nthreads=cores)
             c1_dt = time.perf_counter() - t
             c1_sid = ctx.create_scene(c1s.verts, c1s.tris)
             c1_got = ctx.check_discrete(c1_sid, c1q, 1)

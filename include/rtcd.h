@@ -87,6 +87,10 @@ typedef struct {
     uint32_t reruns;             /* launches of the last check repeated because a candidate buffer or
                                     FP64 queue overflowed (grown and re-run, or split in halves by the
                                     allocation-free entry point) */
+    uint64_t dir_node_fetches[2]; /* counting launches, discrete: link-BVH nodes visited by the narrow
+                                    passes of each direction ([0] obstacle->robot, [1] robot->obstacle) */
+    uint64_t dir_prim_tests[2];   /* their primitive tests (robot triangles / robot edges) */
+    uint64_t dir_candidates[2];   /* their candidate triangles evaluated */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */

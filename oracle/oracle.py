@@ -11,7 +11,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, '_build', 'liboracle.so')
+_PORT_PATH = os.path.join(_HERE, '_build', 'libcpuport.so')
 _lib = None
+_port = None
 
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags='C_CONTIGUOUS')
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags='C_CONTIGUOUS')
@@ -288,3 +290,45 @@ def check_quadratic_brute(robot, mesh, wp, W):
 def check_cubic_brute(robot, mesh, wp, W):
     """(flags, per-piece flags) of the cubic sweep with every edge against every tube."""
     return _curve_brute(lib().orc_check_cubic_brute, 3, robot, mesh, wp, W)
+
+
+def port_lib():
+    """The FP32 CPU port of the discrete check (cpu_port.cpp): the timed CPU baseline."""
+    global _port
+    if _port is None:
+        if not os.path.exists(_PORT_PATH):
+            build()
+        lib()  # the port calls the oracle's FK and FP64 decisions
+        L = C.CDLL(_PORT_PATH)
+        L.cport_create.argtypes = [C.c_int, _f32p, _f32p, _f32p, _f32p, _f32p, _u32p, _i32p, _u32p, _f32p, C.c_int,
+                                   _i32p, C.c_int]
+        L.cport_create.restype = C.c_void_p
+        L.cport_destroy.argtypes = [C.c_void_p]
+        L.cport_check_discrete.argtypes = [C.c_void_p, C.c_int, C.c_int, _f32p, C.c_int, _u32p, C.c_int]
+        _port = L
+    return _port
+
+
+class CpuPort:
+    """The discrete check in FP32 with the device's error bands on the host (FP64 for tests inside a
+    band), over std::thread workers; its flags equal the oracle's."""
+
+    def __init__(self, desc, scene):
+        v, vo, t, to = desc.concat_meshes()
+        self._keep = (v, vo, t, to, scene)
+        self.dof = desc.dof
+        self.h = port_lib().cport_create(desc.dof, desc.base, desc.cos_mask, desc.sin_mask, desc.one_mask, v, vo, t,
+                                         to, np.ascontiguousarray(scene.verts, dtype=np.float32), len(scene.verts),
+                                         np.ascontiguousarray(scene.tris, dtype=np.int32), len(scene.tris))
+
+    def check_discrete(self, poses, direction=3, skip_base=True, nthreads=0):
+        poses = np.ascontiguousarray(poses, dtype=np.float32)
+        out = np.zeros(len(poses), dtype=np.uint32)
+        port_lib().cport_check_discrete(self.h, int(direction), int(bool(skip_base)), poses, len(poses), out,
+                                        int(nthreads))
+        return out
+
+    def __del__(self):
+        if getattr(self, 'h', None) and _port is not None:
+            _port.cport_destroy(self.h)
+            self.h = None

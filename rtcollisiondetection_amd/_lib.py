@@ -61,7 +61,8 @@ class Stats(C.Structure):
                 ('max_depth', C.c_uint32), ('edge_tests', C.c_uint64), ('stage_ms', C.c_double * 5),
                 ('leaf_candidates', C.c_uint64), ('cert_calls', C.c_uint64), ('broad_node_fetches', C.c_uint64),
                 ('narrow_candidates', C.c_uint64), ('graph_replayed', C.c_uint32),
-                ('r2o_records', C.c_uint64), ('reruns', C.c_uint32)]
+                ('r2o_records', C.c_uint64), ('reruns', C.c_uint32), ('dir_node_fetches', C.c_uint64 * 2),
+                ('dir_prim_tests', C.c_uint64 * 2), ('dir_candidates', C.c_uint64 * 2)]
 
 
 class Request(C.Structure):

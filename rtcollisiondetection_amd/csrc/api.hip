@@ -402,6 +402,11 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                     st->edge_tests = k.edge_tests;
                     st->broad_node_fetches = k.broad_nodes;
                     st->narrow_candidates = k.narrow_tris;
+                    for (int d = 0; d < 2; ++d) {
+                        st->dir_node_fetches[d] = k.dir_nodes[d];
+                        st->dir_prim_tests[d] = k.dir_prims[d];
+                        st->dir_candidates[d] = k.dir_tris[d];
+                    }
                 }
                 st->cert_calls = 0;
                 st->graph_replayed = &ws == &c->main && c->dgraph.last_replayed ? 1u : 0u;

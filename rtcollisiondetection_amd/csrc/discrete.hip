@@ -1064,6 +1064,9 @@ __global__ __launch_bounds__(BS) void k_narrow(DiscreteArgs a, int pass) {
         atomicAdd(&a.stats->fp64, (unsigned long long) ctr.fp64);
         atomicAdd(&a.stats->edge_tests, (unsigned long long) ctr.edges);
         atomicAdd(&a.stats->narrow_tris, (unsigned long long) ctr.tris);
+        atomicAdd(&a.stats->dir_nodes[DIR - 1], (unsigned long long) ctr.nodes);
+        atomicAdd(&a.stats->dir_prims[DIR - 1], (unsigned long long) ctr.prims);
+        atomicAdd(&a.stats->dir_tris[DIR - 1], (unsigned long long) ctr.tris);
     }
 }
 

@@ -35,6 +35,8 @@ struct SceneDev {
 
 struct Stats {
     unsigned long long node_fetches, prim_tests, fp64, candidates, edge_tests, certs, broad_nodes, narrow_tris;
+    // discrete narrow passes per direction ([0] obstacle->robot, [1] robot->obstacle)
+    unsigned long long dir_nodes[2], dir_prims[2], dir_tris[2];
 };
 
 struct DiscreteArgs {

@@ -484,3 +484,30 @@ def test_curve_pieces_are_their_own_sweeps(panda, orc_panda, deg):
     alone = fast(orc_panda, sc, parts, np.eye(deg + 1))
     assert np.array_equal(seg.reshape(-1), alone)
     assert 0 < seg.sum() < seg.size
+
+
+# ---------------------------------------------------------------- the FP32 CPU port (the CPU baseline)
+@pytest.mark.parametrize('direction', [1, 2, 3])
+def test_cpu_port_equals_oracle(panda, orc_panda, direction):
+    """The timed CPU baseline (oracle/cpu_port.cpp: FP32 with the device's bands, FP64 inside them)
+    gives the oracle's flags: the dense scene, boxes near the arm, and obstacles on the robot's own
+    surface (many tests inside the band)."""
+    cases = [(scenes.scene_dense(), samplers.uniform_poses(256, panda.lower, panda.upper, seed=81)),
+             (scenes.random_boxes(8, seed=1, center_lo=(0.2, -0.4, 0.1), center_hi=(0.6, 0.4, 0.8)),
+              samplers.uniform_poses(512, panda.lower, panda.upper, seed=82))]
+    q0 = samplers.uniform_poses(1, panda.lower, panda.upper, seed=99)[0]
+    T = O.fk(panda, q0)
+    self_meshes = []
+    for l in (3, 4, 5, 6, 7):
+        R = T[l].reshape(3, 4)
+        self_meshes.append(((panda.link_verts[l].astype(np.float64) @ R[:, :3].T + R[:, 3]).astype(np.float32),
+                            panda.link_tris[l]))
+    qs = np.repeat(q0[None], 32, axis=0)
+    qs[1:] += np.random.default_rng(3).normal(0, 1e-6, size=(31, 7)).astype(np.float32)
+    cases.append((scenes.merge(self_meshes), qs.astype(np.float32)))
+    for sc, q in cases:
+        port = O.CpuPort(panda, sc)
+        got = port.check_discrete(q, direction, nthreads=4)
+        ref = O.check_discrete(orc_panda, O.scene_mesh(sc), q, direction)
+        assert np.array_equal(got, ref), (len(sc.tris), direction)
+        assert 0 < ref.sum()
