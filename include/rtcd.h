@@ -91,6 +91,8 @@ typedef struct {
                                     passes of each direction ([0] obstacle->robot, [1] robot->obstacle) */
     uint64_t dir_prim_tests[2];   /* their primitive tests (robot triangles / robot edges) */
     uint64_t dir_candidates[2];   /* their candidate triangles evaluated */
+    uint32_t split;               /* discrete: 1 if the last check ran the split (obstacle->robot parts
+                                     between broad passes) */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */
@@ -312,8 +314,16 @@ typedef enum {
     RTCD_PARAM_TEST_UNSURE_CAP = 28,   /* FP64 re-decision queue entries (discrete and swept) */
     RTCD_PARAM_TEST_SPILL_CAP = 29,    /* broad-phase and sweep hand-off records (both halves) */
     RTCD_PARAM_TEST_NSPILL_CAP = 30,   /* narrow-phase hand-off records (both halves) */
-    RTCD_PARAM_NARROW_SHARE = 31       /* discrete narrow passes: 1 = once a wave's candidates are used up, its
+    RTCD_PARAM_NARROW_SHARE = 31,      /* discrete narrow passes: 1 = once a wave's candidates are used up, its
                                           idle lanes take pending subtrees of its busy lanes' walks */
+    RTCD_PARAM_SPLIT_MIN_TASKS = 32,   /* discrete: batches of at least this many (pose, link) tasks (default
+                                          2^14) run the split check: RTCD_PARAM_O2R_SPLIT from
+                                          RTCD_PARAM_FRONTIER_MAX_TASKS tasks up, RTCD_PARAM_O2R_SPLIT_SMALL
+                                          below, there only on scenes whose last unsplit check (of at
+                                          least 64 poses) emitted RTCD_PARAM_SPLIT_MIN_DENSITY or more
+                                          candidate triangles per task */
+    RTCD_PARAM_SPLIT_MIN_DENSITY = 33, /* default 32; 0 = split mid-size batches on every scene */
+    RTCD_PARAM_O2R_SPLIT_SMALL = 34    /* the split mask of mid-size batches (default bit 0) */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 /* Rebuilds the robot's and every scene's BVHs with other leaf sizes (1..16; 0 keeps one): the robot

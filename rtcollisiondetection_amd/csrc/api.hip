@@ -79,6 +79,14 @@ size_t discrete_bytes(const rtcd_ctx* c, uint32_t n) {
 }
 
 // Enqueue FK -> broad(count) -> scan -> broad(emit) -> narrow on ws.stream.  No host synchronisation.
+// Mid-size batches (below frontier_max_tasks tasks) split only on scenes where the broad phase emits
+// many candidates per task: there the early obstacle->robot pass flags colliding poses before the
+// later broad passes walk them; on sparse scenes its launches cost more than it saves.
+static bool split_density_ok(const rtcd_ctx* c, const SceneRes& sc) {
+    const float d = sc.cand_per_task.load(std::memory_order_relaxed);
+    return c->split_min_density == 0 || d >= (float) c->split_min_density;
+}
+
 void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const float* d_poses, uint32_t n,
                       uint32_t* d_out, size_t budget, bool events) {
     const hipStream_t s = ws.stream;
@@ -215,11 +223,15 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     // mostly walk the poses deep in clutter, which collide: the early passes flag many of them and the
     // later broad passes skip their walks.  The "broad" stage then holds the early gathers and
     // obstacle->robot passes too.
-    // Only batches of at least frontier_max_tasks tasks split: below, the extra launches cost more
+    // Only batches of at least split_min_tasks tasks split: below, the extra launches cost more
     // than the split saves (4,096 RTCD poses on its 71,746-triangle shelf: 0.237 -> 0.256 us per pose).
     // A robot->obstacle-only check splits the same way, with robot->obstacle parts.
-    const uint32_t split = !tiny && a.F > 0 && a.n_passes > 1 && tasks >= c->frontier_max_tasks
-                               ? c->o2r_split & ((1u << (a.n_passes - 1)) - 1u) : 0u;
+    const bool deep = tasks < c->frontier_max_tasks;
+    const uint32_t split = !tiny && a.F > 0 && a.n_passes > 1 && tasks >= c->split_min_tasks &&
+                                   (!deep || split_density_ok(c, sc))
+                               ? (deep ? c->o2r_split_small : c->o2r_split) & ((1u << (a.n_passes - 1)) - 1u)
+                               : 0u;
+    ws.last_split = split != 0;
     const int part_dir = dir == 2 ? 2 : 1;
     if (split && part_dir == 2) {
         a.narrow_budget = c->narrow_budget;
@@ -300,7 +312,8 @@ static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, c
             P(ws.unsure.p), ws.unsure.n, c->leaf_scene_tri, c->narrow_budget, c->narrow_budget_next, c->narrow_passes, c->o2r_budget,
             c->o2r_budget_next, c->o2r_passes, c->narrow_share,
             P(ws.scalars.p), P(ws.stats.p), P(ws.pin_status.d), P(ws.pub_flags), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
-            c->broad_passes, c->frontier_work, c->frontier_max_tasks, (uint64_t) c->robot_depth, c->narrow_bs,
+            c->broad_passes, c->frontier_work, c->frontier_max_tasks, c->split_min_tasks,
+            c->split_min_density, c->o2r_split_small, (uint64_t) split_density_ok(c, sc), (uint64_t) c->robot_depth, c->narrow_bs,
             c->tiny_batch};
 }
 
@@ -411,6 +424,12 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                 st->cert_calls = 0;
                 st->graph_replayed = &ws == &c->main && c->dgraph.last_replayed ? 1u : 0u;
                 st->reruns = (uint32_t) attempt;
+                st->split = ws.last_split ? 1u : 0u;
+            }
+            if (!ws.last_split && n >= 64) {
+                const uint64_t cands = (uint64_t) sv[4] | ((uint64_t) sv[5] << 32);
+                const uint64_t tasks = (uint64_t) n * (uint64_t) (c->robot.n_links - (c->cfg.skip_base_link ? 1 : 0));
+                if (tasks) sc.cand_per_task.store((float) ((double) cands / (double) tasks), std::memory_order_relaxed);
             }
             ws.timing_pending = false;
             // a full hand-off queue costs time, not results: grow it (within the budget) for later calls
@@ -927,6 +946,18 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
     case RTCD_PARAM_FRONTIER_MAX_TASKS:
         ctx->frontier_max_tasks = value;
         return RTCD_OK;
+    case RTCD_PARAM_SPLIT_MIN_TASKS:
+        ctx->split_min_tasks = value;
+        return RTCD_OK;
+    case RTCD_PARAM_SPLIT_MIN_DENSITY:
+        if (value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "split density must be below 2^32");
+        ctx->split_min_density = (uint32_t) value;
+        return RTCD_OK;
+    case RTCD_PARAM_O2R_SPLIT_SMALL:
+        if (value >= (1ull << kMaxPasses) || __builtin_popcountll(value) > kMaxO2rSplits)
+            return fail(RTCD_ERR_INVALID_ARG, "split must be a mask of at most 3 of the bits 0..15");
+        ctx->o2r_split_small = (uint32_t) value;
+        return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET:
     case RTCD_PARAM_BROAD_BUDGET_SMALL:
     case RTCD_PARAM_BROAD_BUDGET_NEXT:
@@ -1020,6 +1051,9 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_SWEEP_BUDGET: *value = ctx->sweep_budget; return RTCD_OK;
     case RTCD_PARAM_FRONTIER_WORK: *value = ctx->frontier_work; return RTCD_OK;
     case RTCD_PARAM_FRONTIER_MAX_TASKS: *value = ctx->frontier_max_tasks; return RTCD_OK;
+    case RTCD_PARAM_SPLIT_MIN_TASKS: *value = ctx->split_min_tasks; return RTCD_OK;
+    case RTCD_PARAM_SPLIT_MIN_DENSITY: *value = ctx->split_min_density; return RTCD_OK;
+    case RTCD_PARAM_O2R_SPLIT_SMALL: *value = ctx->o2r_split_small; return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET: *value = ctx->broad_budget; return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET_SMALL: *value = ctx->broad_budget_small; return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET_NEXT: *value = ctx->broad_budget_next; return RTCD_OK;

@@ -1,6 +1,7 @@
 // Context state shared by api.hip (discrete checks) and swept.hip (swept checks).
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -62,6 +63,9 @@ struct SceneRes {
     DevBuf<Edge> edges;
     int n_tri = 0, n_edge = 0;
     float scale = 0.f;
+    // triangles in the emitted leaf records per (pose, link) task of the last discrete check on this
+    // scene that ran without the split (-1: none yet); gates the split of mid-size batches
+    std::atomic<float> cand_per_task{-1.f};
     SceneDev dev() const {
         SceneDev s;
         s.tri_nodes = tri_bvh.nodes.p;
@@ -100,6 +104,7 @@ struct Workspace {
     // (a sub-batch of 1/k of a batch with divisor k keeps the batch's level)
     uint64_t frontier_div = 1;
     int last_frontier = 0;  // frontier entries per task of the last enqueued discrete check (0: none)
+    bool last_split = false;  // the last enqueued discrete check ran the split
     // discrete
     DevBuf<double> T64;
     DevBuf<float4> Ti, wbox;
@@ -226,6 +231,10 @@ struct rtcd_ctx {
     uint32_t sweep_budget = 32;                 // work units per sweep task before it hands off its subtrees
     uint64_t frontier_work = (uint64_t) 1 << 23;     // discrete: target (task, frontier entry) threads
     uint64_t frontier_max_tasks = (uint64_t) 1 << 17; // discrete: split batches with fewer tasks
+    // discrete: split checks on batches of at least this many tasks; below frontier_max_tasks only on
+    // scenes whose last unsplit check emitted at least split_min_density candidate triangles per task
+    uint64_t split_min_tasks = (uint64_t) 1 << 14;
+    uint32_t split_min_density = 32;
     // discrete: broad node visits before a walk hands off (large batches, small batches, later passes).
     // With the split check, 32 / 8 took the dense 65,536-pose two-way check 1.99 -> 1.91 ms (16 / 16).
     uint32_t broad_budget = 32;
@@ -287,6 +296,12 @@ struct rtcd_ctx {
     // 0: none); the later broad passes skip the poses it flagged.  After passes 1 and 3: dense 65,536-pose
     // two-way check 2.62 ms unsplit, 1.99 after pass 2 only, 1.95 after 1 and 3, 1.97 after 1, 3 and 5.
     uint32_t o2r_split = (1u << 1) | (1u << 3);
+    // the same for batches below frontier_max_tasks tasks (each task split over a deeper frontier):
+    // after pass 0.  Two-way, us per pose, no split -> split after pass 0 (4,096 / 8,192 poses): dense
+    // 50k-triangle blob 0.170 -> 0.164 / 0.105 -> 0.093; RTCD's shelf 0.187 -> 0.183 / 0.109 -> 0.104
+    // and dense shelf 0.285 -> 0.239 / 0.177 -> 0.134; its simple shelf (11 candidate triangles per
+    // task, below split_min_density) 0.125 -> 0.135 / 0.070 -> 0.076
+    uint32_t o2r_split_small = 1u;
     uint32_t tiny_batch = 4;  // discrete batches of at most this many poses run 2 broad passes, 1 narrow pass
     // starting capacities (records) of the candidate buffer, FP64 queues and hand-off queues; 0 = the
     // automatic sizes.  They exist to exercise the overflow paths (RTCD_PARAM_TEST_*_CAP).

@@ -277,3 +277,42 @@ def test_narrow_share_matches_unshared_schedules(ctx, panda, orc_panda):
                 assert _mismatch(shared[idx], O.check_discrete(orc_panda, m, q[:n][idx], d)) == 0, (n, d)
     finally:
         ctx.destroy_scene(sid)
+
+
+def test_mid_batch_split_follows_scene_density(ctx, panda, orc_panda):
+    """Batches from RTCD_PARAM_SPLIT_MIN_TASKS up to RTCD_PARAM_FRONTIER_MAX_TASKS tasks split after
+    broad pass 0 (RTCD_PARAM_O2R_SPLIT_SMALL) only on scenes whose last unsplit check emitted at least
+    RTCD_PARAM_SPLIT_MIN_DENSITY candidate triangles per task.  The first check on a scene runs
+    unsplit and measures it; on the dense scene the next one splits, on RTCD's simple shelf (its
+    benchmark poses, about 11 candidate triangles per task) it never does unless the gate is off; the
+    flags never change and equal the oracle's on a strided sample."""
+    assert ctx.get_param('split_min_tasks') == 1 << 14 and ctx.get_param('o2r_split_small') == 1
+    assert ctx.get_param('split_min_density') == 32
+    # 4,096 poses: 28,672 tasks
+    cases = ((scenes.scene_dense(), samplers.uniform_poses(4096, panda.lower, panda.upper, seed=75), True),
+             (scenes.curobo_scene('shelf_simple'),
+              samplers.benchmark_poses_discrete(8192, panda.lower, panda.upper)[:4096], False))
+    for sc, q, splits in cases:
+        sid = ctx.create_scene(sc.verts, sc.tris)
+        m = O.scene_mesh(sc)
+        try:
+            for d in (3, 1, 2):
+                first = ctx.check_discrete(sid, q, d)
+                st = ctx.stats()
+                if d == 3:  # the scene's first check
+                    assert st['split'] == 0
+                    dens = st['candidates'] / (7 * len(q))
+                    assert (dens >= 32) == splits, dens
+                for _ in range(3):  # eager, capture, replay
+                    assert np.array_equal(ctx.check_discrete(sid, q, d), first), d
+                    assert ctx.stats()['split'] == int(splits), d
+                ctx.set_param('split_min_density', 0)
+                try:
+                    assert np.array_equal(ctx.check_discrete(sid, q, d), first), d
+                    assert ctx.stats()['split'] == 1
+                finally:
+                    ctx.set_param('split_min_density', 32)
+                idx = np.arange(0, len(q), 8)
+                assert _mismatch(first[idx], O.check_discrete(orc_panda, m, q[idx], d)) == 0, d
+        finally:
+            ctx.destroy_scene(sid)
