@@ -553,11 +553,22 @@ enum { kVisitDone = 0, kVisitMore = 1, kVisitHit = 2 };
 
 // Persistent narrow lanes: a wave takes new candidates once this many lanes are idle, and checks
 // after this many node visits of its active lanes (measured on MI355X; a build may override them).
+// Robot->obstacle walks are short (2.3 link-BVH nodes per candidate on the dense scene), so its waves
+// check after every visit: dense two-way check, robot->obstacle stage 0.50 -> 0.38 ms at 65,536 poses,
+// 0.23 -> 0.13 ms at 8,192, against 4 visits (2: 0.41 / 0.17; 8: 0.64 / 0.32).  The refill threshold
+// (8, 16 or 32 lanes) changed neither direction; obstacle->robot walks (3.5 nodes) keep 4 visits
+// (2, 3 and 6 were no faster).
 #ifndef RTCD_NARROW_REFILL
 #define RTCD_NARROW_REFILL 16
 #endif
 #ifndef RTCD_NARROW_STEPS
 #define RTCD_NARROW_STEPS 4
+#endif
+#ifndef RTCD_R2O_REFILL
+#define RTCD_R2O_REFILL RTCD_NARROW_REFILL
+#endif
+#ifndef RTCD_R2O_STEPS
+#define RTCD_R2O_STEPS 1
 #endif
 
 // Walk of obstacle edge e of a scene triangle (link-frame origin P with abs. error <= delta per
@@ -840,8 +851,8 @@ __device__ __forceinline__ int resume_candidate(const DiscreteArgs& a, uint32_t 
 template <int DIR, int STACK, int BS, bool COUNT, bool SPILL>
 __device__ __forceinline__ void narrow_lanes(const DiscreteArgs& a, const uint2* list, uint32_t total, uint2* q,
                                              uint4* sh, int* st, Ctr& ctr, const NarrowSpill& hs) {
-    constexpr int kRefill = RTCD_NARROW_REFILL;  // idle lanes that make the wave refill
-    constexpr int kSteps = RTCD_NARROW_STEPS;    // node visits between refill checks
+    constexpr int kRefill = DIR == 1 ? RTCD_NARROW_REFILL : RTCD_R2O_REFILL;  // idle lanes that make the wave refill
+    constexpr int kSteps = DIR == 1 ? RTCD_NARROW_STEPS : RTCD_R2O_STEPS;     // node visits between refill checks
     const int lane = (int) (threadIdx.x & 63);
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t lanes = (uint32_t) a.leaf_lanes;
