@@ -93,6 +93,8 @@ typedef struct {
     uint64_t dir_candidates[2];   /* their candidate triangles evaluated */
     uint32_t split;               /* discrete: 1 if the last check ran the split (obstacle->robot parts
                                      between broad passes) */
+    uint32_t broad_handoffs[16];  /* counting launches, discrete: subtrees each broad pass handed off to
+                                     the next one */
 } rtcd_stats;
 
 /* Pointers the caller passes to rtcd_create. */
@@ -276,7 +278,7 @@ typedef enum {
                                           handing its pending subtrees to the next pass (>= 1) */
     RTCD_PARAM_BROAD_BUDGET_SMALL = 5, /* the same for batches below RTCD_PARAM_FRONTIER_MAX_TASKS tasks */
     RTCD_PARAM_BROAD_BUDGET_NEXT = 6,  /* the same in every broad pass after the first */
-    RTCD_PARAM_BROAD_PASSES = 7,       /* broad-phase passes, 1..16; the last never hands off */
+    RTCD_PARAM_BROAD_PASSES = 7,       /* broad-phase passes, 1..16 (default 4); the last never hands off */
     RTCD_PARAM_SWEEP_PASSES = 8,       /* sweep passes, 1..16; the last never hands off */
     RTCD_PARAM_GRAPHS = 9,             /* 1 (default): a discrete check repeated with the same shape and
                                           buffers replays a captured hipGraph; 0: always launch eagerly */
@@ -323,7 +325,12 @@ typedef enum {
                                           least 64 poses) emitted RTCD_PARAM_SPLIT_MIN_DENSITY or more
                                           candidate triangles per task */
     RTCD_PARAM_SPLIT_MIN_DENSITY = 33, /* default 32; 0 = split mid-size batches on every scene */
-    RTCD_PARAM_O2R_SPLIT_SMALL = 34    /* the split mask of mid-size batches (default bit 0) */
+    RTCD_PARAM_O2R_SPLIT_SMALL = 34,   /* the split mask of mid-size batches (default bit 0) */
+    RTCD_PARAM_BROAD_SHARE = 35,       /* discrete: 1 (default) = the last broad pass, when it follows hand-off
+                                          passes, walks the remaining subtrees with persistent lanes that
+                                          share pending subtrees within a wave; 0 = one thread each */
+    RTCD_PARAM_BROAD_PASSES_SMALL = 36 /* broad passes of batches below RTCD_PARAM_FRONTIER_MAX_TASKS tasks
+                                          (default 3) */
 } rtcd_param;
 rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value);
 /* Rebuilds the robot's and every scene's BVHs with other leaf sizes (1..16; 0 keeps one): the robot

@@ -32,7 +32,8 @@ PARAMS = {'sweep_budget': 1, 'frontier_work': 2, 'frontier_max_tasks': 3, 'broad
           'leaf_scene_edge': 19, 'serve_lane_max': 20, 'serve_lane_cus': 21, 'serve_chunk_discrete': 22,
           'serve_chunk_sweep': 23, 'serve_lane_kinds': 24, 'tiny_batch': 25,
           'o2r_split': 26, 'test_cand_cap': 27, 'test_unsure_cap': 28, 'test_spill_cap': 29, 'test_nspill_cap': 30, 'narrow_share': 31,
-          'split_min_tasks': 32, 'split_min_density': 33, 'o2r_split_small': 34}
+          'split_min_tasks': 32, 'split_min_density': 33, 'o2r_split_small': 34,
+          'broad_share': 35, 'broad_passes_small': 36}
 NOALLOC_MAX_POSES = 4096
 NOALLOC_SCRATCH_BYTES = 96 << 20
 REQ_DISCRETE, REQ_LINEAR, REQ_QUADRATIC, REQ_CUBIC = 1, 2, 3, 4
@@ -63,7 +64,8 @@ class Stats(C.Structure):
                 ('leaf_candidates', C.c_uint64), ('cert_calls', C.c_uint64), ('broad_node_fetches', C.c_uint64),
                 ('narrow_candidates', C.c_uint64), ('graph_replayed', C.c_uint32),
                 ('r2o_records', C.c_uint64), ('reruns', C.c_uint32), ('dir_node_fetches', C.c_uint64 * 2),
-                ('dir_prim_tests', C.c_uint64 * 2), ('dir_candidates', C.c_uint64 * 2), ('split', C.c_uint32)]
+                ('dir_prim_tests', C.c_uint64 * 2), ('dir_candidates', C.c_uint64 * 2), ('split', C.c_uint32),
+                ('broad_handoffs', C.c_uint32 * 16)]
 
 
 class Request(C.Structure):

@@ -205,7 +205,8 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     // passes remove (one pose: 158 -> 117 us on the 202k-triangle shelf with 2 broad passes and one
     // robot->obstacle pass)
     const bool tiny = n <= c->tiny_batch;
-    a.n_passes = tiny ? (int) std::min<uint32_t>(c->broad_passes, 2u) : (int) c->broad_passes;
+    const uint32_t passes = tasks < c->frontier_max_tasks ? c->broad_passes_small : c->broad_passes;
+    a.n_passes = tiny ? (int) std::min<uint32_t>(passes, 2u) : (int) passes;
     // starting from the all-leaves level, pass 0 only tests boxes: nothing is handed off
     if (sc.front_leaves && a.F > 0 && foff == sc.front_levels.back().first) a.n_passes = 1;
     a.nspill_cap = (uint32_t) std::min<size_t>(ws.nspill.n / 2, 0x7fffffffu);
@@ -217,6 +218,7 @@ void enqueue_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const f
     a.narrow_passes = (int) c->o2r_passes;
     a.narrow_bs = (int) c->narrow_bs;
     a.narrow_share = (int) c->narrow_share;
+    a.broad_share = (int) c->broad_share;
     const int broad_stack = stack_for(sc.tri_bvh.depth);
     // Split check: after each broad pass k with bit k of c->o2r_split set (below the last pass), an
     // obstacle->robot pass runs over the records gathered since the previous one.  The hand-off passes
@@ -310,9 +312,9 @@ static std::vector<uint64_t> graph_key(const rtcd_ctx* c, const Workspace& ws, c
             P(ws.T64.p), P(ws.Ti.p), P(ws.wbox.p), P(ws.cand.p), ws.cand.n, P(ws.flat.p), ws.flat.n,
             P(ws.seg_count.p), P(ws.seg_pre.p), P(ws.seg_base.p), c->o2r_split, P(ws.spill.p), ws.spill.n, P(ws.nspill.p), ws.nspill.n,
             P(ws.unsure.p), ws.unsure.n, c->leaf_scene_tri, c->narrow_budget, c->narrow_budget_next, c->narrow_passes, c->o2r_budget,
-            c->o2r_budget_next, c->o2r_passes, c->narrow_share,
+            c->o2r_budget_next, c->o2r_passes, c->narrow_share, c->broad_share,
             P(ws.scalars.p), P(ws.stats.p), P(ws.pin_status.d), P(ws.pub_flags), c->broad_budget, c->broad_budget_small, c->broad_budget_next,
-            c->broad_passes, c->frontier_work, c->frontier_max_tasks, c->split_min_tasks,
+            c->broad_passes, c->broad_passes_small, c->frontier_work, c->frontier_max_tasks, c->split_min_tasks,
             c->split_min_density, c->o2r_split_small, (uint64_t) split_density_ok(c, sc), (uint64_t) c->robot_depth, c->narrow_bs,
             c->tiny_batch};
 }
@@ -420,6 +422,9 @@ void finish_discrete(rtcd_ctx* c, Workspace& ws, SceneRes& sc, int dir, const fl
                         st->dir_prim_tests[d] = k.dir_prims[d];
                         st->dir_candidates[d] = k.dir_tris[d];
                     }
+                    // hand-offs per broad pass: scalars [8 .. 8 + kMaxPasses)
+                    const std::vector<uint32_t> sc8 = ws.scalars.download(8 + kMaxPasses, s);
+                    for (int p = 0; p < kMaxPasses; ++p) st->broad_handoffs[p] = sc8[8 + p];
                 }
                 st->cert_calls = 0;
                 st->graph_replayed = &ws == &c->main && c->dgraph.last_replayed ? 1u : 0u;
@@ -949,6 +954,10 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
     case RTCD_PARAM_SPLIT_MIN_TASKS:
         ctx->split_min_tasks = value;
         return RTCD_OK;
+    case RTCD_PARAM_BROAD_SHARE:
+        if (value > 1) return fail(RTCD_ERR_INVALID_ARG, "broad share must be 0 or 1");
+        ctx->broad_share = (uint32_t) value;
+        return RTCD_OK;
     case RTCD_PARAM_SPLIT_MIN_DENSITY:
         if (value > 0xffffffffull) return fail(RTCD_ERR_INVALID_ARG, "split density must be below 2^32");
         ctx->split_min_density = (uint32_t) value;
@@ -978,12 +987,14 @@ rtcd_status rtcd_set_param(rtcd_ctx* ctx, uint32_t param, uint64_t value) {
         if (!ctx->use_graphs) drop_graph(ctx);
         return RTCD_OK;
     case RTCD_PARAM_BROAD_PASSES:
+    case RTCD_PARAM_BROAD_PASSES_SMALL:
     case RTCD_PARAM_SWEEP_PASSES:
     case RTCD_PARAM_NARROW_PASSES:
     case RTCD_PARAM_O2R_PASSES: {
         if (value < 1 || value > (uint64_t) kMaxPasses) return fail(RTCD_ERR_INVALID_ARG, "passes must be 1..16");
-        uint32_t* f = param == RTCD_PARAM_BROAD_PASSES   ? &ctx->broad_passes
-                      : param == RTCD_PARAM_SWEEP_PASSES ? &ctx->sweep_passes
+        uint32_t* f = param == RTCD_PARAM_BROAD_PASSES         ? &ctx->broad_passes
+                      : param == RTCD_PARAM_BROAD_PASSES_SMALL ? &ctx->broad_passes_small
+                      : param == RTCD_PARAM_SWEEP_PASSES       ? &ctx->sweep_passes
                       : param == RTCD_PARAM_NARROW_PASSES ? &ctx->narrow_passes
                                                           : &ctx->o2r_passes;
         *f = (uint32_t) value;
@@ -1053,11 +1064,13 @@ rtcd_status rtcd_get_param(rtcd_ctx* ctx, uint32_t param, uint64_t* value) {
     case RTCD_PARAM_FRONTIER_MAX_TASKS: *value = ctx->frontier_max_tasks; return RTCD_OK;
     case RTCD_PARAM_SPLIT_MIN_TASKS: *value = ctx->split_min_tasks; return RTCD_OK;
     case RTCD_PARAM_SPLIT_MIN_DENSITY: *value = ctx->split_min_density; return RTCD_OK;
+    case RTCD_PARAM_BROAD_SHARE: *value = ctx->broad_share; return RTCD_OK;
     case RTCD_PARAM_O2R_SPLIT_SMALL: *value = ctx->o2r_split_small; return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET: *value = ctx->broad_budget; return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET_SMALL: *value = ctx->broad_budget_small; return RTCD_OK;
     case RTCD_PARAM_BROAD_BUDGET_NEXT: *value = ctx->broad_budget_next; return RTCD_OK;
     case RTCD_PARAM_BROAD_PASSES: *value = ctx->broad_passes; return RTCD_OK;
+    case RTCD_PARAM_BROAD_PASSES_SMALL: *value = ctx->broad_passes_small; return RTCD_OK;
     case RTCD_PARAM_SWEEP_PASSES: *value = ctx->sweep_passes; return RTCD_OK;
     case RTCD_PARAM_GRAPHS: *value = ctx->use_graphs ? 1 : 0; return RTCD_OK;
     case RTCD_PARAM_LEAF_ROBOT: *value = ctx->leaf_robot; return RTCD_OK;

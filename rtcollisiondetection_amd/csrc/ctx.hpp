@@ -240,7 +240,13 @@ struct rtcd_ctx {
     uint32_t broad_budget = 32;
     uint32_t broad_budget_small = 16;
     uint32_t broad_budget_next = 8;
-    uint32_t broad_passes = 12;                 // broad hand-off passes (the last never hands off)
+    // broad passes (the last never hands off; it walks what is left with lanes that share subtrees,
+    // broad_share) for large and small batches.  Two-way dense 65,536 poses: 12 passes without sharing
+    // 1.57 ms, with sharing 12 / 8 / 5 / 4 passes 1.57 / 1.54 / 1.47 / 1.45 ms (split after pass 2).
+    // Small batches (deep frontier), 3 passes: dense 8,192 poses 0.70 -> 0.59 ms, RTCD's shelf 4,096
+    // Halton poses 0.160 -> 0.134 us per pose, its dense shelf 0.191 -> 0.161.
+    uint32_t broad_passes = 4;
+    uint32_t broad_passes_small = 3;
     uint32_t sweep_passes = 12;                 // sweep hand-off passes
     uint32_t narrow_bs = 64;                    // work-group size of the discrete narrow passes
     // BVH leaf sizes (rtcd_config.leaf_size, rtcd_set_leaf_sizes): robot link BVHs, scene triangle
@@ -255,6 +261,7 @@ struct rtcd_ctx {
     uint32_t o2r_budget = 128;                  // obstacle->robot: the same three (one pass: no hand-off)
     uint32_t o2r_budget_next = 64;
     uint32_t o2r_passes = 1;
+    uint32_t broad_share = 1;   // the last broad pass walks the remaining subtrees with lanes that share them
     uint32_t narrow_share = 1;                  // narrow passes: idle lanes share the wave's running walks
     int n_spheres = 0;
     int robot_depth = 1;
@@ -293,9 +300,11 @@ struct rtcd_ctx {
     uint32_t serve_lane_max = 1024, serve_lane_cus = 0, serve_chunk_discrete = 4096, serve_chunk_sweep = 1024;
     uint32_t serve_lane_kinds = 15;  // request kinds the lane takes: bit (kind - 1)
     // bit k: an obstacle->robot pass over the records so far runs after broad pass k (at most 3 bits;
-    // 0: none); the later broad passes skip the poses it flagged.  After passes 1 and 3: dense 65,536-pose
-    // two-way check 2.62 ms unsplit, 1.99 after pass 2 only, 1.95 after 1 and 3, 1.97 after 1, 3 and 5.
-    uint32_t o2r_split = (1u << 1) | (1u << 3);
+    // 0: none); the later broad passes skip the poses it flagged.  With 12 passes and no sharing, after
+    // passes 1 and 3: dense 65,536-pose two-way check 2.62 ms unsplit, 1.99 after pass 2 only, 1.95
+    // after 1 and 3, 1.97 after 1, 3 and 5.  With 4 passes and sharing: after pass 2 1.45 ms, after 1
+    // and 2 1.48, after 1 1.53, after 0 and 1 1.54.
+    uint32_t o2r_split = 1u << 2;
     // the same for batches below frontier_max_tasks tasks (each task split over a deeper frontier):
     // after pass 0.  Two-way, us per pose, no split -> split after pass 0 (4,096 / 8,192 poses): dense
     // 50k-triangle blob 0.170 -> 0.164 / 0.105 -> 0.093; RTCD's shelf 0.187 -> 0.183 / 0.109 -> 0.104

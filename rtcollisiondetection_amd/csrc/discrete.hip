@@ -355,6 +355,171 @@ __global__ __launch_bounds__(kBS) void k_broad(DiscreteArgs a, int pass) {
     if (threadIdx.x == 0 && tris_block) atomicAdd(a.tri_candidates, (unsigned long long) tris_block);
 }
 
+// Last broad pass of a check with hand-offs (a.broad_share): the few subtrees still pending (36,084
+// of them in pass 5 of a dense 65,536-pose check, 51 in pass 9) are long walks deep inside dense
+// geometry, and one thread per subtree in hand-off passes of 8 nodes left most of the GPU idle for
+// five or six launches of about 20 us each.  Here persistent lanes walk them to the end instead: wave
+// w takes subtrees w, w + W, w + 2W, ... (W waves), so a wave holds a few of them, every lane walks
+// its own and, when the wave has no subtree left to start, idle lanes take the bottom pending subtree
+// of busy lanes (the one nearest the walk's start, so the largest) through the wave's LDS slots.
+// Every subtree is still walked once: the records are those of the one-thread walks, in another order.
+#ifndef RTCD_BROAD_STEPS
+#define RTCD_BROAD_STEPS 1
+#endif
+template <int STACK, bool COUNT>
+__global__ __launch_bounds__(kBS) void k_broad_share(DiscreteArgs a, int pass) {
+    constexpr int kSteps = RTCD_BROAD_STEPS;  // node visits between refill checks
+    __shared__ int stk[STACK * kBS];
+    __shared__ uint2 bq[kBS / 64][128];
+    __shared__ uint2 bsh[kBS / 64][64];
+    __shared__ uint32_t wcnt[kBS / 64];
+    __shared__ uint32_t tris_block;
+    int* st = stk + threadIdx.x;
+    const int wv = (int) (threadIdx.x >> 6), lane = (int) (threadIdx.x & 63);
+    const uint64_t below = (1ull << lane) - 1ull;
+    if (threadIdx.x < kBS / 64) wcnt[threadIdx.x] = 0;
+    if (threadIdx.x == 0) tris_block = 0;
+    __syncthreads();
+    uint32_t* my_wcnt = wcnt + wv;
+    uint2* q = bq[wv];
+    uint2* sh = bsh[wv];
+    // segments as in k_broad's later passes
+    const size_t seg = (blockIdx.x + (size_t) pass * 167u) % kSeg;
+    uint32_t* seg_count = a.seg_count + seg * kSegStride;
+    uint2* seg_cand = a.cand + seg * a.seg_cap;
+    const uint2* in_q = (pass - 1) & 1 ? a.spill[1] : a.spill[0];
+    const uint32_t n_items = min(a.spill_count[pass - 1], a.spill_cap);
+    const uint32_t n_waves = gridDim.x * (kBS / 64), wave = blockIdx.x * (kBS / 64) + (uint32_t) wv;
+    uint32_t k_next = 0;  // the wave's next subtree is wave + k_next * n_waves
+    const uint32_t NL = (uint32_t) a.n_links;
+    const float pad = a.broad_pad;
+    uint32_t ntris = 0, c_nodes = 0;
+    auto emit = [&](uint32_t k, int start, int n, uint32_t o) {
+        if (o < a.seg_cap) seg_cand[o] = make_uint2(k, pack_leaf(start, n));
+        else atomicOr(a.overflow, 1u);
+        ntris += (uint32_t) n;
+    };
+    // lane state: the walk, its pending subtrees st[sb .. sp) and the boxes of its (pose, link) task
+    bool act = false;
+    uint32_t key = 0;
+    int ni = 0, sp = 0, sb = 0;
+    float4 wlo = {}, whi = {}, r0 = {}, r1 = {}, r2 = {};
+    float3 lc = {}, lh = {};
+    auto start = [&](uint2 e) {
+        key = e.x;
+        ni = (int) e.y;
+        wlo = a.wbox[(size_t) key * 2];
+        whi = a.wbox[(size_t) key * 2 + 1];
+        r0 = a.Ti[(size_t) key * 3];
+        r1 = a.Ti[(size_t) key * 3 + 1];
+        r2 = a.Ti[(size_t) key * 3 + 2];
+        const LinkInfo& L = a.robot->link[key % NL];
+        lc = make_float3(L.c[0], L.c[1], L.c[2]);
+        lh = make_float3(0.5f * (L.hi[0] - L.lo[0]), 0.5f * (L.hi[1] - L.lo[1]), 0.5f * (L.hi[2] - L.lo[2]));
+        sp = sb = 0;
+        act = true;
+    };
+    int qn = 0;            // queued subtrees (wave-uniform)
+    bool drained = false;  // the wave's subtrees are all taken (wave-uniform)
+    for (;;) {
+        const uint64_t idle = __ballot(!act);
+        const int n_idle = __popcll(idle);
+        if (n_idle > 0) {
+            while (qn < n_idle && !drained) {
+                const uint64_t i = (uint64_t) wave + ((uint64_t) k_next + (uint64_t) lane) * n_waves;
+                if ((uint64_t) wave + (uint64_t) k_next * n_waves >= n_items) {
+                    drained = true;
+                    break;
+                }
+                k_next += 64;
+                bool live = false;
+                uint2 e = make_uint2(0u, 0u);
+                if (i < n_items) {
+                    e = in_q[i];
+                    // a slot of a hand-off that did not fit, or a pose an obstacle->robot part flagged
+                    live = e.x != 0xffffffffu && !load_flag(a.flags + e.x / NL);
+                }
+                const uint64_t m = __ballot(live);
+                if (live) q[qn + __popcll(m & below)] = e;
+                qn += __popcll(m);
+            }
+            __builtin_amdgcn_wave_barrier();
+            // idle lane number r (in lane order) takes queue entry qn - 1 - r
+            const int r = __popcll(idle & below);
+            const int take = min(n_idle, qn);
+            const bool mine = !act && r < take;
+            uint2 e = make_uint2(0u, 0u);
+            if (mine) e = q[qn - 1 - r];
+            qn -= take;
+            __builtin_amdgcn_wave_barrier();
+            if (mine) start(e);
+            if (qn == 0 && drained) {
+                // no subtree left to start: the idle lanes take the bottom pending subtree of busy lanes
+                const uint64_t idle2 = __ballot(!act), donors = __ballot(act && sp > sb);
+                const int n_give = min(__popcll(idle2), __popcll(donors));
+                if (n_give > 0) {
+                    const int dr = __popcll(donors & below);
+                    if (act && sp > sb && dr < n_give) {
+                        sh[dr] = make_uint2(key, (uint32_t) st[sb * kBS]);
+                        if (++sb == sp) sb = sp = 0;
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                    const int ir = __popcll(idle2 & below);
+                    const bool taker = !act && ir < n_give;
+                    uint2 g = make_uint2(0u, 0u);
+                    if (taker) g = sh[ir];
+                    __builtin_amdgcn_wave_barrier();
+                    if (taker) start(g);
+                }
+                if (__ballot(act) == 0) break;
+            }
+        }
+        for (int s = 0; s < kSteps && act; ++s) {
+            const BNode nd = a.sc.tri_nodes[ni];
+            if (COUNT) c_nodes++;
+            bool h0 = nd.c0 >= 0 && broad_test(nd.lo0x, nd.lo0y, nd.lo0z, nd.hi0x, nd.hi0y, nd.hi0z, wlo, whi, r0, r1,
+                                               r2, lc, lh, pad);
+            bool h1 = nd.c1 >= 0 && broad_test(nd.lo1x, nd.lo1y, nd.lo1z, nd.hi1x, nd.hi1y, nd.hi1z, wlo, whi, r0, r1,
+                                               r2, lc, lh, pad);
+            const bool e0 = h0 && nd.n0 > 0, e1 = h1 && nd.n1 > 0;
+            if (e0 || e1) {
+                uint32_t o = wave_alloc(my_wcnt, seg_count, (uint32_t) e0 + (uint32_t) e1);
+                if (e0) {
+                    emit(key, nd.c0, nd.n0, o++);
+                    h0 = false;
+                }
+                if (e1) {
+                    emit(key, nd.c1, nd.n1, o);
+                    h1 = false;
+                }
+            }
+            if (h0 && h1) {
+                if (sp >= STACK) {
+                    atomicOr(a.overflow, 2u);
+                    act = false;
+                    break;
+                }
+                st[sp * kBS] = nd.c1;
+                ++sp;
+                ni = nd.c0;
+            } else if (h0) {
+                ni = nd.c0;
+            } else if (h1) {
+                ni = nd.c1;
+            } else if (sp == sb) {
+                act = false;
+            } else {
+                ni = st[--sp * kBS];
+                if (sp == sb) sb = sp = 0;
+            }
+        }
+    }
+    if (ntris) atomicAdd(&tris_block, ntris);
+    if (COUNT) atomicAdd(&a.stats->broad_nodes, (unsigned long long) c_nodes);
+    __syncthreads();
+    if (threadIdx.x == 0 && tris_block) atomicAdd(a.tri_candidates, (unsigned long long) tris_block);
+}
+
 // ------------------------------------------------------------------------------------------ gather
 // Exclusive prefix of the segment counts (each clamped to seg_cap): one block, seg_pre[kSeg] = total.
 // Split checks gather in parts: seg_base[b] holds the records of segment b gathered so far and
@@ -1170,8 +1335,16 @@ static void broad_t(const DiscreteArgs& a, bool count, hipStream_t s, int p0, in
     const unsigned nb1 = std::min<unsigned>((unsigned) device_cus() * 8u, kSeg);
     for (int pass = p0; pass < (p1 < 0 ? a.n_passes : p1); ++pass) {
         const unsigned nb = pass == 0 ? nb0 : nb1;
-        if (count) hipLaunchKernelGGL((k_broad<STACK, true>), dim3(nb), dim3(kBS), 0, s, a, pass);
-        else hipLaunchKernelGGL((k_broad<STACK, false>), dim3(nb), dim3(kBS), 0, s, a, pass);
+        if (a.broad_share && pass > 0 && pass == a.n_passes - 1) {
+            // persistent: the blocks the CUs hold at once (LDS: 4 per CU)
+            const unsigned nbs = std::min<unsigned>((unsigned) device_cus() * 4u, kSeg);
+            if (count) hipLaunchKernelGGL((k_broad_share<STACK, true>), dim3(nbs), dim3(kBS), 0, s, a, pass);
+            else hipLaunchKernelGGL((k_broad_share<STACK, false>), dim3(nbs), dim3(kBS), 0, s, a, pass);
+        } else if (count) {
+            hipLaunchKernelGGL((k_broad<STACK, true>), dim3(nb), dim3(kBS), 0, s, a, pass);
+        } else {
+            hipLaunchKernelGGL((k_broad<STACK, false>), dim3(nb), dim3(kBS), 0, s, a, pass);
+        }
         RTCD_LAUNCH_CHECK();
     }
 }

@@ -83,6 +83,8 @@ struct DiscreteArgs {
     int narrow_passes;     // passes per direction (<= kMaxPasses); the last never hands off
     int narrow_share;      // 1: once a wave's candidates are used up, its idle lanes take pending
                            // subtrees of its busy lanes' walks
+    int broad_share;       // 1: the last broad pass (after at least one hand-off pass) runs
+                           // k_broad_share: persistent lanes that share pending subtrees
 };
 // Most hand-off passes of one broad phase or sweep: the first walks every task, each later one the
 // subtrees handed off by the previous pass.

@@ -127,13 +127,15 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
     struct Restore {
         rtcd_ctx* c;
         bool counting;
-        uint32_t leaf_r, leaf_t, leaf_e, nbs, passes, bud, bud_small, bud_next, sw_bud, sw_passes, n_passes, n_bud, o_passes;
+        uint32_t leaf_r, leaf_t, leaf_e, nbs, passes, passes_small, bud, bud_small, bud_next, sw_bud, sw_passes, n_passes,
+            n_bud, o_passes;
         bool done = false;
         ~Restore() {
             c->counting = counting;
             if (done) return;
             c->narrow_bs = nbs;
             c->broad_passes = passes;
+            c->broad_passes_small = passes_small;
             c->broad_budget = bud;
             c->broad_budget_small = bud_small;
             c->broad_budget_next = bud_next;
@@ -159,6 +161,7 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
               ctx->leaf_scene_edge,
               ctx->narrow_bs,
               ctx->broad_passes,
+              ctx->broad_passes_small,
               ctx->broad_budget,
               ctx->broad_budget_small,
               ctx->broad_budget_next,
@@ -192,9 +195,11 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
         R.out.reserve(r->n);
         const double before = R.time();
         double best = before;
+        bool small_batch = false;
         if (r->kind == RTCD_REQ_DISCRETE) {
             const size_t tasks = (size_t) r->n * (ctx->robot.n_links - (ctx->cfg.skip_base_link ? 1 : 0));
             const bool small = tasks < ctx->frontier_max_tasks;
+            small_batch = small;
             if (flags & RTCD_TUNE_LEAF_SIZE) {
                 best = descend_leaf(R, &ctx->leaf_scene_tri, best);
                 best = descend_leaf(R, &ctx->leaf_robot, best);
@@ -204,7 +209,7 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
                 {&ctx->narrow_passes, {1, 4, 6, 8}},
                 {&ctx->narrow_budget, {8, 16, 32, 64}},
                 {&ctx->o2r_passes, {1, 2, 4}},
-                {&ctx->broad_passes, {6, 8, 12, 16}},
+                small ? Knob{&ctx->broad_passes_small, {2, 3, 4, 6}} : Knob{&ctx->broad_passes, {3, 4, 6, 8, 12}},
                 {&ctx->broad_budget_next, {8, 16, 32}},
                 small ? Knob{&ctx->broad_budget_small, {8, 16, 32, 64}} : Knob{&ctx->broad_budget, {16, 32, 64, 128}},
             };
@@ -221,7 +226,7 @@ rtcd_status rtcd_autotune(rtcd_ctx* ctx, const rtcd_request* r, uint32_t flags, 
         rep->leaf_scene_tri = ctx->leaf_scene_tri;
         rep->leaf_scene_edge = ctx->leaf_scene_edge;
         rep->narrow_block = ctx->narrow_bs;
-        rep->broad_passes = ctx->broad_passes;
+        rep->broad_passes = small_batch ? ctx->broad_passes_small : ctx->broad_passes;
         rep->broad_budget = ctx->broad_budget;
         rep->broad_budget_small = ctx->broad_budget_small;
         rep->broad_budget_next = ctx->broad_budget_next;

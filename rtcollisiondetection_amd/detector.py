@@ -183,7 +183,7 @@ class Context:
         check(lib().rtcd_get_stats(self.h, C.byref(st)))
         d = {k: getattr(st, k) for k, _ in st._fields_}
         d['stage_ms'] = list(st.stage_ms)
-        for k in ('dir_node_fetches', 'dir_prim_tests', 'dir_candidates'):
+        for k in ('dir_node_fetches', 'dir_prim_tests', 'dir_candidates', 'broad_handoffs'):
             d[k] = list(getattr(st, k))
         return d
 

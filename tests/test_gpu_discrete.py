@@ -226,9 +226,10 @@ def test_split_checks_match_oracle(ctx, panda, orc_panda):
     q = samplers.uniform_poses(1024, panda.lower, panda.upper, seed=21)
     m = O.scene_mesh(sc)
     refs = {d: O.check_discrete(orc_panda, m, q, d) for d in (1, 2, 3)}
-    assert ctx.get_param('o2r_split') == 0b1010
+    assert ctx.get_param('o2r_split') == 0b100 and ctx.get_param('broad_passes') == 4
     old = ctx.get_param('frontier_max_tasks')
     ctx.set_param('frontier_max_tasks', 0)  # one task per (pose, link) from the root's children
+    ctx.set_param('broad_passes', 12)  # room for split points up to pass 10
     try:
         for mask in (0, 1, 4, 0b101, 0b1010100, 1 << 10):
             ctx.set_param('o2r_split', mask)
@@ -246,7 +247,8 @@ def test_split_checks_match_oracle(ctx, panda, orc_panda):
         with pytest.raises(RtcdError):
             ctx.set_param('o2r_split', 1 << 16)
     finally:
-        ctx.set_param('o2r_split', 0b1010)
+        ctx.set_param('o2r_split', 0b100)
+        ctx.set_param('broad_passes', 4)
         ctx.set_param('frontier_max_tasks', old)
     ctx.destroy_scene(sid)
 
@@ -316,3 +318,55 @@ def test_mid_batch_split_follows_scene_density(ctx, panda, orc_panda):
                 assert _mismatch(first[idx], O.check_discrete(orc_panda, m, q[idx], d)) == 0, d
         finally:
             ctx.destroy_scene(sid)
+
+
+def test_broad_share_matches_one_thread_walks(ctx, panda, orc_panda):
+    """The last broad pass with lanes that share pending subtrees (RTCD_PARAM_BROAD_SHARE) walks every
+    subtree once, whichever lane walks it: with the split check off it visits the same scene nodes
+    and emits the same leaf records and candidate triangles as one thread per subtree, under the
+    default budgets and under budgets of 1-2 nodes with 4 and 16 passes (nearly every walk handed
+    off); flags agree with the split check on too, at 1,024 (frontier split), 4,096 (mid-size split)
+    and 20,000 poses (split check), and the oracle agrees on a strided sample."""
+    sc = scenes.scene_dense()
+    sid = ctx.create_scene(sc.verts, sc.tris)
+    m = O.scene_mesh(sc)
+    q = samplers.uniform_poses(20000, panda.lower, panda.upper, seed=73)
+    keys = ('broad_node_fetches', 'leaf_candidates', 'candidates')
+    names = ('broad_share', 'o2r_split', 'o2r_split_small', 'broad_budget', 'broad_budget_small',
+             'broad_budget_next', 'broad_passes', 'broad_passes_small')
+    saved = {k: ctx.get_param(k) for k in names}
+    assert saved['broad_share'] == 1
+
+    def run(n, d, share):
+        ctx.set_param('broad_share', share)
+        ctx.set_counting(True)
+        try:
+            f = ctx.check_discrete(sid, q[:n], d)
+            st = ctx.stats()
+            return f, {k: st[k] for k in keys}, st['broad_handoffs']
+        finally:
+            ctx.set_counting(False)
+
+    try:
+        for budgets in ((32, 16, 8, 12), (2, 2, 1, 16), (2, 2, 1, 4)):
+            for k, v in zip(('broad_budget', 'broad_budget_small', 'broad_budget_next', 'broad_passes'), budgets):
+                ctx.set_param(k, v)
+            ctx.set_param('broad_passes_small', budgets[3])
+            for n in (1024, 4096, 20000):
+                for split in (False, True):
+                    ctx.set_param('o2r_split', saved['o2r_split'] if split else 0)
+                    ctx.set_param('o2r_split_small', saved['o2r_split_small'] if split else 0)
+                    for d in (1, 2, 3):
+                        f1, s1, h1 = run(n, d, 1)
+                        f0, s0, h0 = run(n, d, 0)
+                        assert np.array_equal(f1, f0), (budgets, n, split, d)
+                        if not split:
+                            assert s1 == s0, (budgets, n, d, s1, s0)
+                            assert h1 == h0 and sum(h1) > 0, (budgets, n, d, h1)
+                        if budgets[0] == 32 and split:
+                            idx = np.arange(0, n, max(1, n // 500))
+                            assert _mismatch(f1[idx], O.check_discrete(orc_panda, m, q[:n][idx], d)) == 0, (n, d)
+    finally:
+        for k, v in saved.items():
+            ctx.set_param(k, v)
+        ctx.destroy_scene(sid)

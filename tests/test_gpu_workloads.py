@@ -1,8 +1,9 @@
 """The two workloads the performance figures are quoted on, run as tests at their full size:
 
 - the dense two-way discrete check of 65,536 poses (shelf lattice + 50,000-triangle blob) with the
-  default schedule of large batches (split check after broad passes 1 and 3, budgets 32 / 8, hand-off
-  queues), through the device entry point, captured and replayed;
+  default schedule of large batches (4 broad passes, budgets 32 / 8, hand-off queues, split check after
+  pass 2, last pass with lanes that share subtrees), through the device entry point, captured and
+  replayed;
 - the serving stream: Poisson arrivals at 2,000 requests/s for one second, 50/30/20 discrete / linear /
   quadratic, sizes log-uniform 1..8,192, three resident scenes (8 boxes, shelf, shelf + blob).
 
@@ -28,7 +29,8 @@ def test_dense_65536_default_schedule(panda, orc_panda):
     try:
         ctx.load_robot(panda)
         sid = ctx.create_scene(sc.verts, sc.tris)
-        assert ctx.get_param('frontier_max_tasks') == 1 << 17 and ctx.get_param('o2r_split') == 0b1010
+        assert ctx.get_param('frontier_max_tasks') == 1 << 17 and ctx.get_param('o2r_split') == 0b100
+        assert ctx.get_param('broad_passes') == 4 and ctx.get_param('broad_share') == 1
         dq = torch.from_numpy(q).cuda()
         outs = {}
         for d in (1, 2, 3):
